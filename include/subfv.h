@@ -1,0 +1,179 @@
+/*
+ * subfv_amd C-ABI: the explicit residual path of subfv (subface fluxes and
+ * nodal solvers, node->cell gather, limited gradients, stage update) and its
+ * BCSR / block-Jacobi / GMRES routines on one MI355X (gfx950).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success or a
+ * SUBFV_E_* code; subfv_error() gives the message.  Arrays crossing the
+ * boundary use the reference's Fortran layouts: sol(5,C) is C blocks of 5
+ * doubles, grad(3,5,C) is C blocks of 15 (d fastest), diag(5,5,C) is C
+ * column-major 5x5 blocks, vp(3,V) is V blocks of 3.  Vertex, element and
+ * sub-face ids are those of the reference's build_mesh (1-based in exported
+ * integer arrays).  A context is not thread-safe; use one per host thread.
+ *
+ * Reference interfaces replaced (paths relative to the subfv source tree):
+ *   subfv_mesh_build            read_mesh_msh_2 (src/mesh/mesh_reading_module.F90:989-1324)
+ *                               + build_mesh (src/mesh/mesh_connectivity_module.F90:12-59)
+ *                               + compute_geometry_mesh (src/mesh/mesh_geometry_module.F90:16-57)
+ *   subfv_set_config            read_input_parameters / init_flags
+ *                               (src/ns/ns_global_data_module.F90:170-330), namelist subset
+ *   subfv_state_put/get         the host <-> device copies of sol(5, n_elems)
+ *   subfv_compute_euler_flux    compute_euler_flux (src/ns/ns_euler_module.F90:151-339)
+ *   subfv_compute_cell_grad     compute_cell_grad_from_nodal_grad (src/ns/ns_euler_recon_module.F90:8-59)
+ *   subfv_flux_limiting         flux_limiting (src/ns/ns_euler_recon_module.F90:560-601)
+ *   subfv_add_timedisc_and_solve add_timedisc_and_solve (src/ns/ns_euler_module.F90:521-609)
+ *   subfv_compute_rhs_ns        compute_rhs_ns (src/ns/ns_main.F90:130-179, without diffusion)
+ *   subfv_run                   the time loop of src/ns/ns_main.F90:83-107
+ *   subfv_bcsr_*                bcsr_matrix_type, bcsr_matmul, bcsr_diag, bcsr_gmres_diag
+ *                               (src/linear_algebra/sparse_bcsr_linear_module.F90:6-29,1414-1436,1592-1691)
+ */
+#ifndef SUBFV_H
+#define SUBFV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SUBFV_OK = 0,
+  SUBFV_E_INVALID = 1,      /* bad argument or call order */
+  SUBFV_E_DEVICE = 2,       /* HIP runtime failure */
+  SUBFV_E_BAD_OMEGA = 3,    /* add_timedisc_and_solve "BAD OMEGA" stop (ns_euler_module.F90:595-598) */
+  SUBFV_E_NEG_STAR = 4,     /* multi_point "Negative specific volume" stop (ns_euler_rs_module.F90:257-266) */
+  SUBFV_E_MESH = 5,         /* check_mesh failure (mesh_geometry_module.F90:403-408) */
+  SUBFV_E_BAD_BC = 6,       /* unknown boundary type (ns_euler_recon_module.F90:404-406) */
+  SUBFV_E_NOT_CONVERGED = 7 /* GMRES returned conv = .false. (soft) */
+};
+
+/* scheme ids of ns_global_data_module.F90:63-72 */
+enum {
+  SUBFV_SCHEME_MULTI_POINT = 1,
+  SUBFV_SCHEME_MULTI_POINT_ISO = 2,
+  SUBFV_SCHEME_THREE_WAVE = 3,
+  SUBFV_SCHEME_TWO_WAVE = 4,
+  SUBFV_SCHEME_MODIFIED_THREE_WAVE = 5,
+  SUBFV_SCHEME_MULTI_POINT_PRESSURE = 6,
+  SUBFV_SCHEME_MULTI_POINT_PRESSURE_PH = 8
+};
+
+/* Euler BC ids of ns_global_data_module.F90:50-58 (potential-flow BCs excluded) */
+enum {
+  SUBFV_BC_WALL = 1,
+  SUBFV_BC_ADHERENCE_WALL = 2,
+  SUBFV_BC_FREESTREAM = 3,
+  SUBFV_BC_OUTFLOWSUPERSONIC = 4,
+  SUBFV_BC_INFLOW_POND = 5,
+  SUBFV_BC_INOUT_DOUBLE_MACH = 6,
+  SUBFV_BC_DOUBLE_MACH_BOTTOM = 7
+};
+
+#define SUBFV_MNBC 10
+
+/* The &INPUT_PARAM entries the path reads (ns_global_data_module.F90:177-213). */
+typedef struct subfv_config {
+  int32_t scheme_id;
+  int32_t bc_style;
+  int32_t boundary_2d;
+  int32_t exclude_bound_vert;
+  int32_t local_time_step;
+  int32_t second_order;
+  int32_t method;
+  int32_t n_bc;
+  int32_t bc_euler_id[SUBFV_MNBC]; /* bc_euler_id(1..n_bc) */
+  double bc_val[SUBFV_MNBC][5];    /* bc_val(:, 1..n_bc), primitive */
+  double cfl;
+  double tmax;
+} subfv_config;
+
+typedef struct subfv_ctx subfv_ctx;
+typedef struct subfv_host_mesh subfv_host_mesh;
+
+int subfv_create(subfv_ctx **ctx, int device);
+void subfv_destroy(subfv_ctx *ctx);
+const char *subfv_error(const subfv_ctx *ctx);
+
+/* Mesh in msh 2.2 form: coord(3, n_vert) in file order, gmsh element kinds
+ * (4 tet, 5 hex, 6 prism, 7 pyramid) with 1-based vertex ids, optional tagged
+ * boundary faces (tag = BC index, 0 = default wall).  Builds the subface
+ * connectivity, uploads it and computes the geometry on the device. */
+int subfv_mesh_build(subfv_ctx *ctx, int32_t n_vert, const double *coord, int32_t n_elems,
+                     const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
+                     const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag,
+                     int32_t boundary_2d);
+/* "n_vert", "n_elems", "n_faces", "n_sub_faces", "n_sub_elems", "n_max_sub_faces" */
+int64_t subfv_mesh_count(const subfv_ctx *ctx, const char *what);
+/* Connectivity in the reference numbering (same names as the mesh_type
+ * components, e.g. "vert_sub_face", "sub_face_left_elem"); returns the number
+ * of values or -1. */
+int64_t subfv_mesh_get_i32(subfv_ctx *ctx, const char *name, int32_t *out, int64_t cap);
+/* Geometry computed on the device: "elem_coord", "elem_volume",
+ * "sub_face_norm", "sub_face_area", "face_coord", ... */
+int64_t subfv_mesh_get_f64(subfv_ctx *ctx, const char *name, double *out, int64_t cap);
+
+/* Host-only connectivity build (no device needed). */
+int subfv_host_mesh_build(subfv_host_mesh **out, int32_t n_vert, const double *coord, int32_t n_elems,
+                          const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
+                          const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag);
+int64_t subfv_host_mesh_get_i32(subfv_host_mesh *m, const char *name, int32_t *out, int64_t cap);
+void subfv_host_mesh_free(subfv_host_mesh *m);
+
+int subfv_set_config(subfv_ctx *ctx, const subfv_config *cfg);
+/* physical time t used by time-dependent boundaries and the tmax clamp */
+int subfv_set_time(subfv_ctx *ctx, double t);
+
+int subfv_state_put(subfv_ctx *ctx, const double *sol);
+int subfv_state_get(subfv_ctx *ctx, double *sol);
+
+/* Device arrays in reference layout, by name: "sol", "rhs", "rhs_o2",
+ * "sum_lambda", "sum_lambda_o2", "cell_grad", "nodal_grad", "vp", "diag". */
+int64_t subfv_get_f64(subfv_ctx *ctx, const char *name, double *out, int64_t cap);
+int64_t subfv_put_f64(subfv_ctx *ctx, const char *name, const double *in, int64_t n);
+
+/* compute_euler_flux on the device state.  second_order = 0 writes "rhs"
+ * and "sum_lambda"; second_order = 1 reconstructs with "cell_grad" and writes
+ * "rhs_o2" and "sum_lambda_o2".  want_vp also writes the nodal velocity. */
+int subfv_compute_euler_flux(subfv_ctx *ctx, int32_t second_order, int32_t want_vp);
+/* compute_cell_grad_from_nodal_grad: writes "nodal_grad" and "cell_grad" */
+int subfv_compute_cell_grad(subfv_ctx *ctx, int32_t method);
+/* flux_limiting(sol, rhs, rhs_o2, sum_lambda, cfl): updates "rhs" */
+int subfv_flux_limiting(subfv_ctx *ctx, double cfl);
+/* add_timedisc_and_solve(sol, rhs, diag, sum_lambda, ..., a1, a2, cfl, t, dt)
+ * on the device state; *dt receives the value the reference leaves in dt,
+ * *omega the positivity factor. */
+int subfv_add_timedisc_and_solve(subfv_ctx *ctx, double a1, double a2, double cfl, double t, double *dt,
+                                 double *omega);
+/* compute_rhs_ns without diffusion */
+int subfv_compute_rhs_ns(subfv_ctx *ctx);
+
+/* The time loop on device-resident state without host synchronisation.
+ * time_scheme 0: the reference loop (forward Euler, or its two-stage scheme
+ * when second_order is set); 1: SSP-RK3 with one dt per step.  Runs n_steps
+ * steps or until t >= tmax; *t and *dt receive the final values. */
+int subfv_run(subfv_ctx *ctx, int32_t n_steps, int32_t time_scheme, double *t, double *dt);
+
+/* ------------------------------------------------------------------ BCSR */
+typedef struct subfv_bcsr subfv_bcsr;
+/* bcsr_matrix_type: m block rows, b x b blocks, 1-based row(m+1), col(nnzb),
+ * v(b,b,nnzb) column-major blocks.  Copies to the device. */
+int subfv_bcsr_create(subfv_ctx *ctx, subfv_bcsr **A, int32_t m, int32_t b, int32_t nnzb, const int32_t *row,
+                      const int32_t *col, const double *v);
+void subfv_bcsr_destroy(subfv_bcsr *A);
+/* y = A x (bcsr_matmul), host vectors of length b*m */
+int subfv_bcsr_matmul(subfv_bcsr *A, const double *x, double *y);
+/* x = D^-1 rhs, block-Jacobi via lu_solve_noswp (bcsr_diag) */
+int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs);
+/* bcsr_gmres_diag(A, x, b, tol, m, conv, ...): restarted right-preconditioned
+ * GMRES(restart) from x = 0 with absolute tolerance tol on ||b - Ax||_2.
+ * precision 0: FP64 throughout; 1: FP32 SpMV and block-Jacobi inside each
+ * cycle with FP64 orthogonalisation, residuals and restarts (iterative
+ * refinement).  stats[0] = restarts, stats[1] = final ||b - Ax||_2,
+ * stats[2] = SpMV count. Returns SUBFV_E_NOT_CONVERGED (soft) if not conv. */
+int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
+                          int32_t precision, int32_t *conv, double *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

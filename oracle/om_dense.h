@@ -1,0 +1,173 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the oracle (see om_internal.h).
+ *
+ * Small dense solves used on the path, restated from published algorithms:
+ *  - om_lu_solve: LU with partial pivoting as reference LAPACK dgetrf2 +
+ *    dgetrs compute it (pivot = first largest |a|, L column scaled by the
+ *    reciprocal of the pivot, sequential rank-1 updates; dtrsm-ordered
+ *    triangular solves).  Replaces lu_solve_inplace_lapack,
+ *    src/linear_algebra/linear_solver_module.F90:643-657.
+ *  - om_pinv3: pseudo-inverse V * S^+ * U^T with singular values below
+ *    n*1e-12*max(S) dropped (linear_solver_module.F90:683-715), where the SVD
+ *    is computed by one-sided (Hestenes) Jacobi rotations instead of dgesvd.
+ *  - om_lu_solve_noswp: linear_solver_module.F90:413-445 verbatim order.
+ */
+#ifndef OM_DENSE_H
+#define OM_DENSE_H
+#include <math.h>
+#include <string.h>
+
+/* A is column-major n x n (A[i + n*j]) and is overwritten by its factors. */
+static inline void om_lu_solve(int n, double *A, const double *b, double *x) {
+  int ipiv[8];
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double amax = fabs(A[k + n * k]);
+    for (int i = k + 1; i < n; i++)
+      if (fabs(A[i + n * k]) > amax) {
+        amax = fabs(A[i + n * k]);
+        p = i;
+      }
+    ipiv[k] = p;
+    if (A[p + n * k] != 0.0) {
+      if (p != k)
+        for (int j = 0; j < n; j++) {
+          double t = A[k + n * j];
+          A[k + n * j] = A[p + n * j];
+          A[p + n * j] = t;
+        }
+      double piv = A[k + n * k];
+      if (fabs(piv) >= 2.2250738585072014e-308) {
+        double r = 1.0 / piv;
+        for (int i = k + 1; i < n; i++) A[i + n * k] = A[i + n * k] * r;
+      } else {
+        for (int i = k + 1; i < n; i++) A[i + n * k] = A[i + n * k] / piv;
+      }
+    }
+    for (int j = k + 1; j < n; j++)
+      for (int i = k + 1; i < n; i++) A[i + n * j] = A[i + n * j] - A[i + n * k] * A[k + n * j];
+  }
+  double w[8];
+  for (int i = 0; i < n; i++) w[i] = b[i];
+  for (int k = 0; k < n; k++)
+    if (ipiv[k] != k) {
+      double t = w[k];
+      w[k] = w[ipiv[k]];
+      w[ipiv[k]] = t;
+    }
+  for (int k = 0; k < n; k++)
+    if (w[k] != 0.0)
+      for (int i = k + 1; i < n; i++) w[i] = w[i] - w[k] * A[i + n * k];
+  for (int k = n - 1; k >= 0; k--)
+    if (w[k] != 0.0) {
+      w[k] = w[k] / A[k + n * k];
+      for (int i = 0; i < k; i++) w[i] = w[i] - w[k] * A[i + n * k];
+    }
+  for (int i = 0; i < n; i++) x[i] = w[i];
+}
+
+/* One-sided Jacobi SVD of a column-major 3x3 matrix: a = U diag(s) V^T with
+ * s sorted in decreasing order. */
+static inline void om_svd3(const double *a_in, double *s, double *u, double *v) {
+  double a[9];
+  for (int i = 0; i < 9; i++) a[i] = a_in[i];
+  for (int i = 0; i < 9; i++) v[i] = 0.0;
+  v[0] = v[4] = v[8] = 1.0;
+  for (int sweep = 0; sweep < 16; sweep++) {
+    int rotated = 0;
+    for (int pq = 0; pq < 3; pq++) {
+      int p = (pq == 2) ? 1 : 0;
+      int q = (pq == 0) ? 1 : 2;
+      double al = 0.0, be = 0.0, ga = 0.0;
+      for (int i = 0; i < 3; i++) {
+        al = al + a[i + 3 * p] * a[i + 3 * p];
+        be = be + a[i + 3 * q] * a[i + 3 * q];
+        ga = ga + a[i + 3 * p] * a[i + 3 * q];
+      }
+      if (ga == 0.0) continue;
+      if (fabs(ga) <= 1e-15 * (sqrt(al) * sqrt(be))) continue;
+      rotated = 1;
+      double zeta = (be - al) / (2.0 * ga);
+      double t;
+      if (fabs(zeta) > 1e100)
+        t = 0.5 / zeta;
+      else
+        t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      double c = 1.0 / sqrt(1.0 + t * t);
+      double sn = c * t;
+      for (int i = 0; i < 3; i++) {
+        double ap = a[i + 3 * p], aq = a[i + 3 * q];
+        a[i + 3 * p] = c * ap - sn * aq;
+        a[i + 3 * q] = sn * ap + c * aq;
+        double vp = v[i + 3 * p], vq = v[i + 3 * q];
+        v[i + 3 * p] = c * vp - sn * vq;
+        v[i + 3 * q] = sn * vp + c * vq;
+      }
+    }
+    if (!rotated) break;
+  }
+  double sv[3];
+  for (int k = 0; k < 3; k++) {
+    sv[k] = sqrt(a[3 * k] * a[3 * k] + a[3 * k + 1] * a[3 * k + 1] + a[3 * k + 2] * a[3 * k + 2]);
+    for (int i = 0; i < 3; i++) u[i + 3 * k] = (sv[k] > 0.0) ? a[i + 3 * k] / sv[k] : 0.0;
+  }
+  /* stable sort by decreasing singular value */
+  int ord[3] = {0, 1, 2};
+  for (int i = 1; i < 3; i++) {
+    int key = ord[i], j = i - 1;
+    while (j >= 0 && sv[ord[j]] < sv[key]) {
+      ord[j + 1] = ord[j];
+      j--;
+    }
+    ord[j + 1] = key;
+  }
+  double uu[9], vv[9];
+  for (int k = 0; k < 3; k++) {
+    s[k] = sv[ord[k]];
+    for (int i = 0; i < 3; i++) {
+      uu[i + 3 * k] = u[i + 3 * ord[k]];
+      vv[i + 3 * k] = v[i + 3 * ord[k]];
+    }
+  }
+  for (int i = 0; i < 9; i++) {
+    u[i] = uu[i];
+    v[i] = vv[i];
+  }
+}
+
+/* In-place pseudo-inverse of a column-major 3x3 matrix. */
+static inline void om_pinv3(double *A) {
+  double s[3], u[9], v[9];
+  om_svd3(A, s, u, v);
+  double tol = 3.0 * 1e-12 * s[0];
+  double sp[3];
+  for (int k = 0; k < 3; k++) sp[k] = (s[k] > tol) ? 1.0 / s[k] : 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; k++) acc = acc + v[i + 3 * k] * (sp[k] * u[j + 3 * k]);
+      A[i + 3 * j] = acc;
+    }
+}
+
+/* lu_solve_noswp, linear_solver_module.F90:413-445 (no pivoting) */
+static inline void om_lu_solve_noswp(int n, const double *A, double *x, const double *b) {
+  double C[64];
+  for (int i = 0; i < n * n; i++) C[i] = A[i];
+  for (int i = 0; i < n; i++) x[i] = b[i];
+  for (int k = 0; k < n; k++) {
+    double invdiag = 1.0 / C[k + n * k];
+    for (int i = k + 1; i < n; i++) {
+      double lik = C[i + n * k] * invdiag;
+      for (int j = k; j < n; j++) C[i + n * j] = C[i + n * j] - lik * C[k + n * j];
+      x[i] = x[i] - lik * x[k];
+    }
+  }
+  x[n - 1] = x[n - 1] / C[(n - 1) + n * (n - 1)];
+  for (int k = n - 2; k >= 0; k--) {
+    for (int j = k + 1; j < n; j++) x[k] = x[k] - C[k + n * j] * x[j];
+    x[k] = x[k] / C[k + n * k];
+  }
+}
+
+#endif

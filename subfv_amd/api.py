@@ -1,0 +1,205 @@
+"""Host-side mirror of subfv's explicit residual interface over the C-ABI.
+
+Method names, argument meaning and error behaviour follow the reference's
+module procedures (compute_euler_flux, compute_cell_grad_from_nodal_grad,
+flux_limiting, add_timedisc_and_solve, compute_rhs_ns, the ns_main time
+loop); the reference's fatal ``error stop`` conditions raise ``SubfvError``
+with the reference message.  Arrays use the Fortran layouts of the
+reference: sol(5,C) -> shape (C, 5), grad(3,5,C) -> shape (C, 5, 3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, make_config  # noqa: F401  (re-exported)
+
+
+class SubfvError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[subfv {code}] {msg}")
+        self.code = code
+
+
+class HostMesh:
+    """Connectivity of build_mesh computed on the host only (no device)."""
+
+    def __init__(self, mesh):
+        L = _lib.lib()
+        h = C.c_void_p()
+        coord = _lib.f64(mesh.coord).ravel()
+        kind = _lib.i32(mesh.kind)
+        ev = _lib.i32(mesh.elem_vert)
+        bn, bv, bt = _lib.i32(mesh.bf_nvert), _lib.i32(mesh.bf_vert), _lib.i32(mesh.bf_tag)
+        st = L.subfv_host_mesh_build(C.byref(h), mesh.n_vert, _lib.dptr(coord), mesh.n_elems, _lib.iptr(kind),
+                                     _lib.iptr(ev), int(bn.size), _lib.iptr(bn), _lib.iptr(bv), _lib.iptr(bt))
+        if st:
+            raise SubfvError(st, "host mesh build failed")
+        self.h = h
+        self._cap = 64 * (ev.size + mesh.n_vert + mesh.n_elems) + 1024
+
+    def geti(self, name: str) -> np.ndarray:
+        out = np.zeros(self._cap, np.int32)
+        n = _lib.lib().subfv_host_mesh_get_i32(self.h, name.encode(), _lib.iptr(out), out.size)
+        if n < 0:
+            raise KeyError(name)
+        return out[:n].copy()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.lib().subfv_host_mesh_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Solver:
+    """One device context: mesh, configuration and resident state."""
+
+    def __init__(self, device: int = 0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        st = L.subfv_create(C.byref(h), device)
+        if st:
+            raise SubfvError(st, "subfv_create failed (no usable HIP device?)")
+        self.h = h
+        self.n_vert = self.n_elems = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().subfv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int):
+        if st:
+            msg = _lib.lib().subfv_error(self.h)
+            raise SubfvError(st, msg.decode() if msg else "")
+
+    # ------------------------------------------------------------- mesh
+    def build_mesh(self, mesh):
+        """read_mesh_msh_2 + build_mesh + compute_geometry_mesh."""
+        L = _lib.lib()
+        coord = _lib.f64(mesh.coord).ravel()
+        kind = _lib.i32(mesh.kind)
+        ev = _lib.i32(mesh.elem_vert)
+        bn, bv, bt = _lib.i32(mesh.bf_nvert), _lib.i32(mesh.bf_vert), _lib.i32(mesh.bf_tag)
+        self._check(L.subfv_mesh_build(self.h, mesh.n_vert, _lib.dptr(coord), mesh.n_elems, _lib.iptr(kind),
+                                       _lib.iptr(ev), int(bn.size), _lib.iptr(bn), _lib.iptr(bv), _lib.iptr(bt),
+                                       int(mesh.boundary_2d)))
+        self.n_vert = int(L.subfv_mesh_count(self.h, b"n_vert"))
+        self.n_elems = int(L.subfv_mesh_count(self.h, b"n_elems"))
+        self.n_faces = int(L.subfv_mesh_count(self.h, b"n_faces"))
+        self.n_sub_faces = int(L.subfv_mesh_count(self.h, b"n_sub_faces"))
+        self.n_sub_elems = int(L.subfv_mesh_count(self.h, b"n_sub_elems"))
+        self._cap = 64 * (self.n_sub_faces + self.n_sub_elems + self.n_elems + self.n_vert + self.n_faces) + 64
+
+    def count(self, what: str) -> int:
+        return int(_lib.lib().subfv_mesh_count(self.h, what.encode()))
+
+    def mesh_i(self, name: str) -> np.ndarray:
+        out = np.zeros(self._cap, np.int32)
+        n = _lib.lib().subfv_mesh_get_i32(self.h, name.encode(), _lib.iptr(out), out.size)
+        if n < 0:
+            raise KeyError(name)
+        return out[:n].copy()
+
+    def mesh_d(self, name: str) -> np.ndarray:
+        out = np.zeros(self._cap, np.float64)
+        n = _lib.lib().subfv_mesh_get_f64(self.h, name.encode(), _lib.dptr(out), out.size)
+        if n < 0:
+            raise KeyError(name)
+        return out[:n].copy()
+
+    @property
+    def elem_coord(self) -> np.ndarray:
+        return self.mesh_d("elem_coord").reshape(-1, 3)
+
+    @property
+    def elem_volume(self) -> np.ndarray:
+        return self.mesh_d("elem_volume")
+
+    # ------------------------------------------------------------- config/state
+    def set_config(self, cfg: Config):
+        self._check(_lib.lib().subfv_set_config(self.h, C.byref(cfg)))
+
+    def set_time(self, t: float):
+        self._check(_lib.lib().subfv_set_time(self.h, float(t)))
+
+    def put_sol(self, sol: np.ndarray):
+        s = _lib.f64(sol).reshape(self.n_elems, 5)
+        self._check(_lib.lib().subfv_state_put(self.h, _lib.dptr(s)))
+
+    def get_sol(self) -> np.ndarray:
+        s = np.zeros((self.n_elems, 5))
+        self._check(_lib.lib().subfv_state_get(self.h, _lib.dptr(s)))
+        return s
+
+    _SHAPES = {"sol": (5,), "rhs": (5,), "rhs_o2": (5,), "sum_lambda": (), "sum_lambda_o2": (),
+               "cell_grad": (5, 3), "nodal_grad": (5, 3), "vp": (3,)}
+    _PER_VERT = {"nodal_grad", "vp"}
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.n_vert if name in self._PER_VERT else self.n_elems
+        shape = (n,) + self._SHAPES[name]
+        out = np.zeros(shape)
+        r = _lib.lib().subfv_get_f64(self.h, name.encode(), _lib.dptr(out), out.size)
+        if r < 0:
+            raise KeyError(name)
+        return out
+
+    def put(self, name: str, arr: np.ndarray):
+        a = _lib.f64(arr)
+        r = _lib.lib().subfv_put_f64(self.h, name.encode(), _lib.dptr(a), a.size)
+        if r < 0:
+            raise KeyError(name)
+
+    # ------------------------------------------------------------- procedures
+    def compute_euler_flux(self, sol=None, grad=None, second_order=False, want_vp=False):
+        """compute_euler_flux: returns (rhs, sum_lambda[, vp])."""
+        if sol is not None:
+            self.put_sol(sol)
+        if second_order:
+            if grad is not None:
+                self.put("cell_grad", grad)
+            self._check(_lib.lib().subfv_compute_euler_flux(self.h, 1, int(want_vp)))
+            out = (self.get("rhs_o2"), self.get("sum_lambda_o2"))
+        else:
+            self._check(_lib.lib().subfv_compute_euler_flux(self.h, 0, int(want_vp)))
+            out = (self.get("rhs"), self.get("sum_lambda"))
+        return out + (self.get("vp"),) if want_vp else out
+
+    def compute_cell_grad_from_nodal_grad(self, sol=None, method: int = 4):
+        if sol is not None:
+            self.put_sol(sol)
+        self._check(_lib.lib().subfv_compute_cell_grad(self.h, method))
+        return self.get("cell_grad"), self.get("nodal_grad")
+
+    def flux_limiting(self, cfl: float):
+        self._check(_lib.lib().subfv_flux_limiting(self.h, cfl))
+        return self.get("rhs")
+
+    def add_timedisc_and_solve(self, a1: float, a2: float, cfl: float, t: float):
+        dt = C.c_double(0.0)
+        om = C.c_double(0.0)
+        self._check(_lib.lib().subfv_add_timedisc_and_solve(self.h, a1, a2, cfl, t, C.byref(dt), C.byref(om)))
+        return dt.value, om.value
+
+    def compute_rhs_ns(self):
+        self._check(_lib.lib().subfv_compute_rhs_ns(self.h))
+        return self.get("rhs"), self.get("sum_lambda")
+
+    def run(self, n_steps: int, time_scheme: int = 0):
+        """Time loop on device-resident state; returns (t, dt)."""
+        t = C.c_double(0.0)
+        dt = C.c_double(0.0)
+        self._check(_lib.lib().subfv_run(self.h, n_steps, time_scheme, C.byref(t), C.byref(dt)))
+        return t.value, dt.value

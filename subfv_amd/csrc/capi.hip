@@ -1,0 +1,1118 @@
+// C-ABI of libsubfv_hip: mesh upload, configuration, device-resident state
+// and the stage orchestration of the explicit residual path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/subfv.h"
+#include "device_mesh.hpp"
+#include "kernels_bcsr.hpp"
+#include "kernels_cell.hpp"
+#include "kernels_flux.hpp"
+#include "kernels_geom.hpp"
+#include "mesh_build.hpp"
+
+using namespace sfv;
+
+struct subfv_host_mesh {
+  subfv::HostMesh hm;
+};
+
+struct subfv_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool has_mesh = false;
+  bool b2d = false;
+  subfv::HostMesh hm;
+  DevMesh dm;
+  BCParams bc{};
+  subfv_config cfg{};
+  std::vector<void *> allocs;  // mesh-lifetime device buffers
+  // state and work arrays
+  double *U[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
+  double *P = nullptr, *P2 = nullptr;
+  double *rhs = nullptr, *rhs2 = nullptr, *sl = nullptr, *sl2 = nullptr;
+  double *NG = nullptr, *VMM = nullptr, *G = nullptr, *vp = nullptr, *diag = nullptr;
+  double *blk = nullptr;
+  int *derr = nullptr;
+  StepState *st = nullptr;
+  double host_t = 0.0;
+};
+
+namespace {
+
+int fail(subfv_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                       \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return fail((c), SUBFV_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <class T>
+int dalloc(subfv_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(c, SUBFV_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+template <class T>
+int upload(subfv_ctx *c, T **p, const std::vector<T> &v) {
+  int r = dalloc(c, p, v.size());
+  if (r) return r;
+  if (!v.empty()) HIPCHK(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+void free_mesh(subfv_ctx *c) {
+  for (void *p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  c->dm = DevMesh();
+  c->has_mesh = false;
+  for (auto &u : c->U) u = nullptr;
+  c->P = c->P2 = c->rhs = c->rhs2 = c->sl = c->sl2 = nullptr;
+  c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = nullptr;
+}
+
+inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+int check_err(subfv_ctx *c) {
+  int e = 0;
+  HIPCHK(c, hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  StepState hs;
+  HIPCHK(c, hipMemcpyAsync(&hs, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int bits = e | hs.err;
+  if (bits & kErrNegStarDensity) return fail(c, SUBFV_E_NEG_STAR, "Negative specific volume MPCC !");
+  if (bits & kErrBadOmega) return fail(c, SUBFV_E_BAD_OMEGA, "BAD OMEGA " + std::to_string(hs.omega_min));
+  if (bits & kErrBadBC) return fail(c, SUBFV_E_BAD_BC, "BC TYPE NOT RECOGNIZED !");
+  return 0;
+}
+
+// wave packing: consecutive vertices, <= 64 sub-faces and <= 16 vertices each
+std::vector<int> pack_waves(const subfv::HostMesh &hm) {
+  std::vector<int> w{0};
+  int v = 0;
+  while (v < hm.nv) {
+    int cnt = 0, nvw = 0;
+    while (v < hm.nv && nvw < kWaveVert) {
+      int ns = hm.sf_off[v + 1] - hm.sf_off[v];
+      if (cnt + ns > kWaveSub) break;
+      cnt += ns;
+      nvw++;
+      v++;
+    }
+    w.push_back(v);
+  }
+  return w;
+}
+
+int upload_mesh(subfv_ctx *c) {
+  const subfv::HostMesh &h = c->hm;
+  DevMesh &d = c->dm;
+  d.V = h.nv;
+  d.C = h.ne;
+  d.F = h.nf;
+  d.SF = h.nsf;
+  d.SE = h.nse;
+  int r;
+#define UP(field, vec) \
+  if ((r = upload(c, &d.field, vec))) return r
+  UP(vcoord, h.vcoord);
+  UP(sf_off, h.sf_off);
+  UP(se_off, h.v_seoff);
+  UP(v_eoff, h.v_eoff);
+  UP(v_elem, h.v_elem);
+  UP(v_seid, h.v_se);
+  UP(f_voff, h.f_voff);
+  UP(f_vert, h.f_vert);
+  UP(f_left, h.f_left);
+  UP(f_right, h.f_right);
+  UP(f_sfoff, h.f_sfoff);
+  UP(f_sf, h.f_sf);
+  UP(sf_vert, h.sf_vert);
+  UP(sf_face, h.sf_face);
+  UP(sf_le, h.sf_le);
+  std::vector<int> re(h.nsf);
+  for (int s = 0; s < h.nsf; s++) re[s] = h.sf_re[s] - 1;  // device right-neighbour code
+  UP(sf_re, re);
+  UP(se_sfoff, h.se_sfoff);
+  UP(se_sf, h.se_sf);
+  UP(se_elem, h.se_elem);
+  std::vector<int> secell(h.nse);
+  for (int q = 0; q < h.nse; q++) secell[q] = h.se_elem[h.v_se[q]];
+  UP(se_cell, secell);
+  UP(e_voff, h.e_voff);
+  UP(e_vert, h.e_vert);
+  UP(e_foff, h.e_foff);
+  UP(e_face, h.e_face);
+  // each cell's node-sorted slots in increasing vertex id (scatter order of
+  // compute_euler_flux, ns_euler_module.F90:321-336)
+  std::vector<int> cslot(h.nse);
+  for (int e = 0; e < h.ne; e++) {
+    int b = h.e_voff[e], n = h.e_voff[e + 1] - b;
+    std::vector<std::pair<int, int>> pv(n);
+    for (int j = 0; j < n; j++) {
+      int se = b + j, v = h.se_vert[se];
+      pv[j] = {v, h.v_seoff[v] + h.se_loc[se]};
+    }
+    std::sort(pv.begin(), pv.end());
+    for (int j = 0; j < n; j++) cslot[b + j] = pv[j].second;
+  }
+  UP(c_slot, cslot);
+  std::vector<int> waves = pack_waves(h);
+  d.nwaves = (int)waves.size() - 1;
+  UP(wave_v, waves);
+#undef UP
+  if ((r = dalloc(c, &d.vvol, h.nv))) return r;
+  if ((r = dalloc(c, &d.vbound, h.nv))) return r;
+  if ((r = dalloc(c, &d.Bp, 3L * h.nv))) return r;
+  if ((r = dalloc(c, &d.fcoord, 3L * h.nf))) return r;
+  if ((r = dalloc(c, &d.fnorm, 3L * h.nf))) return r;
+  if ((r = dalloc(c, &d.farea, h.nf))) return r;
+  if ((r = dalloc(c, &d.sfn, 4L * h.nsf))) return r;
+  if ((r = dalloc(c, &d.se_vol, h.nse))) return r;
+  if ((r = dalloc(c, &d.ccoord, 3L * h.ne))) return r;
+  if ((r = dalloc(c, &d.cvol, h.ne))) return r;
+  // state and work arrays
+  const long C = h.ne, V = h.nv, SE = h.nse;
+  for (int i = 0; i < 3; i++)
+    if ((r = dalloc(c, &c->U[i], 5 * C))) return r;
+  if ((r = dalloc(c, &c->P, 6 * SE))) return r;
+  if ((r = dalloc(c, &c->P2, 6 * SE))) return r;
+  if ((r = dalloc(c, &c->rhs, 5 * C))) return r;
+  if ((r = dalloc(c, &c->rhs2, 5 * C))) return r;
+  if ((r = dalloc(c, &c->sl, C))) return r;
+  if ((r = dalloc(c, &c->sl2, C))) return r;
+  if ((r = dalloc(c, &c->NG, 15 * V))) return r;
+  if ((r = dalloc(c, &c->VMM, 4 * V))) return r;
+  if ((r = dalloc(c, &c->G, 15 * C))) return r;
+  if ((r = dalloc(c, &c->vp, 3 * V))) return r;
+  if ((r = dalloc(c, &c->blk, nblk(std::max(C, V), kCellThreads) + 64))) return r;
+  HIPCHK(c, hipMemsetAsync(c->G, 0, sizeof(double) * 15 * C, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->NG, 0, sizeof(double) * 15 * V, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->vp, 0, sizeof(double) * 3 * V, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->U[0], 0, sizeof(double) * 5 * C, c->stream));
+  c->cur = 0;
+  return 0;
+}
+
+int geometry(subfv_ctx *c) {
+  DevMesh &d = c->dm;
+  const int T = 256;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_face_centroid, dim3(nblk(d.F, T)), dim3(T), 0, s, d);
+  hipLaunchKernelGGL(k_elem_centroid, dim3(nblk(d.C, T)), dim3(T), 0, s, d);
+  hipLaunchKernelGGL(k_subface_norm, dim3(nblk(d.SF, T)), dim3(T), 0, s, d);
+  hipLaunchKernelGGL(k_face_norm, dim3(nblk(d.F, T)), dim3(T), 0, s, d);
+  hipLaunchKernelGGL(k_subelem_volume, dim3(nblk(d.SE, T)), dim3(T), 0, s, d);
+  int *bad = nullptr;
+  HIPCHK(c, hipMallocAsync((void **)&bad, sizeof(int), s));
+  HIPCHK(c, hipMemsetAsync(bad, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_elem_volume_check, dim3(nblk(d.C, T)), dim3(T), 0, s, d, bad);
+  hipLaunchKernelGGL(k_vert_volume_bound, dim3(nblk(d.V, T)), dim3(T), 0, s, d, (int)c->b2d);
+  HIPCHK(c, hipGetLastError());
+  int hb = 0;
+  HIPCHK(c, hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipFreeAsync(bad, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (hb) return fail(c, SUBFV_E_MESH, "MESH GEOMETRY IS NOT OK (" + std::to_string(hb) + " cells)");
+  return 0;
+}
+
+int apply_bc(subfv_ctx *c) {
+  if (!c->has_mesh) return 0;
+  DevMesh &d = c->dm;
+  hipLaunchKernelGGL(k_wall_normal, dim3(nblk(d.V, 256)), dim3(256), 0, c->stream, d, c->bc);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// --------------------------------------------------------------- launches
+int launch_flux(subfv_ctx *c, const double *U, const double *G, double *P, bool want_vp) {
+  FluxArgs a;
+  a.m = c->dm;
+  a.bc = c->bc;
+  a.U = U;
+  a.G = G;
+  a.P = P;
+  a.vp = want_vp ? c->vp : nullptr;
+  a.err = c->derr;
+  a.t = &c->st->t;
+  a.second_order = G != nullptr;
+  int nb = (c->dm.nwaves + kFluxWaves - 1) / kFluxWaves;
+  a.per_xcd = (nb + 7) / 8;
+  a.nblocks = nb;
+  dim3 grid(8 * a.per_xcd), blk(64 * kFluxWaves);
+  switch (c->bc.scheme) {
+    case kMultiPoint: hipLaunchKernelGGL(k_node_flux<kMultiPoint>, grid, blk, 0, c->stream, a); break;
+    case kMultiPointIso: hipLaunchKernelGGL(k_node_flux<kMultiPointIso>, grid, blk, 0, c->stream, a); break;
+    case kThreeWave: hipLaunchKernelGGL(k_node_flux<kThreeWave>, grid, blk, 0, c->stream, a); break;
+    case kTwoWave: hipLaunchKernelGGL(k_node_flux<kTwoWave>, grid, blk, 0, c->stream, a); break;
+    case kModifiedThreeWave: hipLaunchKernelGGL(k_node_flux<kModifiedThreeWave>, grid, blk, 0, c->stream, a); break;
+    case kMultiPointPressure: hipLaunchKernelGGL(k_node_flux<kMultiPointPressure>, grid, blk, 0, c->stream, a); break;
+    case kMultiPointPressurePh: hipLaunchKernelGGL(k_node_flux<kMultiPointPressurePh>, grid, blk, 0, c->stream, a); break;
+    default: return fail(c, SUBFV_E_INVALID, "scheme not supported on the device");
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int launch_gather(subfv_ctx *c, const double *P, const double *P2, double *rhs, double *rhs2, double *sl,
+                  int fixed_dt) {
+  GatherArgs g;
+  g.m = c->dm;
+  g.P = P;
+  g.P2 = P2;
+  g.rhs = rhs;
+  g.rhs2 = rhs2;
+  g.sl = sl;
+  g.blk = c->blk;
+  g.st = c->st;
+  g.fixed_dt = fixed_dt;
+  hipLaunchKernelGGL(k_gather, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int launch_grad(subfv_ctx *c, const double *U, int method) {
+  GradArgs g;
+  g.m = c->dm;
+  g.U = U;
+  g.NG = c->NG;
+  g.VMM = c->VMM;
+  g.G = c->G;
+  g.method = method;
+  hipLaunchKernelGGL(k_nodal_grad, dim3(nblk(c->dm.V, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
+  hipLaunchKernelGGL(k_cell_grad, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout, double a1, double a2, double w0,
+                  double w1, int flux_limit, int advance_t) {
+  UpdateArgs u;
+  u.m = c->dm;
+  u.U = U;
+  u.U0 = U0;
+  u.Uout = Uout;
+  u.rhs = c->rhs;
+  u.rhs2 = c->rhs2;
+  u.sl = c->sl;
+  u.diag = c->diag;
+  u.blk = c->blk;
+  u.st = c->st;
+  u.a1 = a1;
+  u.a2 = a2;
+  u.w0 = w0;
+  u.w1 = w1;
+  u.flux_limit = flux_limit;
+  u.advance_t = advance_t;
+  u.fixup_pass = 0;
+  dim3 g(nblk(c->dm.C, kCellThreads)), b(kCellThreads);
+  hipLaunchKernelGGL(k_update, g, b, 0, c->stream, u);
+  hipLaunchKernelGGL(k_update_fixup, g, b, 0, c->stream, u);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int sync_state_scalars(subfv_ctx *c, StepState *hs) {
+  HIPCHK(c, hipMemcpyAsync(hs, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int push_state_scalars(subfv_ctx *c, double t) {
+  StepState hs{};
+  hs.t = t;
+  hs.tmax = c->cfg.tmax;
+  hs.cfl = c->cfg.cfl;
+  hs.lts = c->cfg.local_time_step;
+  hs.omega = 1.0;
+  HIPCHK(c, hipMemcpyAsync(c->st, &hs, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->derr, 0, sizeof(int), c->stream));
+  return 0;
+}
+
+int mesh_i32(const subfv::HostMesh &h, const char *name, int32_t *out, int64_t cap) {
+  std::vector<int> v;
+  std::string n(name);
+  auto list = [&](const std::vector<int> &off, const std::vector<int> &ids, int shift) {
+    for (size_t i = 0; i < ids.size(); i++) v.push_back(ids[i] + shift);
+    (void)off;
+  };
+  if (n == "vert_nto") list({}, h.vert_nto, 1);
+  else if (n == "elem_nto") list({}, h.elem_nto, 1);
+  else if (n == "vert_n_sub_faces") for (int i = 0; i < h.nv; i++) v.push_back(h.sf_off[i + 1] - h.sf_off[i]);
+  else if (n == "vert_n_sub_elems") for (int i = 0; i < h.nv; i++) v.push_back(h.v_seoff[i + 1] - h.v_seoff[i]);
+  else if (n == "vert_n_elems") for (int i = 0; i < h.nv; i++) v.push_back(h.v_eoff[i + 1] - h.v_eoff[i]);
+  else if (n == "vert_sub_face") for (int i = 0; i < h.nsf; i++) v.push_back(i + 1);
+  else if (n == "vert_sub_elem") list({}, h.v_se, 1);
+  else if (n == "vert_elem") list({}, h.v_elem, 1);
+  else if (n == "elem_n_vert") for (int i = 0; i < h.ne; i++) v.push_back(h.e_voff[i + 1] - h.e_voff[i]);
+  else if (n == "elem_kind") list({}, h.e_kind, 0);
+  else if (n == "elem_vert") list({}, h.e_vert, 1);
+  else if (n == "elem_face") list({}, h.e_face, 1);
+  else if (n == "elem_sub_elem") for (int i = 0; i < h.nse; i++) v.push_back(i + 1);
+  else if (n == "elem_tag") list({}, h.e_tag, 0);
+  else if (n == "face_n_vert") for (int i = 0; i < h.nf; i++) v.push_back(h.f_voff[i + 1] - h.f_voff[i]);
+  else if (n == "face_vert") list({}, h.f_vert, 1);
+  else if (n == "face_left") list({}, h.f_left, 1);
+  else if (n == "face_right") list({}, h.f_right, 0);
+  else if (n == "face_sub_face") list({}, h.f_sf, 1);
+  else if (n == "sub_face_vert") list({}, h.sf_vert, 1);
+  else if (n == "sub_face_face") list({}, h.sf_face, 1);
+  else if (n == "sub_face_loc") for (int i = 0; i < h.nv; i++) for (int s = h.sf_off[i]; s < h.sf_off[i + 1]; s++) v.push_back(s - h.sf_off[i] + 1);
+  else if (n == "sub_face_left_elem") list({}, h.sf_le, 1);
+  else if (n == "sub_face_right_elem") list({}, h.sf_re, 0);
+  else if (n == "sub_face_left_sub_elem") list({}, h.sf_lse, 1);
+  else if (n == "sub_face_right_sub_elem") list({}, h.sf_rse, 1);
+  else if (n == "sub_elem_loc") list({}, h.se_loc, 1);
+  else if (n == "sub_elem_vert") list({}, h.se_vert, 1);
+  else if (n == "sub_elem_elem") list({}, h.se_elem, 1);
+  else if (n == "sub_elem_sub_face") list({}, h.se_sf, 1);
+  else return -1;
+  if ((int64_t)v.size() > cap) return -1;
+  std::copy(v.begin(), v.end(), out);
+  return (int64_t)v.size();
+}
+
+int dev_to_host_soa(subfv_ctx *c, const double *d, int ncomp, long n, double *out) {
+  std::vector<double> tmp((size_t)ncomp * n);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (long i = 0; i < n; i++)
+    for (int k = 0; k < ncomp; k++) out[(size_t)i * ncomp + k] = tmp[(size_t)k * n + i];
+  return 0;
+}
+
+int host_to_dev_soa(subfv_ctx *c, const double *in, int ncomp, long n, double *d) {
+  std::vector<double> tmp((size_t)ncomp * n);
+  for (long i = 0; i < n; i++)
+    for (int k = 0; k < ncomp; k++) tmp[(size_t)k * n + i] = in[(size_t)i * ncomp + k];
+  HIPCHK(c, hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// dt from the device sum_lambda alone (standalone add_timedisc_and_solve)
+__global__ __launch_bounds__(kCellThreads) void k_dt_from_sl(DevMesh m, const double *sl, double *blk, StepState *st) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double local = (c < m.C) ? m.cvol[c] / sl[c] : 1e10;
+  double mn;
+  if (block_min_last(local, blk, &st->ticket, gridDim.x, &mn)) {
+    double d = dmin(1e10, mn);
+    st->dtmin = d;
+    st->dt_fl = st->cfl * d;
+    double dt = st->cfl * d;
+    if (!st->lts) dt = dmin(st->tmax - st->t + 1e-12, dt);
+    st->dt = dt;
+  }
+}
+
+// flux_limiting as a standalone pass (ns_euler_recon_module.F90:586-593)
+__global__ __launch_bounds__(kCellThreads) void k_flux_limiting(DevMesh m, const double *U, double *rhs,
+                                                                const double *rhs2, const double *sl,
+                                                                const StepState *st) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  const long C = m.C;
+  double cf = m.cvol[c] / st->dt_fl + sl[c];
+  double x[5], y[5], r[5], r2[5];
+  for (int k = 0; k < 5; k++) {
+    r[k] = rhs[k * C + c];
+    r2[k] = rhs2[k * C + c];
+    x[k] = U[k * C + c] + r[k] / cf;
+    y[k] = (r2[k] - r[k]) / cf;
+  }
+  double om = max_omega_pos(x, y);
+  for (int k = 0; k < 5; k++) rhs[k * C + c] = r[k] + dmin(1.0, om) * (r2[k] - r[k]);
+}
+
+}  // namespace
+
+// ======================================================================= C-ABI
+extern "C" {
+
+int subfv_create(subfv_ctx **out, int device) {
+  if (!out) return SUBFV_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SUBFV_E_DEVICE;
+  if (device < 0 || device >= n) return SUBFV_E_INVALID;
+  subfv_ctx *c = new subfv_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipMalloc((void **)&c->derr, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&c->st, sizeof(StepState)) != hipSuccess) {
+    delete c;
+    return SUBFV_E_DEVICE;
+  }
+  (void)hipMemset(c->derr, 0, sizeof(int));
+  (void)hipMemset(c->st, 0, sizeof(StepState));
+  c->cfg.scheme_id = SUBFV_SCHEME_MULTI_POINT;
+  c->cfg.bc_style = 2;
+  c->cfg.method = 4;
+  c->cfg.cfl = 0.4;
+  c->cfg.tmax = 1e30;
+  c->bc.scheme = kMultiPoint;
+  c->bc.bc_style = 2;
+  *out = c;
+  return 0;
+}
+
+void subfv_destroy(subfv_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  if (c->derr) (void)hipFree(c->derr);
+  if (c->st) (void)hipFree(c->st);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *subfv_error(const subfv_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int subfv_host_mesh_build(subfv_host_mesh **out, int32_t n_vert, const double *coord, int32_t n_elems,
+                          const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
+                          const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag) {
+  if (!out || !coord || !kind || !elem_vert) return SUBFV_E_INVALID;
+  auto *m = new subfv_host_mesh();
+  std::string e = subfv::build_host_mesh(m->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
+                                         bf_vert, bf_tag);
+  if (!e.empty()) {
+    delete m;
+    *out = nullptr;
+    return SUBFV_E_INVALID;
+  }
+  *out = m;
+  return 0;
+}
+
+int64_t subfv_host_mesh_get_i32(subfv_host_mesh *m, const char *name, int32_t *out, int64_t cap) {
+  if (!m || !name || !out) return -1;
+  return mesh_i32(m->hm, name, out, cap);
+}
+
+void subfv_host_mesh_free(subfv_host_mesh *m) { delete m; }
+
+int subfv_mesh_build(subfv_ctx *c, int32_t n_vert, const double *coord, int32_t n_elems, const int32_t *kind,
+                     const int32_t *elem_vert, int32_t n_bfaces, const int32_t *bf_nvert, const int32_t *bf_vert,
+                     const int32_t *bf_tag, int32_t boundary_2d) {
+  if (!c || !coord || !kind || !elem_vert) return fail(c, SUBFV_E_INVALID, "null argument");
+  if (n_bfaces > 0 && (!bf_nvert || !bf_vert || !bf_tag)) return fail(c, SUBFV_E_INVALID, "null boundary array");
+  HIPCHK(c, hipSetDevice(c->device));
+  free_mesh(c);
+  std::string e = subfv::build_host_mesh(c->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
+                                         bf_vert, bf_tag);
+  if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
+  for (int v = 0; v < c->hm.nv; v++)
+    if (c->hm.sf_off[v + 1] - c->hm.sf_off[v] > kWaveSub)
+      return fail(c, SUBFV_E_INVALID, "a vertex has more than 64 sub-faces");
+  c->b2d = boundary_2d != 0;
+  int r = upload_mesh(c);
+  if (r) return r;
+  r = geometry(c);
+  if (r) return r;
+  c->has_mesh = true;
+  r = apply_bc(c);
+  if (r) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
+  if (!c || !c->has_mesh || !what) return -1;
+  std::string w(what);
+  if (w == "n_vert") return c->hm.nv;
+  if (w == "n_elems") return c->hm.ne;
+  if (w == "n_faces") return c->hm.nf;
+  if (w == "n_sub_faces") return c->hm.nsf;
+  if (w == "n_sub_elems") return c->hm.nse;
+  if (w == "n_max_sub_faces") return c->hm.nmax_sfn;
+  if (w == "n_waves") return c->dm.nwaves;
+  return -1;
+}
+
+int64_t subfv_mesh_get_i32(subfv_ctx *c, const char *name, int32_t *out, int64_t cap) {
+  if (!c || !c->has_mesh || !name || !out) return -1;
+  if (std::string(name) == "vert_is_bound") {
+    std::vector<uint8_t> b(c->hm.nv);
+    if (hipMemcpy(b.data(), c->dm.vbound, b.size(), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (cap < c->hm.nv) return -1;
+    for (int i = 0; i < c->hm.nv; i++) out[i] = b[i];
+    return c->hm.nv;
+  }
+  return mesh_i32(c->hm, name, out, cap);
+}
+
+int64_t subfv_mesh_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) {
+  if (!c || !c->has_mesh || !name || !out) return -1;
+  std::string n(name);
+  const DevMesh &d = c->dm;
+  auto cp = [&](const double *src, long cnt) -> int64_t {
+    if (cnt > cap) return -1;
+    if (hipMemcpy(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return cnt;
+  };
+  if (n == "vert_coord") {
+    if (3L * d.V > cap) return -1;
+    std::copy(c->hm.vcoord.begin(), c->hm.vcoord.end(), out);
+    return 3L * d.V;
+  }
+  if (n == "vert_volume") return cp(d.vvol, d.V);
+  if (n == "elem_coord") return cp(d.ccoord, 3 * d.C);
+  if (n == "elem_volume") return cp(d.cvol, d.C);
+  if (n == "face_coord") return cp(d.fcoord, 3 * d.F);
+  if (n == "face_norm") return cp(d.fnorm, 3 * d.F);
+  if (n == "face_area") return cp(d.farea, d.F);
+  if (n == "sub_elem_volume") return cp(d.se_vol, d.SE);
+  if (n == "sub_face_area") return cp(d.sfn + 3 * d.SF, d.SF);
+  if (n == "sub_face_norm") {
+    if (3 * d.SF > cap) return -1;
+    return dev_to_host_soa(c, d.sfn, 3, d.SF, out) ? -1 : 3 * d.SF;
+  }
+  return -1;
+}
+
+int subfv_set_config(subfv_ctx *c, const subfv_config *cfg) {
+  if (!c || !cfg) return fail(c, SUBFV_E_INVALID, "null argument");
+  switch (cfg->scheme_id) {
+    case kMultiPoint: case kMultiPointIso: case kThreeWave: case kTwoWave: case kModifiedThreeWave:
+    case kMultiPointPressure: case kMultiPointPressurePh: break;
+    default: return fail(c, SUBFV_E_INVALID, "ERROR: scheme unrecognized");
+  }
+  if (cfg->n_bc < 0 || cfg->n_bc > SUBFV_MNBC) return fail(c, SUBFV_E_INVALID, "n_bc out of range");
+  if (cfg->method < 0 || cfg->method > 6) return fail(c, SUBFV_E_INVALID, "Method for second order unknown !");
+  for (int i = 0; i < cfg->n_bc; i++)
+    if (cfg->bc_euler_id[i] < 1 || cfg->bc_euler_id[i] > 7)
+      return fail(c, SUBFV_E_INVALID, "ERROR: bc_type unrecognized");
+  c->cfg = *cfg;
+  BCParams &b = c->bc;
+  b = BCParams{};
+  b.scheme = cfg->scheme_id;
+  b.bc_style = cfg->bc_style;
+  b.b2d = cfg->boundary_2d;
+  b.excl = cfg->exclude_bound_vert;
+  b.n_bc = cfg->n_bc;
+  b.lts = cfg->local_time_step;
+  for (int i = 0; i < cfg->n_bc; i++) {
+    b.bc_id[i + 1] = cfg->bc_euler_id[i];
+    for (int k = 0; k < 5; k++) b.bc_w[i + 1][k] = cfg->bc_val[i][k];
+  }
+  int r = apply_bc(c);
+  if (r) return r;
+  return push_state_scalars(c, c->host_t);
+}
+
+int subfv_set_time(subfv_ctx *c, double t) {
+  if (!c) return SUBFV_E_INVALID;
+  c->host_t = t;
+  return push_state_scalars(c, t);
+}
+
+int subfv_state_put(subfv_ctx *c, const double *sol) {
+  if (!c || !c->has_mesh || !sol) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
+  return host_to_dev_soa(c, sol, 5, c->dm.C, c->U[c->cur]);
+}
+
+int subfv_state_get(subfv_ctx *c, double *sol) {
+  if (!c || !c->has_mesh || !sol) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
+  return dev_to_host_soa(c, c->U[c->cur], 5, c->dm.C, sol);
+}
+
+int64_t subfv_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) {
+  if (!c || !c->has_mesh || !name || !out) return -1;
+  std::string n(name);
+  const long C = c->dm.C, V = c->dm.V;
+  struct Item { const char *nm; double *p; int nc; long n; };
+  Item items[] = {{"sol", c->U[c->cur], 5, C},   {"rhs", c->rhs, 5, C},      {"rhs_o2", c->rhs2, 5, C},
+                  {"sum_lambda", c->sl, 1, C},   {"sum_lambda_o2", c->sl2, 1, C}, {"cell_grad", c->G, 15, C},
+                  {"nodal_grad", c->NG, 15, V}, {"vp", c->vp, 3, V}};
+  for (auto &it : items)
+    if (n == it.nm) {
+      if ((int64_t)it.nc * it.n > cap) return -1;
+      return dev_to_host_soa(c, it.p, it.nc, it.n, out) ? -1 : (int64_t)it.nc * it.n;
+    }
+  return -1;
+}
+
+int64_t subfv_put_f64(subfv_ctx *c, const char *name, const double *in, int64_t nval) {
+  if (!c || !c->has_mesh || !name || !in) return -1;
+  std::string n(name);
+  const long C = c->dm.C;
+  struct Item { const char *nm; double *p; int nc; long n; };
+  Item items[] = {{"sol", c->U[c->cur], 5, C}, {"rhs", c->rhs, 5, C}, {"rhs_o2", c->rhs2, 5, C},
+                  {"sum_lambda", c->sl, 1, C}, {"cell_grad", c->G, 15, C}};
+  for (auto &it : items)
+    if (n == it.nm) {
+      if ((int64_t)it.nc * it.n != nval) return -1;
+      return host_to_dev_soa(c, in, it.nc, it.n, it.p) ? -1 : nval;
+    }
+  return -1;
+}
+
+int subfv_compute_euler_flux(subfv_ctx *c, int32_t second_order, int32_t want_vp) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  const double *U = c->U[c->cur];
+  double *P = second_order ? c->P2 : c->P;
+  int r = launch_flux(c, U, second_order ? c->G : nullptr, P, want_vp != 0);
+  if (r) return r;
+  r = launch_gather(c, P, nullptr, second_order ? c->rhs2 : c->rhs, nullptr, second_order ? c->sl2 : c->sl, 1);
+  if (r) return r;
+  return check_err(c);
+}
+
+int subfv_compute_cell_grad(subfv_ctx *c, int32_t method) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  if (method < 0 || method > 6) return fail(c, SUBFV_E_INVALID, "Method for second order unknown !");
+  int r = launch_grad(c, c->U[c->cur], method);
+  if (r) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int subfv_flux_limiting(subfv_ctx *c, double cfl) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  StepState hs;
+  int r = sync_state_scalars(c, &hs);
+  if (r) return r;
+  hs.cfl = cfl;
+  HIPCHK(c, hipMemcpyAsync(c->st, &hs, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
+  const DevMesh &d = c->dm;
+  hipLaunchKernelGGL(k_dt_from_sl, dim3(nblk(d.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, d, c->sl,
+                     c->blk, c->st);
+  hipLaunchKernelGGL(k_flux_limiting, dim3(nblk(d.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, d,
+                     c->U[c->cur], c->rhs, c->rhs2, c->sl, c->st);
+  HIPCHK(c, hipGetLastError());
+  hs.cfl = c->cfg.cfl;
+  HIPCHK(c, hipMemcpyAsync(&c->st->cfl, &hs.cfl, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int subfv_add_timedisc_and_solve(subfv_ctx *c, double a1, double a2, double cfl, double t, double *dt,
+                                 double *omega) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  StepState hs{};
+  hs.t = t;
+  hs.tmax = c->cfg.tmax;
+  hs.cfl = cfl;
+  hs.lts = c->cfg.local_time_step;
+  hs.omega = 1.0;
+  HIPCHK(c, hipMemcpyAsync(c->st, &hs, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
+  const DevMesh &d = c->dm;
+  hipLaunchKernelGGL(k_dt_from_sl, dim3(nblk(d.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, d, c->sl,
+                     c->blk, c->st);
+  int nxt = (c->cur + 1) % 3;
+  int r = launch_update(c, c->U[c->cur], nullptr, c->U[nxt], a1, a2, 0.0, 1.0, 0, 0);
+  if (r) return r;
+  r = sync_state_scalars(c, &hs);
+  if (r) return r;
+  if (omega) *omega = hs.omega;
+  if (dt) *dt = hs.dt_adv;
+  r = check_err(c);
+  if (r) return r;
+  c->cur = nxt;
+  HIPCHK(c, hipMemcpyAsync(c->st, &hs, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+int subfv_compute_rhs_ns(subfv_ctx *c) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  int r = subfv_compute_euler_flux(c, 0, 1);
+  if (r) return r;
+  if (c->cfg.second_order) {
+    r = subfv_compute_cell_grad(c, c->cfg.method);
+    if (r) return r;
+    r = subfv_compute_euler_flux(c, 1, 1);
+    if (r) return r;
+    r = subfv_flux_limiting(c, c->cfg.cfl);
+    if (r) return r;
+  }
+  return 0;
+}
+
+// One full residual evaluation on state U into rhs (and rhs2 in second
+// order), with dt from the first-order sum_lambda.
+static int residual(subfv_ctx *c, const double *U, int fixed_dt) {
+  int r;
+  if (c->cfg.second_order) {
+    if ((r = launch_grad(c, U, c->cfg.method))) return r;
+    if ((r = launch_flux(c, U, nullptr, c->P, false))) return r;
+    if ((r = launch_flux(c, U, c->G, c->P2, false))) return r;
+    return launch_gather(c, c->P, c->P2, c->rhs, c->rhs2, c->sl, fixed_dt);
+  }
+  if ((r = launch_flux(c, U, nullptr, c->P, false))) return r;
+  return launch_gather(c, c->P, nullptr, c->rhs, nullptr, c->sl, fixed_dt);
+}
+
+int subfv_run(subfv_ctx *c, int32_t n_steps, int32_t time_scheme, double *t, double *dt) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  if (time_scheme == 1 && c->cfg.local_time_step)
+    return fail(c, SUBFV_E_INVALID, "SSP-RK3 needs local_time_step = .false.");
+  int r;
+  const int fl = c->cfg.second_order ? 1 : 0;
+  for (int it = 0; it < n_steps; it++) {
+    int a = c->cur, b = (a + 1) % 3, q = (a + 2) % 3;
+    if (time_scheme == 0) {
+      if ((r = residual(c, c->U[a], 0))) return r;
+      if (c->cfg.second_order) {
+        // ns_main.F90:88-95: half step into U*, residual at U*, full step from U^n
+        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 2.0, 1.0, 0.0, 1.0, fl, 0))) return r;
+        if ((r = residual(c, c->U[b], 0))) return r;
+        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, fl, 1))) return r;
+      } else {
+        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, 0, 1))) return r;
+      }
+      c->cur = b;
+    } else {
+      // SSP-RK3: U1 = L(Un) -> b; U2 = 3/4 Un + 1/4 L(U1) -> q; Un+1 = 1/3 Un + 2/3 L(U2) -> b
+      if ((r = residual(c, c->U[a], 0))) return r;
+      if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, fl, 0))) return r;
+      if ((r = residual(c, c->U[b], 1))) return r;
+      if ((r = launch_update(c, c->U[b], c->U[a], c->U[q], 1.0, 1.0, 0.75, 0.25, fl, 0))) return r;
+      if ((r = residual(c, c->U[q], 1))) return r;
+      if ((r = launch_update(c, c->U[q], c->U[a], c->U[b], 1.0, 1.0, 1.0 / 3.0, 2.0 / 3.0, fl, 2))) return r;
+      c->cur = b;
+    }
+  }
+  StepState hs;
+  r = sync_state_scalars(c, &hs);
+  if (r) return r;
+  if (t) *t = hs.t;
+  if (dt) *dt = (time_scheme == 0) ? hs.dt_adv : hs.dt;
+  c->host_t = hs.t;
+  return check_err(c);
+}
+
+}  // extern "C"
+
+// ===================================================================== BCSR
+struct subfv_bcsr {
+  subfv_ctx *ctx = nullptr;
+  int m = 0, b = 0, nnzb = 0;
+  long n = 0;
+  int *row = nullptr, *col = nullptr, *diag_idx = nullptr;
+  double *v = nullptr, *F64 = nullptr;
+  float *v32 = nullptr, *F32 = nullptr;
+  // GMRES workspace
+  int mr = 0;
+  double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
+  double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr;
+  unsigned int *ticket = nullptr;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+template <class T>
+int balloc(subfv_bcsr *A, T **p, size_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(A->ctx, SUBFV_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  A->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+constexpr int kRedBlocks = 1024;
+
+int spmv(subfv_bcsr *A, bool fp32, const double *x, double *y) {
+  hipStream_t s = A->ctx->stream;
+  long thr = (long)A->m * A->b;
+  dim3 g(nblk(thr, 256)), bl(256);
+  if (A->b == 4) {
+    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 4>), g, bl, 0, s, A->m, A->row, A->col, A->v32, x, y);
+    else hipLaunchKernelGGL((k_bcsr_spmv<double, 4>), g, bl, 0, s, A->m, A->row, A->col, A->v, x, y);
+  } else if (A->b == 5) {
+    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 5>), g, bl, 0, s, A->m, A->row, A->col, A->v32, x, y);
+    else hipLaunchKernelGGL((k_bcsr_spmv<double, 5>), g, bl, 0, s, A->m, A->row, A->col, A->v, x, y);
+  } else if (A->b == 3) {
+    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 3>), g, bl, 0, s, A->m, A->row, A->col, A->v32, x, y);
+    else hipLaunchKernelGGL((k_bcsr_spmv<double, 3>), g, bl, 0, s, A->m, A->row, A->col, A->v, x, y);
+  } else {
+    return fail(A->ctx, SUBFV_E_INVALID, "block size must be 3, 4 or 5");
+  }
+  HIPCHK(A->ctx, hipGetLastError());
+  return 0;
+}
+
+int bjacobi(subfv_bcsr *A, bool fp32, const double *bvec, double *x) {
+  hipStream_t s = A->ctx->stream;
+  dim3 g(nblk(A->m, 256)), bl(256);
+  switch (A->b) {
+    case 3:
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 3>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 3>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      break;
+    case 4:
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 4>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 4>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      break;
+    default:
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 5>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 5>), g, bl, 0, s, A->m, A->F64, bvec, x);
+  }
+  HIPCHK(A->ctx, hipGetLastError());
+  return 0;
+}
+
+// *out = <a, b> (or <w, w>), device scalar
+void mgs_step(subfv_bcsr *A, double *w, const double *vprev, const double *h, const double *vnext, double *out) {
+  hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(kVecThreads), 0, A->ctx->stream, A->n, w, vprev, h, vnext,
+                     A->part, A->ticket, out);
+}
+
+int norm_host(subfv_bcsr *A, double *w, double *out) {
+  mgs_step(A, w, nullptr, nullptr, nullptr, A->scal);
+  double ss = 0.0;
+  HIPCHK(A->ctx, hipMemcpyAsync(&ss, A->scal, sizeof(double), hipMemcpyDeviceToHost, A->ctx->stream));
+  HIPCHK(A->ctx, hipStreamSynchronize(A->ctx->stream));
+  *out = sqrt(ss);
+  return 0;
+}
+
+int ensure_workspace(subfv_bcsr *A, int mr) {
+  if (A->mr >= mr) return 0;
+  for (double **p : {&A->V, &A->H}) {
+    if (*p) {
+      (void)hipFree(*p);
+      A->allocs.erase(std::find(A->allocs.begin(), A->allocs.end(), (void *)*p));
+      *p = nullptr;
+    }
+  }
+  int r;
+  if ((r = balloc(A, &A->V, (size_t)A->n * (mr + 1)))) return r;
+  if ((r = balloc(A, &A->H, (size_t)(mr + 1) * mr + 8))) return r;
+  A->mr = mr;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int32_t nnzb, const int32_t *row,
+                      const int32_t *col, const double *v) {
+  if (!c || !out || !row || !col || !v || m <= 0 || nnzb <= 0) return fail(c, SUBFV_E_INVALID, "bad BCSR arguments");
+  if (b < 3 || b > 5) return fail(c, SUBFV_E_INVALID, "block size must be 3, 4 or 5");
+  if (row[0] != 1 || row[m] != nnzb + 1) return fail(c, SUBFV_E_INVALID, "row(1) must be 1 and row(m+1) nnzb+1");
+  std::vector<int> r0(m + 1), c0(nnzb), di(m, -1);
+  for (int i = 0; i <= m; i++) r0[i] = row[i] - 1;
+  for (int i = 0; i < m; i++) {
+    if (r0[i + 1] < r0[i]) return fail(c, SUBFV_E_INVALID, "row offsets decrease");
+    for (int j = r0[i]; j < r0[i + 1]; j++) {
+      c0[j] = col[j] - 1;
+      if (c0[j] < 0 || c0[j] >= m) return fail(c, SUBFV_E_INVALID, "column id out of range");
+      if (c0[j] == i && di[i] < 0) di[i] = j;  // first diagonal block, as bcsr_diag's exit
+    }
+    if (di[i] < 0) return fail(c, SUBFV_E_INVALID, "block row without a diagonal block");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  auto *A = new subfv_bcsr();
+  A->ctx = c;
+  A->m = m;
+  A->b = b;
+  A->nnzb = nnzb;
+  A->n = (long)m * b;
+  size_t nv = (size_t)nnzb * b * b;
+  std::vector<float> v32(nv);
+  for (size_t q = 0; q < nv; q++) v32[q] = (float)v[q];
+  int r = 0;
+  auto up = [&](auto **p, const auto *src, size_t cnt) {
+    if (r) return;
+    r = balloc(A, p, cnt);
+    if (!r && hipMemcpy(*p, src, cnt * sizeof(**p), hipMemcpyHostToDevice) != hipSuccess)
+      r = fail(c, SUBFV_E_DEVICE, "BCSR upload failed");
+  };
+  up(&A->row, r0.data(), r0.size());
+  up(&A->col, c0.data(), c0.size());
+  up(&A->diag_idx, di.data(), di.size());
+  up(&A->v, v, nv);
+  up(&A->v32, v32.data(), nv);
+  if (!r) r = balloc(A, &A->F64, (size_t)m * b * b);
+  if (!r) r = balloc(A, &A->F32, (size_t)m * b * b);
+  for (double **p : {&A->w, &A->r, &A->x, &A->rhs, &A->tmp, &A->zp})
+    if (!r) r = balloc(A, p, (size_t)A->n);
+  if (!r) r = balloc(A, &A->scal, 16);
+  if (!r) r = balloc(A, &A->z, 256);
+  if (!r) r = balloc(A, &A->part, kRedBlocks);
+  if (!r) r = balloc(A, &A->ticket, 1);
+  if (r) {
+    subfv_bcsr_destroy(A);
+    return r;
+  }
+  (void)hipMemset(A->ticket, 0, sizeof(unsigned int));
+  dim3 g(nblk(m, 256)), bl(256);
+  hipStream_t s = c->stream;
+  switch (b) {
+    case 3:
+      hipLaunchKernelGGL((k_bcsr_factor<double, 3>), g, bl, 0, s, m, A->diag_idx, A->v, A->F64);
+      hipLaunchKernelGGL((k_bcsr_factor<float, 3>), g, bl, 0, s, m, A->diag_idx, A->v, A->F32);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_bcsr_factor<double, 4>), g, bl, 0, s, m, A->diag_idx, A->v, A->F64);
+      hipLaunchKernelGGL((k_bcsr_factor<float, 4>), g, bl, 0, s, m, A->diag_idx, A->v, A->F32);
+      break;
+    default:
+      hipLaunchKernelGGL((k_bcsr_factor<double, 5>), g, bl, 0, s, m, A->diag_idx, A->v, A->F64);
+      hipLaunchKernelGGL((k_bcsr_factor<float, 5>), g, bl, 0, s, m, A->diag_idx, A->v, A->F32);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(s));
+  *out = A;
+  return 0;
+}
+
+void subfv_bcsr_destroy(subfv_bcsr *A) {
+  if (!A) return;
+  if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+  for (void *p : A->allocs) (void)hipFree(p);
+  delete A;
+}
+
+int subfv_bcsr_matmul(subfv_bcsr *A, const double *x, double *y) {
+  if (!A || !x || !y) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  HIPCHK(c, hipMemcpyAsync(A->x, x, A->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int r = spmv(A, false, A->x, A->tmp);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(y, A->tmp, A->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs) {
+  if (!A || !x || !rhs) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  HIPCHK(c, hipMemcpyAsync(A->rhs, rhs, A->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int r = bjacobi(A, false, A->rhs, A->tmp);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(x, A->tmp, A->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// bcsr_gmres_diag (sparse_bcsr_linear_module.F90:1592-1691) with the Arnoldi
+// process of modifiedArnoldi_diag (:1568-1590) on the device and the
+// Givens rotations / back substitution on the host in the reference order.
+int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x_out, const double *rhs, double tol, int32_t mr,
+                          int32_t precision, int32_t *conv_out, double *stats) {
+  if (!A || !x_out || !rhs || mr < 1 || mr > 200) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  hipStream_t s = c->stream;
+  const bool fp32 = (precision & 1) != 0;
+  const int maxit = 40;
+  int r = ensure_workspace(A, mr);
+  if (r) return r;
+  const long n = A->n;
+  const int ldh = mr + 1;
+  auto Vc = [&](int j) { return A->V + (long)(j - 1) * n; };  // 1-based column
+  auto Hd = [&](int i, int j) { return A->H + (i - 1) + (long)ldh * (j - 1); };
+  dim3 vg(kRedBlocks), vb(kVecThreads);
+  long nspmv = 0;
+  HIPCHK(c, hipMemcpyAsync(A->rhs, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(A->x, 0, n * sizeof(double), s));
+  if ((r = spmv(A, false, A->x, A->tmp))) return r;
+  nspmv++;
+  hipLaunchKernelGGL(k_sub, vg, vb, 0, s, n, A->rhs, A->tmp, A->r);
+  double rho = 0.0;
+  if ((r = norm_host(A, A->r, &rho))) return r;
+  int conv = 0, jrest = 0;
+  std::vector<double> H((size_t)ldh * mr), y(mr + 2), cs(mr + 2), sn(mr + 2), z(mr + 2);
+  if (rho < tol) {
+    conv = 1;
+  } else {
+    for (jrest = 1; jrest <= maxit; jrest++) {
+      std::fill(y.begin(), y.end(), 0.0);
+      y[1] = rho;
+      HIPCHK(c, hipMemcpyAsync(A->scal, &rho, sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
+      for (int j = 2; j <= mr + 1; j++) {
+        if ((r = bjacobi(A, fp32, Vc(j - 1), A->tmp))) return r;
+        if ((r = spmv(A, fp32, A->tmp, Vc(j)))) return r;
+        nspmv++;
+        double *w = Vc(j);
+        mgs_step(A, w, nullptr, nullptr, Vc(1), Hd(1, j - 1));
+        for (int i = 2; i <= j - 1; i++) mgs_step(A, w, Vc(i - 1), Hd(i - 1, j - 1), Vc(i), Hd(i, j - 1));
+        mgs_step(A, w, Vc(j - 1), Hd(j - 1, j - 1), nullptr, A->scal + 1);
+        hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, w, w, A->scal + 1, 1, Hd(j, j - 1));
+      }
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(H.data(), A->H, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(c, hipStreamSynchronize(s));
+      auto h = [&](int i, int j) -> double & { return H[(i - 1) + (size_t)ldh * (j - 1)]; };
+      int nr = mr, i, exited = 0;
+      for (i = 1; i <= mr; i++) {
+        for (int k = 2; k <= i; k++) {
+          double t = h(k - 1, i);
+          h(k - 1, i) = cs[k - 1] * h(k - 1, i) + sn[k - 1] * h(k, i);
+          h(k, i) = -sn[k - 1] * t + cs[k - 1] * h(k, i);
+        }
+        double delta = sqrt(h(i, i) * h(i, i) + h(i + 1, i) * h(i + 1, i));
+        cs[i] = h(i, i) / delta;
+        sn[i] = h(i + 1, i) / delta;
+        h(i, i) = cs[i] * h(i, i) + sn[i] * h(i + 1, i);
+        for (int k = i + 1; k <= mr + 1; k++) h(k, i) = 0.0;
+        y[i + 1] = -sn[i] * y[i];
+        y[i] = cs[i] * y[i];
+        rho = fabs(y[i + 1]);
+        if (rho < tol) {
+          nr = i + 1;
+          exited = 1;
+          break;
+        }
+      }
+      if (!exited) i = mr + 1;
+      std::fill(z.begin(), z.end(), 0.0);
+      if (i >= mr) {
+        nr = mr;
+        z[nr - 1] = y[nr] / h(nr, nr);
+      }
+      for (int k = nr - 1; k >= 1; k--) {
+        z[k - 1] = y[k];
+        for (int l = k + 1; l <= nr; l++) z[k - 1] = z[k - 1] - h(k, l) * z[l - 1];
+        z[k - 1] = z[k - 1] / h(k, k);
+      }
+      HIPCHK(c, hipMemcpyAsync(A->z, z.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_combine, vg, vb, 0, s, n, nr, A->V, n, A->z, A->r);
+      if ((r = bjacobi(A, fp32, A->r, A->zp))) return r;
+      hipLaunchKernelGGL(k_add, vg, vb, 0, s, n, A->x, A->zp);
+      if ((r = spmv(A, false, A->x, A->tmp))) return r;
+      nspmv++;
+      hipLaunchKernelGGL(k_sub, vg, vb, 0, s, n, A->rhs, A->tmp, A->r);
+      if ((r = norm_host(A, A->r, &rho))) return r;
+      if (rho < tol) {
+        conv = 1;
+        break;
+      }
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(x_out, A->x, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (conv_out) *conv_out = conv;
+  if (stats) {
+    stats[0] = (double)std::min(jrest, maxit);
+    stats[1] = rho;
+    stats[2] = (double)nspmv;
+  }
+  return conv ? 0 : fail(c, SUBFV_E_NOT_CONVERGED, "GMRES did not converge");
+}
+
+}  // extern "C"

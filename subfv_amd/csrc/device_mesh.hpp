@@ -1,0 +1,47 @@
+// Device-resident mesh: structure-of-arrays geometry and the node-sorted
+// sub-face / sub-element CSR the kernels stream through.  0-based ids.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfv {
+
+constexpr int kWaveSub = 64;   // sub-faces per wavefront (one lane each)
+constexpr int kWaveVert = 16;  // vertices per wavefront at most
+
+struct DevMesh {
+  long V = 0, C = 0, F = 0, SF = 0, SE = 0;
+  int nwaves = 0;
+  // vertices
+  double *vcoord = nullptr;   // [V][3]
+  double *vvol = nullptr;     // [V]
+  uint8_t *vbound = nullptr;  // [V]
+  int *sf_off = nullptr;      // [V+1] sub-faces of v: [sf_off[v], sf_off[v+1])
+  int *se_off = nullptr;      // [V+1] node-sorted sub-element slots of v
+  int *v_eoff = nullptr, *v_elem = nullptr;  // vertex -> cells (reference order)
+  int *v_seid = nullptr;      // [SE] node-sorted slot -> cell-major sub-element id
+  double *Bp = nullptr;       // [3][V] wall normal (depends on the BC set)
+  int *wave_v = nullptr;      // [nwaves+1] first vertex of each wavefront
+  // faces
+  int *f_voff = nullptr, *f_vert = nullptr, *f_left = nullptr, *f_right = nullptr;
+  int *f_sfoff = nullptr, *f_sf = nullptr;
+  double *fcoord = nullptr;   // [F][3]
+  double *fnorm = nullptr;    // [F][3]
+  double *farea = nullptr;    // [F]
+  // sub-faces (vertex-major)
+  int *sf_vert = nullptr, *sf_face = nullptr;
+  int *sf_le = nullptr;       // left cell
+  int *sf_re = nullptr;       // right code: >=0 cell, -1 wall, <=-2 BC (-code-1)
+  double *sfn = nullptr;      // [4][SF]: nx, ny, nz, area
+  // sub-elements
+  int *se_sfoff = nullptr, *se_sf = nullptr, *se_elem = nullptr;  // cell-major ids
+  double *se_vol = nullptr;   // [SE] cell-major
+  int *se_cell = nullptr;     // [SE] node-sorted slot -> cell
+  // cells
+  int *e_voff = nullptr, *e_vert = nullptr, *e_foff = nullptr, *e_face = nullptr;
+  int *c_slot = nullptr;      // [SE] node-sorted slots of each cell, ascending vertex
+  double *ccoord = nullptr;   // [C][3]
+  double *cvol = nullptr;     // [C]
+};
+
+}  // namespace sfv

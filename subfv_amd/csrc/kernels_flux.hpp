@@ -1,0 +1,349 @@
+// Node-centred subface sweep: compute_euler_flux / compute_rhs_around_vert
+// (src/ns/ns_euler_module.F90:151-339, 341-519) with the nodal solvers of
+// src/ns/ns_euler_rs_module.F90:288-553, 681-758.
+//
+// Mapping: one wavefront owns a run of consecutive vertices whose sub-faces
+// (contiguous, vertex-major) fit in its 64 lanes; lane = sub-face.  Every
+// per-vertex sum (nodal matrix, right-hand side, per-sub-element flux) is
+// formed by one lane walking the vertex's sub-faces in reference order
+// through LDS, so results are deterministic and atomic-free.  The output is
+// one row of partial sums per node-sorted sub-element slot:
+// P[k][slot] = sum of area*flux_k (k < 5) and P[5][slot] = sum of
+// area*max(0, -s) over that vertex's sub-faces of that sub-element.
+#pragma once
+#include "device_mesh.hpp"
+#include "physics.hpp"
+
+namespace sfv {
+
+struct FluxArgs {
+  DevMesh m;
+  BCParams bc;
+  const double *U;      // [5][C]
+  const double *G;      // [15][C] cell gradients, (3*k + d) major (second order)
+  double *P;            // [6][SE]
+  double *vp;           // [3][V] nodal velocity output or null
+  int *err;
+  const double *t;      // physical time (time-dependent BCs)
+  int second_order;
+  int per_xcd;          // logical blocks per XCD slice
+  int nblocks;          // logical blocks
+};
+
+SFV_DEV void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// blockIdx -> logical block so that each XCD gets a contiguous slice of the
+// (spatially ordered) work and its L2 sees neighbouring cells.
+SFV_DEV int xcd_block(int per_xcd) {
+  int b = blockIdx.x;
+  return (b & 7) * per_xcd + (b >> 3);
+}
+
+constexpr int kFluxWaves = 4;
+
+template <int SCHEME>
+__global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
+  __shared__ double s_red[kFluxWaves][12][kWaveSub];
+  __shared__ double s_sum[kFluxWaves][kWaveVert][9];
+  __shared__ double s_vn[kFluxWaves][kWaveVert][3];
+  __shared__ int s_vtab[kFluxWaves][kWaveVert + 1];
+  __shared__ int s_setab[kFluxWaves][kWaveVert + 1];
+  __shared__ int s_lslot[kFluxWaves][kWaveSub];
+  __shared__ int s_rslot[kFluxWaves][kWaveSub];
+
+  const DevMesh &m = a.m;
+  const BCParams &bc = a.bc;
+  const int wib = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int lb = xcd_block(a.per_xcd);
+  if (lb >= a.nblocks) return;
+  const int w = lb * kFluxWaves + wib;
+  if (w >= m.nwaves) return;  // whole wavefront leaves; only wave-level sync below
+
+  const int vb = m.wave_v[w], ve = m.wave_v[w + 1];
+  const int nvw = ve - vb;
+  const int sf0 = m.sf_off[vb], se0 = m.se_off[vb];
+  if (lane <= nvw) {
+    s_vtab[wib][lane] = m.sf_off[vb + lane] - sf0;
+    s_setab[wib][lane] = m.se_off[vb + lane] - se0;
+  }
+  wsync();
+  const int nsfw = s_vtab[wib][nvw];
+  const bool active = lane < nsfw;
+  int vl = 0;
+  for (int i = 1; i < nvw; i++)
+    if (s_vtab[wib][i] <= lane) vl = i;
+  const int v = vb + vl;
+  // inactive lanes mirror a valid sub-face of the wave so every load is in
+  // bounds; their contributions are masked below
+  const long s = (nsfw > 0) ? sf0 + (active ? lane : 0) : 0;
+
+  // ---- sub-face data and left/right primitive states (reconstruct_lr_w)
+  double n[3], area;
+  n[0] = m.sfn[s];
+  n[1] = m.sfn[m.SF + s];
+  n[2] = m.sfn[2 * m.SF + s];
+  area = m.sfn[3 * m.SF + s];
+  const int le = m.sf_le[s];
+  const int re = m.sf_re[s];
+  const long C = m.C;
+  double wl[5], wr[5];
+  {
+    double u[5];
+    for (int k = 0; k < 5; k++) u[k] = a.U[k * C + le];
+    to_prim(u, wl);
+    if (a.second_order) {
+      double dx[3];
+      for (int d = 0; d < 3; d++) dx[d] = m.vcoord[3L * v + d] - m.ccoord[3L * le + d];
+      for (int k = 0; k < 5; k++)
+        wl[k] = wl[k] + (a.G[(3 * k) * C + le] * dx[0] + a.G[(3 * k + 1) * C + le] * dx[1] +
+                         a.G[(3 * k + 2) * C + le] * dx[2]);
+    }
+    if (re >= 0) {
+      for (int k = 0; k < 5; k++) u[k] = a.U[k * C + re];
+      to_prim(u, wr);
+      if (a.second_order) {
+        double dx[3];
+        for (int d = 0; d < 3; d++) dx[d] = m.vcoord[3L * v + d] - m.ccoord[3L * re + d];
+        for (int k = 0; k < 5; k++)
+          wr[k] = wr[k] + (a.G[(3 * k) * C + re] * dx[0] + a.G[(3 * k + 1) * C + re] * dx[1] +
+                           a.G[(3 * k + 2) * C + re] * dx[2]);
+      }
+    } else {
+      double ul[5], ug[5];
+      to_cons(wl, ul);
+      const double *fc = &m.fcoord[3L * m.sf_face[s]];
+      int e = ghost_state(bc, re, n, fc, a.t ? *a.t : 0.0, ul, ug);
+      if (e && active) atomicOr(a.err, e);
+      to_prim(ug, wr);
+    }
+  }
+  const bool wall = is_wall(bc, re);
+  const bool excl = bc.excl && m.vbound[v];
+
+  // ---- nodal solve
+  double lam_l = 0.0, lam_r = 0.0, vbar = 0.0;
+  double v_node[3] = {0.0, 0.0, 0.0};
+  double p_nodal = 0.0;
+  double warea = 1.0;
+  if (SCHEME == kMultiPoint || SCHEME == kMultiPointIso) {
+    const bool iso = (SCHEME == kMultiPointIso);
+    const double rhol = wl[0], pl = wl[4], al = sound_speed(wl);
+    const double rhor = wr[0], pr = wr[4], ar = sound_speed(wr);
+    const double vn_l = wl[1] * n[0] + wl[2] * n[1] + wl[3] * n[2];
+    const double vn_r = wr[1] * n[0] + wr[2] * n[1] + wr[3] * n[2];
+    for (int iter = 1; iter <= 4; iter++) {
+      if (iter == 1) {
+        lam_l = dmax(dmax(dmax(lam_l, al * rhol), sqrt(rhol * dmax(0.0, pr - pl))), -rhol * (vn_r - vn_l));
+        lam_r = dmax(dmax(dmax(lam_r, ar * rhor), sqrt(rhor * dmax(0.0, pl - pr))), -rhor * (vn_r - vn_l));
+        vbar = (lam_l * vn_l + lam_r * vn_r - (pr - pl)) / (lam_r + lam_l);
+        lam_l = dmax(lam_l, al * rhol * (1.0 + 1.5 * dmax(0.0, -(vbar - vn_l) / al)));
+        lam_r = dmax(lam_r, ar * rhor * (1.0 + 1.5 * dmax(0.0, (vbar - vn_r) / ar)));
+      } else if (wall && bc.bc_style == 1) {
+        lam_l = dmax(lam_l, al * rhol * (1.0 + 1.5 * dmax(0.0, -(vbar - vn_l) / al)));
+        lam_r = dmax(lam_r, ar * rhor * (1.0 + 1.5 * dmax(0.0, (vbar - vn_r) / ar)));
+      } else {
+        double v_et = v_node[0] * n[0] + v_node[1] * n[1] + v_node[2] * n[2];
+        lam_l = dmax(lam_l, al * rhol * (1.0 + 1.5 * dmax(0.0, -(v_et - vn_l) / al)));
+        lam_r = dmax(lam_r, ar * rhor * (1.0 + 1.5 * dmax(0.0, (v_et - vn_r) / ar)));
+      }
+      vbar = (lam_l * vn_l + lam_r * vn_r - (pr - pl)) / (lam_r + lam_l);
+      // contributions to the nodal matrix (6 distinct entries) and Rp
+      double cm = iso ? (lam_r + lam_l) : (lam_r + lam_l) * area;
+      double cr = iso ? (lam_r + lam_l) * vbar : area * (lam_r + lam_l) * vbar;
+      bool addm = true, addr = true;
+      if (wall) {
+        if (bc.bc_style == 0 || bc.bc_style == 2) {
+          if (bc.bc_style == 2) {
+            addr = bc.b2d ? (fabs(n[2]) < 1e-8) : true;
+            cr = iso ? pl : area * pl;
+          }
+        } else {
+          addm = addr = false;
+        }
+      }
+      if (!active || excl) addm = addr = false;
+      double c9[9];
+      c9[0] = addm ? cm * (n[0] * n[0]) : 0.0;
+      c9[1] = addm ? cm * (n[0] * n[1]) : 0.0;
+      c9[2] = addm ? cm * (n[0] * n[2]) : 0.0;
+      c9[3] = addm ? cm * (n[1] * n[1]) : 0.0;
+      c9[4] = addm ? cm * (n[1] * n[2]) : 0.0;
+      c9[5] = addm ? cm * (n[2] * n[2]) : 0.0;
+      c9[6] = addr ? cr * n[0] : 0.0;
+      c9[7] = addr ? cr * n[1] : 0.0;
+      c9[8] = addr ? cr * n[2] : 0.0;
+      for (int c = 0; c < 9; c++) s_red[wib][c][lane] = c9[c];
+      wsync();
+      for (int p = lane; p < nvw * 9; p += 64) {
+        int vv = p / 9, c = p - vv * 9;
+        double acc = 0.0;
+        for (int j = s_vtab[wib][vv]; j < s_vtab[wib][vv + 1]; j++) acc = acc + s_red[wib][c][j];
+        s_sum[wib][vv][c] = acc;
+      }
+      wsync();
+      if (lane < nvw) {
+        const int vv = vb + lane;
+        double vn3[3] = {0.0, 0.0, 0.0};
+        if (!(bc.excl && m.vbound[vv])) {
+          const double *S = s_sum[wib][lane];
+          double M[9] = {S[0], S[1], S[2], S[1], S[3], S[4], S[2], S[4], S[5]};
+          double R[3] = {S[6], S[7], S[8]};
+          if (bc.bc_style == 2) {
+            double B[3] = {m.Bp[vv], m.Bp[m.V + vv], m.Bp[2 * m.V + vv]};
+            double mb = dmax(dmax(fabs(B[0]), fabs(B[1])), fabs(B[2]));
+            if (mb > 1e-8) {
+              pinv3(M);
+              double mR[3], mB[3];
+              for (int i = 0; i < 3; i++) {
+                mR[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
+                mB[i] = M[i] * B[0] + M[i + 3] * B[1] + M[i + 6] * B[2];
+              }
+              double pb = (mR[0] * B[0] + mR[1] * B[1] + mR[2] * B[2]) / (mB[0] * B[0] + mB[1] * B[1] + mB[2] * B[2]);
+              double rr[3];
+              for (int d = 0; d < 3; d++) rr[d] = R[d] - pb * B[d];
+              for (int i = 0; i < 3; i++) vn3[i] = M[i] * rr[0] + M[i + 3] * rr[1] + M[i + 6] * rr[2];
+              double sc = (vn3[0] * B[0] + vn3[1] * B[1] + vn3[2] * B[2]) / (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+              for (int d = 0; d < 3; d++) vn3[d] = vn3[d] - sc * B[d];
+            } else {
+              lu_solve<3>(M, R, vn3);
+            }
+          } else {
+            pinv3(M);
+            for (int i = 0; i < 3; i++) vn3[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
+          }
+        }
+        for (int d = 0; d < 3; d++) s_vn[wib][lane][d] = vn3[d];
+      }
+      wsync();
+      for (int d = 0; d < 3; d++) v_node[d] = s_vn[wib][vl][d];
+    }
+    if (a.vp && lane < nvw)
+      for (int d = 0; d < 3; d++) a.vp[d * m.V + vb + lane] = s_vn[wib][lane][d];
+    if (iso) {
+      // warea = minval(area)/area over the vertex's sub-faces
+      s_red[wib][0][lane] = active ? area : 1e300;
+      wsync();
+      if (lane < nvw) {
+        double mn = s_red[wib][0][s_vtab[wib][lane]];
+        for (int j = s_vtab[wib][lane] + 1; j < s_vtab[wib][lane + 1]; j++) mn = dmin(mn, s_red[wib][0][j]);
+        s_sum[wib][lane][0] = mn;
+      }
+      wsync();
+      warea = s_sum[wib][vl][0] / area;
+      wsync();
+    }
+  } else if (SCHEME == kMultiPointPressure || SCHEME == kMultiPointPressurePh) {
+    const double rhol = wl[0], pl = wl[4], al = sound_speed(wl);
+    const double rhor = wr[0], pr = wr[4], ar = sound_speed(wr);
+    const double vn_l = wl[1] * n[0] + wl[2] * n[1] + wl[3] * n[2];
+    const double vn_r = wr[1] * n[0] + wr[2] * n[1] + wr[3] * n[2];
+    const bool w1 = (bc.bc_style == 1 && wall);
+    for (int iter = 1; iter <= 10; iter++) {
+      if (iter == 1) {
+        lam_l = dmax(dmax(dmax(lam_l, al * rhol), sqrt(rhol * dmax(0.0, pr - pl))), -rhol * (vn_r - vn_l));
+        lam_r = dmax(dmax(dmax(lam_r, ar * rhor), sqrt(rhor * dmax(0.0, pl - pr))), -rhor * (vn_r - vn_l));
+        vbar = (lam_r * pl + lam_l * pr - lam_r * lam_l * (vn_r - vn_l)) / (lam_r + lam_l);
+        lam_l = dmax(lam_l, al * rhol + 1.5 * sqrt(dmax(0.0, (vbar - pl) * rhol)));
+        lam_r = dmax(lam_r, ar * rhor + 1.5 * sqrt(dmax(0.0, (vbar - pr) * rhor)));
+      } else if (w1) {
+        lam_l = dmax(lam_l, al * rhol + 1.5 * sqrt(dmax(0.0, (vbar - pl) * rhol)));
+        lam_r = dmax(lam_r, ar * rhor + 1.5 * sqrt(dmax(0.0, (vbar - pr) * rhor)));
+      } else {
+        lam_l = dmax(lam_l, al * rhol + 1.5 * sqrt(dmax(0.0, (p_nodal - pl) * rhol)));
+        lam_r = dmax(lam_r, ar * rhor + 1.5 * sqrt(dmax(0.0, (p_nodal - pr) * rhor)));
+      }
+      vbar = (lam_r * pl + lam_l * pr - lam_r * lam_l * (vn_r - vn_l)) / (lam_r + lam_l);
+      double c1, c2;
+      if (w1) {
+        c1 = (1.0 / lam_r + 1.0 / lam_l) * area * (lam_r * pl + lam_r * lam_l * vn_l) / (lam_r + lam_l);
+        c2 = (1.0 / lam_l) * area;
+      } else {
+        c1 = (1.0 / lam_r + 1.0 / lam_l) * area * vbar;
+        c2 = (1.0 / lam_r + 1.0 / lam_l) * area;
+      }
+      s_red[wib][0][lane] = active ? c1 : 0.0;
+      s_red[wib][1][lane] = active ? c2 : 0.0;
+      wsync();
+      if (lane < nvw) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int j = s_vtab[wib][lane]; j < s_vtab[wib][lane + 1]; j++) {
+          a1 = a1 + s_red[wib][0][j];
+          a2 = a2 + s_red[wib][1][j];
+        }
+        s_sum[wib][lane][0] = a1 / a2;
+      }
+      wsync();
+      p_nodal = s_sum[wib][vl][0];
+      wsync();
+    }
+  }
+
+  // ---- sub-face flux
+  double fx[10], sl = 0.0, sr = 0.0;
+  if (SCHEME == kMultiPoint || SCHEME == kMultiPointIso) {
+    if (excl) {
+      flux_two_wave(wl, wr, n, fx, sl, sr);
+    } else {
+      double vn = wall ? 0.0 : (v_node[0] * n[0] + v_node[1] * n[1] + v_node[2] * n[2]);
+      bool ok = flux_multi_point(wl, wr, n, fx, vn, lam_l, lam_r, sl, sr);
+      if (!ok && active) atomicOr(a.err, (int)kErrNegStarDensity);
+    }
+    if (SCHEME == kMultiPointIso) {
+      double f3[10];
+      flux_three_wave<false>(wl, wr, n, f3, sl, sr);
+      for (int k = 0; k < 10; k++) fx[k] = warea * fx[k] + (1.0 - warea) * f3[k];
+    }
+  } else if (SCHEME == kThreeWave) {
+    flux_three_wave<false>(wl, wr, n, fx, sl, sr);
+  } else if (SCHEME == kModifiedThreeWave) {
+    flux_three_wave<true>(wl, wr, n, fx, sl, sr);
+  } else if (SCHEME == kTwoWave) {
+    flux_two_wave(wl, wr, n, fx, sl, sr);
+  } else if (SCHEME == kMultiPointPressure) {
+    flux_multi_point_pressure<false>(wl, wr, n, fx, p_nodal, lam_l, lam_r, sl, sr);
+  } else {
+    flux_multi_point_pressure<true>(wl, wr, n, fx, p_nodal, lam_l, lam_r, sl, sr);
+  }
+
+  // ---- accumulate area*flux per sub-element slot of the vertex
+  int lslot = -1, rslot = -1;
+  if (active) {
+    const int sb = m.se_off[v], nse = m.se_off[v + 1] - sb;
+    for (int k = 0; k < nse; k++) {
+      int c = m.se_cell[sb + k];
+      if (c == le) lslot = k;
+      if (c == re) rslot = k;
+    }
+    if (re < 0) rslot = -1;
+  }
+  for (int k = 0; k < 5; k++) {
+    s_red[wib][k][lane] = area * fx[k];
+    s_red[wib][6 + k][lane] = area * fx[5 + k];
+  }
+  s_red[wib][5][lane] = area * dmax(0.0, -sl);
+  s_red[wib][11][lane] = area * dmax(0.0, sr);
+  s_lslot[wib][lane] = lslot;
+  s_rslot[wib][lane] = rslot;
+  wsync();
+  const int nsew = s_setab[wib][nvw];
+  for (int p = lane; p < 6 * nsew; p += 64) {
+    int comp = p / nsew, q = p - comp * nsew;
+    int vv = 0;
+    for (int i = 1; i < nvw; i++)
+      if (s_setab[wib][i] <= q) vv = i;
+    int k = q - s_setab[wib][vv];
+    double acc = 0.0;
+    for (int j = s_vtab[wib][vv]; j < s_vtab[wib][vv + 1]; j++) {
+      if (s_lslot[wib][j] == k) acc = acc + s_red[wib][comp][j];
+      if (s_rslot[wib][j] == k) acc = acc + s_red[wib][6 + comp][j];
+    }
+    a.P[(long)comp * m.SE + se0 + q] = acc;
+  }
+}
+
+}  // namespace sfv
