@@ -1,0 +1,339 @@
+"""Pins for the oracle (CPU): every check is an independent fact (a library
+routine, a closed form, an invariant or a textbook algorithm), chosen so that
+a dropped term, a wrong sign or a transposed operand in the restatement
+fails one of them."""
+import collections
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from exact_riemann import sample
+from subfv_amd import meshgen as mg
+
+SCHEMES = list(O.SCHEMES)
+
+
+# ----------------------------------------------------------------- dense
+def _lu(A, b):
+    L = O.lib()
+    fn = L.om_test_lu_solve
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    fn.argtypes = [C.c_int, dp, dp, dp]
+    x = np.zeros(len(b))
+    fn(len(b), np.ascontiguousarray(A.T), np.ascontiguousarray(b), x)  # column-major
+    return x
+
+
+def _pinv(A):
+    fn = O.lib().om_test_pinv3
+    fn.argtypes = [np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+    cm = np.ascontiguousarray(A.T).copy()
+    fn(cm)
+    return cm.reshape(3, 3).T
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_lu_solve_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    for _ in range(20):
+        A = rng.standard_normal((n, n)) + n * np.eye(n) * rng.random()
+        A[0, 0] = 1e-3 * A[0, 0]  # forces a row interchange
+        b = rng.standard_normal(n)
+        x = _lu(A, b)
+        ref = np.linalg.solve(A, b)
+        assert np.allclose(x, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+
+
+def test_lu_solve_on_scaled_identity_is_division():
+    # the update of add_timedisc_and_solve: LU of c*I is exactly b / c
+    c = 3.7e-5
+    b = np.array([1.1, -2.3, 0.0, 7.9e-9, 5.5])
+    assert np.array_equal(_lu(c * np.eye(5), b), b / c)
+
+
+def test_pinv3_matches_numpy_including_cutoff():
+    rng = np.random.default_rng(0)
+    for k in range(30):
+        Q1, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        Q2, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        s = np.array([1.0 + rng.random(), 0.1 + rng.random(), [1.0, 1e-13, 1e-11, 0.0][k % 4]])
+        A = Q1 @ np.diag(s) @ Q2.T
+        # the reference drops singular values <= 3 * 1e-12 * max
+        ref = np.linalg.pinv(A, rcond=3e-12)
+        got = _pinv(A)
+        kept = s[s > 3e-12 * s.max()]
+        rtol = max(1e-12, 50 * 2.2e-16 * kept.max() / kept.min())  # conditioning of the kept part
+        assert np.allclose(got, ref, rtol=rtol, atol=rtol * np.abs(ref).max()), (k, s)
+        assert (np.abs(got).max() > 1e9) == (s[2] == 1e-11)  # 1e-11 is kept, 1e-13 and 0 dropped
+
+
+def test_lu_solve_noswp_matches_numpy():
+    fn = O.lib().om_test_lu_solve_noswp
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    fn.argtypes = [C.c_int, dp, dp, dp]
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((4, 4)) + 8 * np.eye(4)
+    b = rng.standard_normal(4)
+    x = np.zeros(4)
+    fn(4, np.ascontiguousarray(A.T), b, x)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-12)
+
+
+def test_max_omega_for_positivity_closed_form():
+    U = mg.primit_to_conserv(np.array([1.0, 0.3, -0.2, 0.1, 1.0]))
+    assert O.max_omega_for_positivity(U, 0.5 * U) == 1.0
+    # U + w*dU = (1 - 2w) U is positive iff w < 0.5: first 0.9^k below 0.5
+    om = 1.0
+    while not (1.0 - 2.0 * om > 0):
+        om *= 0.9
+    expect = 1.0
+    for _ in range(7):
+        expect *= 0.9
+    assert om == expect
+    assert O.max_omega_for_positivity(U, -2.0 * U) == expect
+
+
+# -------------------------------------------------------------- residual
+def _uniform(n, w):
+    return np.tile(mg.primit_to_conserv(np.asarray(w, float)), (n, 1))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("mesh", ["quad", "tri", "tet"])
+def test_free_stream_preservation(scheme, mesh):
+    """A uniform state at rest between walls is an exact steady state."""
+    m = {"quad": lambda: mg.quad_mesh(7), "tri": lambda: mg.tri_mesh(6), "tet": lambda: mg.tet_mesh(3)}[mesh]()
+    om = O.OracleMesh(m)
+    cfg = O.make_config(scheme, boundary_2d=m.boundary_2d)
+    sol = _uniform(om.n_elems, [1.3, 0.0, 0.0, 0.0, 0.7])
+    rhs, sl = om.compute_euler_flux(cfg, sol)
+    scale = 0.7 * om.getd("sub_face_area").max()
+    assert np.abs(rhs).max() <= 1e-12 * scale, np.abs(rhs).max() / scale
+    assert np.all(sl > 0)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_uniform_flow_with_freestream_ends(scheme):
+    """Uniform flow along x, freestream at x = 0 and x = 1, slip walls
+    elsewhere (parallel to the flow): zero residual."""
+    m = mg.quad_mesh(8, tags="lr")
+    om = O.OracleMesh(m)
+    w = [1.0, 0.4, 0.0, 0.0, 1.0]
+    cfg = O.make_config(scheme, bcs=[("freestream", w), ("freestream", w)])
+    rhs, _ = om.compute_euler_flux(cfg, _uniform(om.n_elems, w))
+    scale = om.getd("sub_face_area").max()
+    assert np.abs(rhs).max() <= 1e-12 * scale
+
+
+def test_nodal_velocity_equals_uniform_velocity():
+    m = mg.tri_mesh(8)
+    om = O.OracleMesh(m)
+    cfg = O.make_config("multi_point")
+    u = [1.0, 0.25, -0.15, 0.0, 1.0]
+    rhs, sl, vp = om.compute_euler_flux(cfg, _uniform(om.n_elems, u), want_vp=True)
+    interior = om.geti("vert_is_bound") == 0
+    assert interior.sum() > 50
+    assert np.allclose(vp[interior], [0.25, -0.15, 0.0], atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme", ["multi_point", "two_wave", "three_wave", "multi_point_pressure"])
+def test_conservation_with_walls(scheme):
+    m = mg.tri_mesh(12)
+    om = O.OracleMesh(m)
+    vol = om.elem_volume
+    sol = mg.init_1drp(om.elem_coord, 0.45, [1, 0, 0, 0, 1], [0.125, 0, 0, 0, 0.1])
+    st, out, info = om.run(O.make_config(scheme, cfl=0.4), sol, 30)
+    assert st == 0
+    # mass: every flux is antisymmetric across a sub-face and zero through a
+    # wall; energy: exact only for the per-sub-face conservative fluxes (the
+    # nodal schemes balance energy per node, and at wall corners the wall
+    # normal of the nodal solve differs from the sub-face normal)
+    for k in ((0, 4) if scheme in ("two_wave", "three_wave") else (0,)):
+        m0, m1 = (sol[:, k] * vol).sum(), (out[:, k] * vol).sum()
+        assert abs(m1 - m0) <= 1e-14 * abs(m0) * 30
+
+
+def test_second_order_conserves_and_stays_positive():
+    m = mg.quad_mesh(16)
+    om = O.OracleMesh(m)
+    vol = om.elem_volume
+    sol = mg.init_1drp(om.elem_coord, 0.5, [1, 0, 0, 0, 1], [0.125, 0, 0, 0, 0.1])
+    for ts in (0, 1):
+        st, out, _ = om.run(O.make_config("multi_point", second_order=True, cfl=0.4), sol, 20, ts)
+        assert st == 0
+        assert abs((out[:, 0] * vol).sum() - (sol[:, 0] * vol).sum()) <= 1e-13
+        assert out[:, 0].min() > 0
+
+
+def _sod_error(n, scheme, second_order=False):
+    m = mg.quad_mesh(n, jitter=0.0, ly=4.0 / n)
+    om = O.OracleMesh(m)
+    xc = om.elem_coord
+    wl, wr = [1, 0, 0, 0, 1], [0.125, 0, 0, 0, 0.1]
+    sol = mg.init_1drp(xc, 0.5, wl, wr)
+    cfg = O.make_config(scheme, cfl=0.4, tmax=0.15, second_order=second_order)
+    st, out, info = om.run(cfg, sol, 10000)
+    assert st == 0 and abs(info["t"] - 0.15) < 1e-9
+    ex = sample(xc[:, 0], info["t"], 0.5, (1, 0, 1), (0.125, 0, 0.1))
+    return np.abs(out[:, 0] - ex[:, 0]).mean()
+
+
+@pytest.mark.parametrize("scheme", ["multi_point", "two_wave", "three_wave"])
+def test_sod_converges_to_exact_riemann_solution(scheme):
+    e32, e64 = _sod_error(32, scheme), _sod_error(64, scheme)
+    assert e64 < 0.03
+    assert e32 / e64 > 1.3
+
+
+def test_second_order_is_more_accurate_on_sod():
+    assert _sod_error(64, "multi_point", True) < 0.8 * _sod_error(64, "multi_point", False)
+
+
+def test_nodal_gradient_exact_for_linear_fields():
+    m = mg.tri_mesh(10)
+    om = O.OracleMesh(m)
+    xc = om.elem_coord
+    b = np.array([[0.3, -0.2, 0.0], [0.1, 0.05, 0.0], [-0.07, 0.2, 0.0], [0.0, 0.0, 0.0], [0.2, 0.1, 0.0]])
+    w = np.stack([2.0 + xc @ b[0], xc @ b[1], xc @ b[2], np.zeros(len(xc)), 2.0 + xc @ b[4]], 1)
+    grad, ng = om.compute_cell_grad(mg.primit_to_conserv(w), method=4)
+    interior = om.geti("vert_is_bound") == 0
+    assert np.allclose(ng[interior], b[None, :, :], atol=1e-11)
+    assert np.all(ng[~interior] == 0.0)
+
+
+# ---------------------------------------------------------------- mesh
+def _textbook_bfs_order(m):
+    nv = m.n_vert
+    off = mg.elem_offsets(m.kind)
+    ev = m.elem_vert - 1
+    v2e = [[] for _ in range(nv)]
+    for e in range(m.n_elems):
+        for q in range(off[e], off[e + 1]):
+            v2e[ev[q]].append(e)
+    seen = np.full(nv, False)
+    order = []
+    for start in range(nv):
+        if seen[start]:
+            continue
+        seen[start] = True
+        dq = collections.deque([start])
+        while dq:
+            v = dq.popleft()
+            order.append(v)
+            for e in v2e[v]:
+                for q in range(off[e], off[e + 1]):
+                    w = ev[q]
+                    if not seen[w]:
+                        seen[w] = True
+                        dq.append(w)
+    return np.array(order) + 1
+
+
+@pytest.mark.parametrize("mk", [lambda: mg.quad_mesh(5), lambda: mg.tri_mesh(5), lambda: mg.tet_mesh(2)])
+def test_vertex_numbering_is_breadth_first(mk):
+    m = mk()
+    om = O.OracleMesh(m)
+    assert np.array_equal(om.geti("vert_nto"), _textbook_bfs_order(m))
+
+
+def test_numbering_and_counts_invariants():
+    m = mg.tri_mesh(6)
+    om = O.OracleMesh(m)
+    fnv = om.geti("face_n_vert")
+    assert om.n_sub_faces == fnv.sum()
+    assert om.n_sub_elems == om.geti("elem_n_vert").sum()
+    # sub-faces are vertex-major: vertex of sub-face ids is non-decreasing
+    assert np.all(np.diff(om.geti("sub_face_vert")) >= 0)
+    # faces ordered by smallest vertex id
+    fv = om.geti("face_vert")
+    offs = np.concatenate([[0], np.cumsum(fnv)])
+    mins = np.array([fv[offs[i]:offs[i + 1]].min() for i in range(len(fnv))])
+    assert np.all(np.diff(mins) >= 0)
+    # elements numbered by first appearance over the vertices' element lists
+    ve = om.geti("vert_elem")
+    first = []
+    seen = set()
+    for e in ve:
+        if e not in seen:
+            seen.add(e)
+            first.append(e)
+    assert first == sorted(first)
+
+
+def test_geometry_closed_forms_on_regular_meshes():
+    n = 4
+    h = 1.0 / n
+    om = O.OracleMesh(mg.quad_mesh(n, jitter=0.0))
+    assert np.allclose(om.elem_volume, h ** 3, rtol=1e-14)
+    xc = om.elem_coord
+    assert np.allclose(np.sort(np.unique(np.round(xc[:, 0] / h - 0.5, 9))), np.arange(n))
+    assert np.allclose(xc[:, 2], 0.5 * h)
+    assert np.allclose(om.getd("sub_face_area"), h * h / 4)
+    assert abs(om.getd("vert_volume").sum() - h) < 1e-15
+    omt = O.OracleMesh(mg.tet_mesh(3))
+    assert abs(omt.elem_volume.sum() - 1.0) < 1e-14
+    assert omt.status == 0
+
+
+def test_bad_geometry_is_rejected():
+    m = mg.tet_mesh(2)
+    ev = m.elem_vert.reshape(-1, 4).copy()
+    ev[0, [1, 2]] = ev[0, [2, 1]]  # one inverted tet
+    m.elem_vert = ev.ravel()
+    om = O.OracleMesh(m)
+    assert om.status == 5  # check_mesh: "MESH GEOMETRY IS NOT OK"
+
+
+# ---------------------------------------------------------------- BCSR
+def _random_bcsr(m, b, seed, diag_shift=8.0):
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(m):
+        nb = sorted({i, (i + 1) % m, (i - 3) % m, int(rng.integers(m))})
+        for j in nb:
+            blk = rng.standard_normal((b, b)) * 0.3
+            if j == i:
+                blk += diag_shift * np.eye(b)
+            rows.append(i)
+            cols.append(j)
+            vals.append(blk)
+    row = np.zeros(m + 1, np.int32)
+    for i in rows:
+        row[i + 1] += 1
+    row = np.cumsum(row).astype(np.int32) + 1
+    v = np.stack([blk.T for blk in vals]).reshape(-1)  # column-major blocks
+    return row, np.array(cols, np.int32) + 1, v, vals, rows, cols
+
+
+def _dense(m, b, vals, rows, cols):
+    A = np.zeros((m * b, m * b))
+    for blk, i, j in zip(vals, rows, cols):
+        A[i * b:(i + 1) * b, j * b:(j + 1) * b] += blk
+    return A
+
+
+def test_bcsr_matmul_and_diag():
+    m, b = 40, 4
+    row, col, v, vals, rows, cols = _random_bcsr(m, b, 1)
+    A = _dense(m, b, vals, rows, cols)
+    x = np.random.default_rng(2).standard_normal(m * b)
+    assert np.allclose(O.bcsr_matmul(m, b, row, col, v, x), A @ x, rtol=1e-13, atol=1e-13)
+    D = np.zeros_like(A)
+    for blk, i, j in zip(vals, rows, cols):
+        if i == j:
+            D[i * b:(i + 1) * b, j * b:(j + 1) * b] = blk
+    assert np.allclose(O.bcsr_diag(m, b, row, col, v, x), np.linalg.solve(D, x), rtol=1e-12)
+
+
+def test_bcsr_gmres_matches_direct_solve():
+    m, b = 60, 4
+    row, col, v, vals, rows, cols = _random_bcsr(m, b, 3, diag_shift=3.0)
+    A = _dense(m, b, vals, rows, cols)
+    rhs = np.random.default_rng(5).standard_normal(m * b)
+    tol = 1e-10 * np.linalg.norm(rhs)
+    x, conv, info = O.bcsr_gmres_diag(m, b, row, col, v, rhs, tol, 10)
+    assert conv
+    assert np.linalg.norm(rhs - A @ x) < tol
+    assert np.allclose(x, np.linalg.solve(A, rhs), rtol=1e-8, atol=1e-9)
