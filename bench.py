@@ -1,0 +1,173 @@
+"""Benchmark of the explicit residual path on one MI355X.
+
+Default workload: 1024 x 1024 jittered quads split along seeded diagonals
+(2,097,152 one-layer prisms), FP64, second-order reconstruction with
+limited gradients (method 4), multi_point nodal-solver fluxes, SSP-RK3, a
+seeded vortex initial state.  A "step" is one SSP-RK3 step (3 stages).
+``value`` = cell updates per RK stage per second over the timed steps, with
+the state resident in HBM.  One JSON line is printed.
+
+The roofline block times the dominant kernel with HIP events recorded on
+the context's stream during the timed region and divides its algorithmic
+bytes per launch (mesh/state bytes it must read, intermediates excluded) by
+the average launch time.  The CPU baseline times the oracle (single core)
+on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s per RK stage (% HBM roofline); served solves/s + p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(C, V, SF, SE, second_order):
+    """Bytes a node sweep must move: state of every cell, sub-face normal,
+    area and left/right cell ids, node-sorted sub-element -> cell ids and the
+    vertex offsets; the second-order sweep also reads the cell gradients,
+    cell centroids and vertex coordinates."""
+    b = C * 40 + SF * 40 + SE * 4 + V * 4
+    if second_order:
+        b += C * 120 + C * 24 + V * 24
+    return b
+
+
+def stage_bytes_m1(C, V, SF, SE):
+    """First-order residual + update per stage: U in and out, volume, the
+    sub-face data, sub-element -> cell ids and vertex offsets."""
+    return C * 88 + SF * 40 + SE * 4 + V * 4
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel factor), if a summary exists."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(n=512, steps=1):
+    """Oracle (single core) on an n x n tri mesh, second order, SSP-RK3."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    from subfv_amd import meshgen as mg
+
+    om = O.OracleMesh(mg.tri_mesh(n))
+    sol = mg.vortex_state(om.elem_coord)
+    cfg = O.make_config("multi_point", second_order=True, method=4, cfl=0.4)
+    t0 = time.perf_counter()
+    st, _, _ = om.run(cfg, sol, steps, 1)
+    dt = time.perf_counter() - t0
+    if st:
+        raise RuntimeError(f"oracle status {st}")
+    return {"value": om.n_elems * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"tri {n}x{n} ({om.n_elems} prisms), {steps} SSP-RK3 step(s), second order, multi_point"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024, help="quads per side before the diagonal split")
+    ap.add_argument("--first-order", action="store_true", help="first-order forward-Euler stage (M1 headline)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path is single-GPU; multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    from subfv_amd import meshgen as mg
+    from subfv_amd.api import Solver, make_config
+
+    t0 = time.perf_counter()
+    mesh = mg.tri_mesh(args.n)
+    s = Solver(0)
+    s.build_mesh(mesh)
+    C, V = s.n_elems, s.n_vert
+    SF, SE = s.n_sub_faces, s.n_sub_elems
+    o2 = not args.first_order
+    s.set_config(make_config("multi_point", second_order=o2, method=4, cfl=0.4))
+    s.set_time(0.0)
+    s.put_sol(mg.vortex_state(s.elem_coord))
+    setup_s = time.perf_counter() - t0
+    print(f"# mesh {C} cells, {V} vertices, {SF} sub-faces, {SE} sub-elements; setup {setup_s:.1f}s",
+          file=sys.stderr, flush=True)
+
+    scheme = 0 if args.first_order else 1
+    stages = 1 if args.first_order else 3
+    if args.warmup:
+        s.run(args.warmup, scheme)
+    s.profile(True)
+    t1 = time.perf_counter()
+    tf, dtf = s.run(args.steps, scheme)  # ends with a stream synchronisation
+    elapsed = time.perf_counter() - t1
+    phases = {}
+    for ph in ("node_flux", "node_flux_o2", "grad", "gather", "update"):
+        ms, cnt = s.profile_read(ph)
+        if cnt:
+            phases[ph] = {"ms_total": ms, "launches": cnt, "us_avg": 1e3 * ms / cnt}
+    s.profile(False)
+
+    value = C * stages * args.steps / elapsed
+    dom = max(phases, key=lambda p: phases[p]["ms_total"])
+    dom_o2 = dom == "node_flux_o2"
+    abytes = algorithmic_bytes(C, V, SF, SE, dom_o2)
+    achieved = abytes / (phases[dom]["us_avg"] * 1e-6) / 1e9
+    traffic = read_traffic(f"{dom}_tri{args.n}")
+    stage_us = 1e6 * elapsed / (args.steps * stages)
+    m1 = stage_bytes_m1(C, V, SF, SE)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"tri {args.n}x{args.n} jittered, {C} prisms, multi_point, "
+                         + ("first order, forward Euler" if args.first_order else
+                            "second order (method 4 limited gradients), SSP-RK3")),
+            "cells": C, "stages_per_step": stages, "inviscid": True,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": abytes, "us_per_launch": phases[dom]["us_avg"],
+        },
+        "stage": {
+            "us_per_stage": stage_us,
+            "m1_bytes": m1,
+            "m1_frac_of_peak": m1 / (stage_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "phases_us_avg": {k: v["us_avg"] for k, v in phases.items()},
+        },
+        "t_final": tf,
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,6 +153,13 @@ int subfv_compute_rhs_ns(subfv_ctx *ctx);
  * steps or until t >= tmax; *t and *dt receive the final values. */
 int subfv_run(subfv_ctx *ctx, int32_t n_steps, int32_t time_scheme, double *t, double *dt);
 
+/* Kernel timing (no counterpart in the reference): with profiling on, HIP
+ * events bracket each phase ("node_flux", "node_flux_o2", "grad", "gather",
+ * "update") on the context's stream; read returns the summed milliseconds
+ * and the number of launches since profiling was (re)enabled. */
+int subfv_profile_enable(subfv_ctx *ctx, int32_t on);
+int subfv_profile_read(subfv_ctx *ctx, const char *phase, double *ms_total, int64_t *count);
+
 /* ------------------------------------------------------------------ BCSR */
 typedef struct subfv_bcsr subfv_bcsr;
 /* bcsr_matrix_type: m block rows, b x b blocks, 1-based row(m+1), col(nnzb),

@@ -150,6 +150,65 @@ static inline void om_pinv3(double *A) {
     }
 }
 
+/* Pseudo-inverse of a symmetric 3x3 matrix (the nodal matrices of the
+ * velocity solvers): cyclic Jacobi rotations M = Q L Q^T, singular values
+ * |L| sorted decreasingly, eigenvalues with |l| <= 3e-12 max|l| dropped;
+ * P = sum_k q_k (1/l_k) q_k^T.  A column-major in, P column-major out. */
+static inline void om_pinv3_sym(const double *A, double *P) {
+  double a[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) a[i][j] = A[i + 3 * j];
+  double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int sweep = 0; sweep < 16; sweep++) {
+    int rotated = 0;
+    for (int pq = 0; pq < 3; pq++) {
+      int p = (pq == 2) ? 1 : 0, qq = (pq == 0) ? 1 : 2, r = 3 - p - qq;
+      double app = a[p][p], aqq = a[qq][qq], apq = a[p][qq], arp = a[r][p], arq = a[r][qq];
+      if (apq == 0.0) continue;
+      if (fabs(apq) <= 1e-15 * sqrt(fabs(app) * fabs(aqq))) continue;
+      rotated = 1;
+      double theta = (aqq - app) / (2.0 * apq);
+      double t;
+      if (fabs(theta) > 1e100)
+        t = 0.5 / theta;
+      else
+        t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+      double c = 1.0 / sqrt(1.0 + t * t);
+      double s = t * c;
+      for (int i = 0; i < 3; i++) {
+        double vp = q[i + 3 * p], vq = q[i + 3 * qq];
+        q[i + 3 * p] = c * vp - s * vq;
+        q[i + 3 * qq] = s * vp + c * vq;
+      }
+      a[p][p] = app - t * apq;
+      a[qq][qq] = aqq + t * apq;
+      a[p][qq] = a[qq][p] = 0.0;
+      a[r][p] = a[p][r] = c * arp - s * arq;
+      a[r][qq] = a[qq][r] = s * arp + c * arq;
+    }
+    if (!rotated) break;
+  }
+  double l[3] = {a[0][0], a[1][1], a[2][2]};
+  int ord[3] = {0, 1, 2};
+  for (int i = 1; i < 3; i++) {
+    int key = ord[i], j = i - 1;
+    while (j >= 0 && fabs(l[ord[j]]) < fabs(l[key])) {
+      ord[j + 1] = ord[j];
+      j--;
+    }
+    ord[j + 1] = key;
+  }
+  double tol = 3.0 * 1e-12 * fabs(l[ord[0]]);
+  double ip[3];
+  for (int k = 0; k < 3; k++) ip[k] = (fabs(l[ord[k]]) > tol) ? 1.0 / l[ord[k]] : 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; k++) acc = acc + q[i + 3 * ord[k]] * (ip[k] * q[j + 3 * ord[k]]);
+      P[i + 3 * j] = acc;
+    }
+}
+
 /* lu_solve_noswp, linear_solver_module.F90:413-445 (no pivoting) */
 static inline void om_lu_solve_noswp(int n, const double *A, double *x, const double *b) {
   double C[64];

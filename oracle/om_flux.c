@@ -435,7 +435,7 @@ static void solve_nodal_velocity(const om_mesh *m, const om_config *cfg, int v, 
     if (cfg->bc_style == 2) {
       double mb = dmax(dmax(fabs(Bp[0]), fabs(Bp[1])), fabs(Bp[2]));
       if (mb > 1e-8) {
-        om_pinv3(mat);
+        om_pinv3_sym(mat, mat);
         double mR[3], mB[3];
         for (int a = 0; a < 3; a++) {
           mR[a] = mat[a] * Rp[0] + mat[a + 3] * Rp[1] + mat[a + 6] * Rp[2];
@@ -451,7 +451,7 @@ static void solve_nodal_velocity(const om_mesh *m, const om_config *cfg, int v, 
         om_lu_solve(3, mat, Rp, v_node);
       }
     } else {
-      om_pinv3(mat);
+      om_pinv3_sym(mat, mat);
       double vn[3];
       for (int a = 0; a < 3; a++) vn[a] = mat[a] * Rp[0] + mat[a + 3] * Rp[1] + mat[a + 6] * Rp[2];
       for (int d = 0; d < 3; d++) v_node[d] = vn[d];
@@ -1030,3 +1030,4 @@ void om_test_lu_solve(int n, const double *A, const double *b, double *x) {
 }
 void om_test_pinv3(double *A) { om_pinv3(A); }
 void om_test_lu_solve_noswp(int n, const double *A, const double *b, double *x) { om_lu_solve_noswp(n, A, x, b); }
+void om_test_pinv3_sym(double *A) { om_pinv3_sym(A, A); }

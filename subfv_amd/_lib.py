@@ -114,6 +114,8 @@ SIGNATURES = {
     "subfv_add_timedisc_and_solve": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "subfv_compute_rhs_ns": (C.c_int, [_P]),
     "subfv_run": (C.c_int, [_P, _i32, _i32, _dp, _dp]),
+    "subfv_profile_enable": (C.c_int, [_P, _i32]),
+    "subfv_profile_read": (C.c_int, [_P, C.c_char_p, _dp, C.POINTER(C.c_int64)]),
     "subfv_bcsr_create": (C.c_int, [_P, _PP, _i32, _i32, _i32, _ip, _ip, _dp]),
     "subfv_bcsr_destroy": (None, [_P]),
     "subfv_bcsr_matmul": (C.c_int, [_P, _dp, _dp]),

@@ -197,6 +197,16 @@ class Solver:
         self._check(_lib.lib().subfv_compute_rhs_ns(self.h))
         return self.get("rhs"), self.get("sum_lambda")
 
+    def profile(self, on: bool = True):
+        self._check(_lib.lib().subfv_profile_enable(self.h, int(on)))
+
+    def profile_read(self, phase: str):
+        """(total milliseconds, launches) of one phase since profile(True)."""
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        self._check(_lib.lib().subfv_profile_read(self.h, phase.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def run(self, n_steps: int, time_scheme: int = 0):
         """Time loop on device-resident state; returns (t, dt)."""
         t = C.c_double(0.0)

@@ -43,6 +43,9 @@ struct subfv_ctx {
   int *derr = nullptr;
   StepState *st = nullptr;
   double host_t = 0.0;
+  // optional per-phase kernel timing with HIP events on the context stream
+  bool prof = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
 };
 
 namespace {
@@ -89,6 +92,27 @@ void free_mesh(subfv_ctx *c) {
 }
 
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// HIP events around one phase when profiling is on (recorded on the stream
+// the kernels run on; read back by subfv_profile_read)
+struct Prof {
+  subfv_ctx *c;
+  hipEvent_t a = nullptr, b = nullptr;
+  const char *name;
+  Prof(subfv_ctx *ctx, const char *nm) : c(ctx), name(nm) {
+    if (!c->prof) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      a = b = nullptr;
+      return;
+    }
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~Prof() {
+    if (!a) return;
+    (void)hipEventRecord(b, c->stream);
+    c->prof_ev.push_back({name, {a, b}});
+  }
+};
 
 int check_err(subfv_ctx *c) {
   int e = 0;
@@ -160,20 +184,19 @@ int upload_mesh(subfv_ctx *c) {
   UP(e_vert, h.e_vert);
   UP(e_foff, h.e_foff);
   UP(e_face, h.e_face);
-  // each cell's node-sorted slots in increasing vertex id (scatter order of
-  // compute_euler_flux, ns_euler_module.F90:321-336)
-  std::vector<int> cslot(h.nse);
+  // each cell's sub-elements in increasing vertex id (the scatter order of
+  // compute_euler_flux, ns_euler_module.F90:321-336), 3 bits per entry
+  std::vector<uint32_t> cperm(h.ne);
   for (int e = 0; e < h.ne; e++) {
     int b = h.e_voff[e], n = h.e_voff[e + 1] - b;
-    std::vector<std::pair<int, int>> pv(n);
-    for (int j = 0; j < n; j++) {
-      int se = b + j, v = h.se_vert[se];
-      pv[j] = {v, h.v_seoff[v] + h.se_loc[se]};
-    }
-    std::sort(pv.begin(), pv.end());
-    for (int j = 0; j < n; j++) cslot[b + j] = pv[j].second;
+    std::pair<int, int> pv[8];
+    for (int j = 0; j < n; j++) pv[j] = {h.se_vert[b + j], j};
+    std::sort(pv, pv + n);
+    uint32_t code = 0;
+    for (int j = 0; j < n; j++) code |= (uint32_t)pv[j].second << (3 * j);
+    cperm[e] = code;
   }
-  UP(c_slot, cslot);
+  UP(c_perm, cperm);
   std::vector<int> waves = pack_waves(h);
   d.nwaves = (int)waves.size() - 1;
   UP(wave_v, waves);
@@ -258,6 +281,7 @@ int launch_flux(subfv_ctx *c, const double *U, const double *G, double *P, bool 
   a.per_xcd = (nb + 7) / 8;
   a.nblocks = nb;
   dim3 grid(8 * a.per_xcd), blk(64 * kFluxWaves);
+  Prof pf(c, G ? "node_flux_o2" : "node_flux");
   switch (c->bc.scheme) {
     case kMultiPoint: hipLaunchKernelGGL(k_node_flux<kMultiPoint>, grid, blk, 0, c->stream, a); break;
     case kMultiPointIso: hipLaunchKernelGGL(k_node_flux<kMultiPointIso>, grid, blk, 0, c->stream, a); break;
@@ -284,7 +308,11 @@ int launch_gather(subfv_ctx *c, const double *P, const double *P2, double *rhs, 
   g.blk = c->blk;
   g.st = c->st;
   g.fixed_dt = fixed_dt;
-  hipLaunchKernelGGL(k_gather, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
+  Prof pf(c, "gather");
+  const int nb = (int)nblk(c->dm.C, kCellThreads);
+  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kCellThreads), 0, c->stream, g);
+  hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(kCellThreads), 0, c->stream, (const double *)c->blk, nb, c->st,
+                     fixed_dt);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -297,7 +325,11 @@ int launch_grad(subfv_ctx *c, const double *U, int method) {
   g.VMM = c->VMM;
   g.G = c->G;
   g.method = method;
-  hipLaunchKernelGGL(k_nodal_grad, dim3(nblk(c->dm.V, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
+  int nb = (c->dm.nwaves + kFluxWaves - 1) / kFluxWaves;
+  g.per_xcd = (nb + 7) / 8;
+  g.nblocks = nb;
+  Prof pf(c, "grad");
+  hipLaunchKernelGGL(k_nodal_grad, dim3(8 * g.per_xcd), dim3(64 * kFluxWaves), 0, c->stream, g);
   hipLaunchKernelGGL(k_cell_grad, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -322,10 +354,17 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
   u.w1 = w1;
   u.flux_limit = flux_limit;
   u.advance_t = advance_t;
-  u.fixup_pass = 0;
   dim3 g(nblk(c->dm.C, kCellThreads)), b(kCellThreads);
-  hipLaunchKernelGGL(k_update, g, b, 0, c->stream, u);
-  hipLaunchKernelGGL(k_update_fixup, g, b, 0, c->stream, u);
+  Prof pf(c, "update");
+  if (c->diag) {
+    hipLaunchKernelGGL(k_update<true>, g, b, 0, c->stream, u);
+    hipLaunchKernelGGL(k_finalize_omega, dim3(1), b, 0, c->stream, (const double *)c->blk, (int)g.x, c->st, advance_t);
+    hipLaunchKernelGGL(k_update_fixup<true>, g, b, 0, c->stream, u);
+  } else {
+    hipLaunchKernelGGL(k_update<false>, g, b, 0, c->stream, u);
+    hipLaunchKernelGGL(k_finalize_omega, dim3(1), b, 0, c->stream, (const double *)c->blk, (int)g.x, c->st, advance_t);
+    hipLaunchKernelGGL(k_update_fixup<false>, g, b, 0, c->stream, u);
+  }
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -410,18 +449,9 @@ int host_to_dev_soa(subfv_ctx *c, const double *in, int ncomp, long n, double *d
 }
 
 // dt from the device sum_lambda alone (standalone add_timedisc_and_solve)
-__global__ __launch_bounds__(kCellThreads) void k_dt_from_sl(DevMesh m, const double *sl, double *blk, StepState *st) {
+__global__ __launch_bounds__(kCellThreads) void k_dt_from_sl(DevMesh m, const double *sl, double *blk) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double local = (c < m.C) ? m.cvol[c] / sl[c] : 1e10;
-  double mn;
-  if (block_min_last(local, blk, &st->ticket, gridDim.x, &mn)) {
-    double d = dmin(1e10, mn);
-    st->dtmin = d;
-    st->dt_fl = st->cfl * d;
-    double dt = st->cfl * d;
-    if (!st->lts) dt = dmin(st->tmax - st->t + 1e-12, dt);
-    st->dt = dt;
-  }
+  block_min_store((c < m.C) ? m.cvol[c] / sl[c] : 1e10, blk);
 }
 
 // flux_limiting as a standalone pass (ns_euler_recon_module.F90:586-593)
@@ -642,8 +672,9 @@ int64_t subfv_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) 
   const long C = c->dm.C, V = c->dm.V;
   struct Item { const char *nm; double *p; int nc; long n; };
   Item items[] = {{"sol", c->U[c->cur], 5, C},   {"rhs", c->rhs, 5, C},      {"rhs_o2", c->rhs2, 5, C},
-                  {"sum_lambda", c->sl, 1, C},   {"sum_lambda_o2", c->sl2, 1, C}, {"cell_grad", c->G, 15, C},
-                  {"nodal_grad", c->NG, 15, V}, {"vp", c->vp, 3, V}};
+                  {"sum_lambda", c->sl, 1, C},   {"sum_lambda_o2", c->sl2, 1, C},
+                  // gradients are stored in the reference layout already
+                  {"cell_grad", c->G, 1, 15 * C}, {"nodal_grad", c->NG, 1, 15 * V}, {"vp", c->vp, 3, V}};
   for (auto &it : items)
     if (n == it.nm) {
       if ((int64_t)it.nc * it.n > cap) return -1;
@@ -658,7 +689,7 @@ int64_t subfv_put_f64(subfv_ctx *c, const char *name, const double *in, int64_t 
   const long C = c->dm.C;
   struct Item { const char *nm; double *p; int nc; long n; };
   Item items[] = {{"sol", c->U[c->cur], 5, C}, {"rhs", c->rhs, 5, C}, {"rhs_o2", c->rhs2, 5, C},
-                  {"sum_lambda", c->sl, 1, C}, {"cell_grad", c->G, 15, C}};
+                  {"sum_lambda", c->sl, 1, C}, {"cell_grad", c->G, 1, 15 * C}};
   for (auto &it : items)
     if (n == it.nm) {
       if ((int64_t)it.nc * it.n != nval) return -1;
@@ -696,7 +727,9 @@ int subfv_flux_limiting(subfv_ctx *c, double cfl) {
   HIPCHK(c, hipMemcpyAsync(c->st, &hs, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
   const DevMesh &d = c->dm;
   hipLaunchKernelGGL(k_dt_from_sl, dim3(nblk(d.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, d, c->sl,
-                     c->blk, c->st);
+                     c->blk);
+  hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(kCellThreads), 0, c->stream, (const double *)c->blk,
+                     (int)nblk(d.C, kCellThreads), c->st, 0);
   hipLaunchKernelGGL(k_flux_limiting, dim3(nblk(d.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, d,
                      c->U[c->cur], c->rhs, c->rhs2, c->sl, c->st);
   HIPCHK(c, hipGetLastError());
@@ -718,7 +751,9 @@ int subfv_add_timedisc_and_solve(subfv_ctx *c, double a1, double a2, double cfl,
   HIPCHK(c, hipMemcpyAsync(c->st, &hs, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
   const DevMesh &d = c->dm;
   hipLaunchKernelGGL(k_dt_from_sl, dim3(nblk(d.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, d, c->sl,
-                     c->blk, c->st);
+                     c->blk);
+  hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(kCellThreads), 0, c->stream, (const double *)c->blk,
+                     (int)nblk(d.C, kCellThreads), c->st, 0);
   int nxt = (c->cur + 1) % 3;
   int r = launch_update(c, c->U[c->cur], nullptr, c->U[nxt], a1, a2, 0.0, 1.0, 0, 0);
   if (r) return r;
@@ -760,6 +795,35 @@ static int residual(subfv_ctx *c, const double *U, int fixed_dt) {
   }
   if ((r = launch_flux(c, U, nullptr, c->P, false))) return r;
   return launch_gather(c, c->P, nullptr, c->rhs, nullptr, c->sl, fixed_dt);
+}
+
+int subfv_profile_enable(subfv_ctx *c, int32_t on) {
+  if (!c) return SUBFV_E_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &e : c->prof_ev) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  c->prof_ev.clear();
+  c->prof = on != 0;
+  return 0;
+}
+
+int subfv_profile_read(subfv_ctx *c, const char *phase, double *ms_total, int64_t *count) {
+  if (!c || !phase) return SUBFV_E_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  int64_t n = 0;
+  for (auto &e : c->prof_ev)
+    if (e.first == phase) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, e.second.first, e.second.second));
+      tot += ms;
+      n++;
+    }
+  if (ms_total) *ms_total = tot;
+  if (count) *count = n;
+  return 0;
 }
 
 int subfv_run(subfv_ctx *c, int32_t n_steps, int32_t time_scheme, double *t, double *dt) {
@@ -876,7 +940,9 @@ int bjacobi(subfv_bcsr *A, bool fp32, const double *bvec, double *x) {
 // *out = <a, b> (or <w, w>), device scalar
 void mgs_step(subfv_bcsr *A, double *w, const double *vprev, const double *h, const double *vnext, double *out) {
   hipLaunchKernelGGL(k_mgs_step, dim3(kRedBlocks), dim3(kVecThreads), 0, A->ctx->stream, A->n, w, vprev, h, vnext,
-                     A->part, A->ticket, out);
+                     A->part);
+  hipLaunchKernelGGL(k_sum_finalize, dim3(1), dim3(kVecThreads), 0, A->ctx->stream, (const double *)A->part,
+                     kRedBlocks, out);
 }
 
 int norm_host(subfv_bcsr *A, double *w, double *out) {

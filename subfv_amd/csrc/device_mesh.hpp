@@ -39,7 +39,7 @@ struct DevMesh {
   int *se_cell = nullptr;     // [SE] node-sorted slot -> cell
   // cells
   int *e_voff = nullptr, *e_vert = nullptr, *e_foff = nullptr, *e_face = nullptr;
-  int *c_slot = nullptr;      // [SE] node-sorted slots of each cell, ascending vertex
+  uint32_t *c_perm = nullptr; // [C] local sub-element order by increasing vertex id, 3 bits each
   double *ccoord = nullptr;   // [C][3]
   double *cvol = nullptr;     // [C]
 };

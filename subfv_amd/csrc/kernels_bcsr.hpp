@@ -89,44 +89,37 @@ __global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
 }
 
 // -------------------------------------------------------- vector reductions
-// out[slot] = sum a.b over the vector; per-block partials in part[], the
-// last block sums them in block order.
-SFV_DEV void block_sum_last(double v, double *part, unsigned int *ticket, double *out) {
+// Sums in two launches: per-block partials (fixed grid, fixed order), then
+// one block adds them in block order; deterministic for a given grid.
+SFV_DEV void block_sum_store(double v, double *part) {
   __shared__ double s[kVecThreads];
-  __shared__ bool last;
   s[threadIdx.x] = v;
   __syncthreads();
   for (int o = kVecThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = s[0];
-    __threadfence();
-    last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(kVecThreads) void k_sum_finalize(const double *part, int nblk, double *out) {
+  __shared__ double s[kVecThreads];
   double acc = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kVecThreads) acc = acc + ((volatile double *)part)[i];
+  for (int i = threadIdx.x; i < nblk; i += kVecThreads) acc = acc + part[i];
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int o = kVecThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    *out = s[0];
-    *ticket = 0u;
-  }
+  if (threadIdx.x == 0) *out = s[0];
 }
 
-// w = w - (*h) * v_prev (if v_prev), then *out = <v_next, w> (or <w, w>
-// when v_next is null).  One pass fuses an MGS update with the next dot.
+// w = w - (*h) * v_prev (if v_prev), then partials of <v_next, w> (or of
+// <w, w> when v_next is null).  One pass fuses an MGS update with the next
+// dot product.
 __global__ __launch_bounds__(kVecThreads) void k_mgs_step(long n, double *w, const double *v_prev,
-                                                          const double *h, const double *v_next, double *part,
-                                                          unsigned int *ticket, double *out) {
+                                                          const double *h, const double *v_next, double *part) {
   double acc = 0.0;
   const double hh = v_prev ? *h : 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -137,7 +130,7 @@ __global__ __launch_bounds__(kVecThreads) void k_mgs_step(long n, double *w, con
     }
     acc = acc + (v_next ? v_next[i] * x : x * x);
   }
-  block_sum_last(acc, part, ticket, out);
+  block_sum_store(acc, part);
 }
 
 // out = a / *s   (normalisation by a device scalar, sqrt taken if take_sqrt)

@@ -1,7 +1,7 @@
 // Cell-side kernels: node->cell gather of the sweep partials, the global dt
 // reduction, the stage update (add_timedisc_and_solve) with its positivity
-// scaling, flux limiting, and the gradient + limiter kernels of the second
-// order reconstruction.
+// scaling and flux limiting, and the gradient + limiter kernels of the
+// second-order reconstruction.
 #pragma once
 #include "device_mesh.hpp"
 #include "kernels_flux.hpp"
@@ -26,45 +26,79 @@ struct StepState {
 
 constexpr int kCellThreads = 256;
 
-// Block min of one value per thread, then the last block to finish reduces
-// the per-block minima.  Returns true in the thread that holds the final
-// value in *out_all (thread 0 of the last block).
-SFV_DEV bool block_min_last(double v, double *blk, unsigned int *ticket, int nblk, double *out_all) {
+// Global minima are taken in two launches: every block stores its minimum
+// (plain stores; the kernel boundary makes them visible), then one block
+// reduces them.  A device-wide fence per block would write back the whole
+// L2 of the issuing XCD, so none is used.  min is order-independent, so the
+// result is exact.
+SFV_DEV void block_min_store(double v, double *blk) {
   __shared__ double s_m[kCellThreads];
-  __shared__ bool s_last;
   s_m[threadIdx.x] = v;
   __syncthreads();
   for (int o = kCellThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) s_m[threadIdx.x] = dmin(s_m[threadIdx.x], s_m[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    blk[blockIdx.x] = s_m[0];
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    s_last = (t == (unsigned int)nblk - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+  if (threadIdx.x == 0) blk[blockIdx.x] = s_m[0];
+}
+
+SFV_DEV double reduce_min_one_block(const double *blk, int nblk) {
+  __shared__ double s_m[kCellThreads];
   double acc = 1e300;
-  for (int i = threadIdx.x; i < nblk; i += kCellThreads) acc = dmin(acc, ((volatile double *)blk)[i]);
+  for (int i = threadIdx.x; i < nblk; i += kCellThreads) acc = dmin(acc, blk[i]);
   s_m[threadIdx.x] = acc;
   __syncthreads();
   for (int o = kCellThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) s_m[threadIdx.x] = dmin(s_m[threadIdx.x], s_m[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    *out_all = s_m[0];
-    *ticket = 0u;
+  return s_m[0];
+}
+
+// dt = cfl*min(vol/sum_lambda) (ns_euler_module.F90:546-560; the flux
+// limiter's own dt, ns_euler_recon_module.F90:575-581, has no tmax clamp)
+__global__ __launch_bounds__(kCellThreads) void k_finalize_dt(const double *blk, int nblk, StepState *st,
+                                                              int fixed_dt) {
+  double mn = reduce_min_one_block(blk, nblk);
+  if (threadIdx.x != 0) return;
+  double d = dmin(1e10, mn);
+  st->dtmin = d;
+  st->dt_fl = st->cfl * d;
+  if (!fixed_dt) {
+    double dt = st->cfl * d;
+    if (!st->lts) dt = dmin(st->tmax - st->t + 1e-12, dt);
+    st->dt = dt;
   }
-  return threadIdx.x == 0;
+}
+
+// omega checks and time advance of add_timedisc_and_solve (:595-608)
+__global__ __launch_bounds__(kCellThreads) void k_finalize_omega(const double *blk, int nblk, StepState *st,
+                                                                 int advance_t) {
+  double om = reduce_min_one_block(blk, nblk);
+  if (threadIdx.x != 0) return;
+  if (st->done || st->err) return;
+  double omega = dmin(1.0, om);
+  st->omega_min = om;
+  st->omega = omega;
+  if (omega <= 1e-6) {
+    st->err |= kErrBadOmega;
+    st->fixup = 0;
+    return;
+  }
+  st->fixup = omega < 1.0;
+  double dt = st->dt;
+  if (!st->lts && omega < 1.0) dt = (1.0 - omega) * dt;
+  st->dt_adv = dt;
+  if (advance_t) {
+    st->t = st->t + (advance_t == 1 ? dt : st->dt);
+    st->steps += 1;
+    if (!st->lts && st->t >= st->tmax) st->done = 1;
+  }
 }
 
 struct GatherArgs {
   DevMesh m;
-  const double *P;   // first-order partials [6][SE]
+  const double *P;   // first-order partials [SE][6], cell-major
   const double *P2;  // second-order partials or null
   double *rhs;       // [5][C]
   double *rhs2;      // [5][C] (second order)
@@ -82,39 +116,29 @@ __global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double local = 1e10;
   if (c < m.C) {
-    const long SE = m.SE, C = m.C;
+    const long C = m.C;
+    const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
+    const uint32_t perm = m.c_perm[c];
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double lam = 1e-12;
-    for (int q = m.e_voff[c]; q < m.e_voff[c + 1]; q++) {
-      int slot = m.c_slot[q];
-      for (int k = 0; k < 5; k++) r[k] = r[k] - a.P[k * SE + slot];
-      lam = lam + a.P[5 * SE + slot];
+    for (int q = 0; q < nv; q++) {
+      const double *p = a.P + 6L * (b + ((perm >> (3 * q)) & 7u));
+      for (int k = 0; k < 5; k++) r[k] = r[k] - p[k];
+      lam = lam + p[5];
     }
     for (int k = 0; k < 5; k++) a.rhs[k * C + c] = r[k];
     a.sl[c] = lam;
     if (a.P2) {
       double r2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int q = m.e_voff[c]; q < m.e_voff[c + 1]; q++) {
-        int slot = m.c_slot[q];
-        for (int k = 0; k < 5; k++) r2[k] = r2[k] - a.P2[k * SE + slot];
+      for (int q = 0; q < nv; q++) {
+        const double *p = a.P2 + 6L * (b + ((perm >> (3 * q)) & 7u));
+        for (int k = 0; k < 5; k++) r2[k] = r2[k] - p[k];
       }
       for (int k = 0; k < 5; k++) a.rhs2[k * C + c] = r2[k];
     }
     local = m.cvol[c] / lam;
   }
-  double mn;
-  int nblk = gridDim.x;
-  if (block_min_last(local, a.blk, &a.st->ticket, nblk, &mn)) {
-    StepState *st = a.st;
-    double d = dmin(1e10, mn);
-    st->dtmin = d;
-    st->dt_fl = st->cfl * d;
-    if (!a.fixed_dt) {
-      double dt = st->cfl * d;
-      if (!st->lts) dt = dmin(st->tmax - st->t + 1e-12, dt);
-      st->dt = dt;
-    }
-  }
+  block_min_store(local, a.blk);
 }
 
 struct UpdateArgs {
@@ -123,18 +147,18 @@ struct UpdateArgs {
   const double *U0;   // combination partner (w0 != 0) or null
   double *Uout;       // [5][C]
   const double *rhs, *rhs2, *sl;
-  const double *diag; // [25][C] or null (pure c*I diagonal)
+  const double *diag; // [C][25] column-major 5x5 blocks, or null (c*I diagonal)
   double *blk;
   StepState *st;
   double a1, a2, w0, w1;
   int flux_limit;     // second order: flux_limiting before the update
   int advance_t;      // closes a step: 1 -> t += dt_adv (reference), 2 -> t += dt
-  int fixup_pass;     // 0 speculative (omega = 1), 1 correction pass
 };
 
 // flux_limiting (ns_euler_recon_module.F90:586-593) + the per-cell part of
 // add_timedisc_and_solve (ns_euler_module.F90:565-580).  Returns the cell's
-// omega and the increment.
+// omega; u and delta receive the state and the increment.
+template <bool DIAG>
 SFV_DEV double stage_cell(const UpdateArgs &a, long c, double *u, double *delta) {
   const long C = a.m.C;
   const StepState *st = a.st;
@@ -158,15 +182,15 @@ SFV_DEV double stage_cell(const UpdateArgs &a, long c, double *u, double *delta)
   }
   double cc = a.a1 * (vol / st->dt + (st->lts ? lam : 0.0));
   double nw[5];
-  if (a.diag) {
+  if (DIAG) {
     double D[25];
-    for (int q = 0; q < 25; q++) D[q] = a.diag[q * C + c];
+    for (int q = 0; q < 25; q++) D[q] = a.diag[25 * c + q];
     for (int q = 0; q < 5; q++) D[q + 5 * q] = D[q + 5 * q] + cc;
     double b[5];
     for (int k = 0; k < 5; k++) b[k] = r[k] + cc * u[k];
     lu_solve<5>(D, b, nw);
   } else {
-    // LU of cc*I reduces exactly to one division per component
+    // the LU solve of cc*I reduces exactly to one division per component
     for (int k = 0; k < 5; k++) nw[k] = (r[k] + cc * u[k]) / cc;
   }
   double ad[5];
@@ -186,6 +210,7 @@ SFV_DEV void store_stage(const UpdateArgs &a, long c, const double *u, const dou
   }
 }
 
+template <bool DIAG>
 __global__ __launch_bounds__(kCellThreads) void k_update(UpdateArgs a) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   StepState *st = a.st;
@@ -197,40 +222,22 @@ __global__ __launch_bounds__(kCellThreads) void k_update(UpdateArgs a) {
   double om_c = 1.0;
   if (c < a.m.C) {
     double u[5], d[5];
-    om_c = stage_cell(a, c, u, d);
+    om_c = stage_cell<DIAG>(a, c, u, d);
     store_stage(a, c, u, d, 1.0);
   }
-  double om;
-  if (block_min_last(om_c, a.blk, &st->ticket, gridDim.x, &om)) {
-    double omega = dmin(1.0, om);
-    st->omega_min = om;
-    st->omega = omega;
-    if (omega <= 1e-6) {
-      st->err |= kErrBadOmega;
-      st->fixup = 0;
-      return;
-    }
-    st->fixup = omega < 1.0;
-    double dt = st->dt;
-    if (!st->lts && omega < 1.0) dt = (1.0 - omega) * dt;
-    st->dt_adv = dt;
-    if (a.advance_t) {
-      st->t = st->t + (a.advance_t == 1 ? dt : st->dt);
-      st->steps += 1;
-      if (!st->lts && st->t >= st->tmax) st->done = 1;
-    }
-  }
+  block_min_store(om_c, a.blk);
 }
 
 // Correction when the global omega came out below one: redo the cell with
 // sol = sol + a2*omega*delta (ns_euler_module.F90:608).
+template <bool DIAG>
 __global__ __launch_bounds__(kCellThreads) void k_update_fixup(UpdateArgs a) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const StepState *st = a.st;
   if (!st->fixup || st->err) return;
   if (c >= a.m.C) return;
   double u[5], d[5];
-  stage_cell(a, c, u, d);
+  stage_cell<DIAG>(a, c, u, d);
   store_stage(a, c, u, d, st->omega);
 }
 
@@ -238,66 +245,99 @@ __global__ __launch_bounds__(kCellThreads) void k_update_fixup(UpdateArgs a) {
 struct GradArgs {
   DevMesh m;
   const double *U;   // [5][C]
-  double *NG;        // [15][V] nodal gradients, (3k+d) major
-  double *VMM;       // [4][V] rho_min, rho_max, p_min, p_max over vertex cells
-  double *G;         // [15][C]
+  double *NG;        // [V][15] nodal gradients, 3*k + d fastest
+  double *VMM;       // [V][4] rho_min, rho_max, p_min, p_max over the vertex's cells
+  double *G;         // [C][15]
   int method;
+  int per_xcd, nblocks;
 };
 
-// compute_nodal_grad (ns_euler_recon_module.F90:453-490) and the per-vertex
-// part of the bounds of relaxed_maximum_principle (:82-97)
-__global__ __launch_bounds__(kCellThreads) void k_nodal_grad(GradArgs a) {
+struct GradLds {
+  double red[28][kWaveSub];
+  double sum[kWaveVert][24];
+  int vtab[kWaveVert + 1];
+  uint8_t vbound[kWaveVert];
+};
+
+// compute_nodal_grad (ns_euler_recon_module.F90:453-490), lane = sub-face,
+// and the per-vertex bounds of relaxed_maximum_principle (:82-97) taken over
+// the cells on either side of the vertex's sub-faces (which are exactly the
+// cells around the vertex; min and max do not depend on the order).
+__global__ __launch_bounds__(256) void k_nodal_grad(GradArgs a) {
+  __shared__ GradLds lds_all[kFluxWaves];
   const DevMesh &m = a.m;
-  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= m.V) return;
+  const int wib = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  GradLds &L = lds_all[wib];
+  const int lb = xcd_block(a.per_xcd);
+  if (lb >= a.nblocks) return;
+  const int w = lb * kFluxWaves + wib;
+  if (w >= m.nwaves) return;
+  const int vb = m.wave_v[w], ve = m.wave_v[w + 1];
+  const int nvw = ve - vb;
+  const int sf0 = m.sf_off[vb];
+  if (lane <= nvw) L.vtab[lane] = m.sf_off[vb + lane] - sf0;
+  if (lane < nvw) L.vbound[lane] = m.vbound[vb + lane];
+  wsync();
+  const int nsfw = L.vtab[nvw];
+  const bool active = lane < nsfw;
+  const long s = (nsfw > 0) ? sf0 + (active ? lane : 0) : 0;
   const long C = m.C, SF = m.SF;
-  double ng[15], M[9];
-  for (int i = 0; i < 15; i++) ng[i] = 0.0;
-  for (int i = 0; i < 9; i++) M[i] = 0.0;
-  for (int s = m.sf_off[v]; s < m.sf_off[v + 1]; s++) {
-    int re = m.sf_re[s];
-    if (re < 0) continue;
-    int le = m.sf_le[s];
-    double ul[5], ur[5], wl[5], wr[5];
-    for (int k = 0; k < 5; k++) {
-      ul[k] = a.U[k * C + le];
-      ur[k] = a.U[k * C + re];
-    }
-    to_prim(ul, wl);
-    to_prim(ur, wr);
-    double n[3] = {m.sfn[s], m.sfn[SF + s], m.sfn[2 * SF + s]};
-    double ar = m.sfn[3 * SF + s];
-    for (int k = 0; k < 5; k++) {
-      double dw = wr[k] - wl[k];
-      for (int d = 0; d < 3; d++) ng[3 * k + d] = ng[3 * k + d] + ar * (n[d] * dw);
-    }
-    for (int b = 0; b < 3; b++) {
-      double dx = m.ccoord[3L * re + b] - m.ccoord[3L * le + b];
-      for (int d = 0; d < 3; d++) M[d + 3 * b] = M[d + 3 * b] + ar * (n[d] * dx);
-    }
+  const int le = m.sf_le[s], re = m.sf_re[s];
+  const bool inner = active && re >= 0;
+  double ul[5], ur[5], wl[5], wr[5];
+  for (int k = 0; k < 5; k++) {
+    ul[k] = a.U[k * C + le];
+    ur[k] = a.U[k * C + (re >= 0 ? re : le)];
   }
-  pinv3(M);
-  const bool bnd = m.vbound[v];
-  for (int k = 0; k < 5; k++)
-    for (int d = 0; d < 3; d++) {
-      double o = M[d] * ng[3 * k] + M[d + 3] * ng[3 * k + 1] + M[d + 6] * ng[3 * k + 2];
-      a.NG[(3 * k + d) * m.V + v] = bnd ? 0.0 : o;
-    }
-  double rmin = 1e100, rmax = 0.0, pmin = 1e100, pmax = 0.0;
-  for (int q = m.v_eoff[v]; q < m.v_eoff[v + 1]; q++) {
-    int e = m.v_elem[q];
-    double u[5], w[5];
-    for (int k = 0; k < 5; k++) u[k] = a.U[k * C + e];
-    to_prim(u, w);
-    pmin = dmin(pmin, w[4]);
-    pmax = dmax(pmax, w[4]);
-    rmin = dmin(rmin, w[0]);
-    rmax = dmax(rmax, w[0]);
+  to_prim(ul, wl);
+  to_prim(ur, wr);
+  const double n[3] = {m.sfn[s], m.sfn[SF + s], m.sfn[2 * SF + s]};
+  const double ar = m.sfn[3 * SF + s];
+  for (int k = 0; k < 5; k++) {
+    double dw = wr[k] - wl[k];
+    for (int d = 0; d < 3; d++) L.red[3 * k + d][lane] = inner ? ar * (n[d] * dw) : 0.0;
   }
-  a.VMM[v] = rmin;
-  a.VMM[m.V + v] = rmax;
-  a.VMM[2 * m.V + v] = pmin;
-  a.VMM[3 * m.V + v] = pmax;
+  for (int b = 0; b < 3; b++) {
+    double dx = inner ? m.ccoord[3L * re + b] - m.ccoord[3L * le + b] : 0.0;
+    for (int d = 0; d < 3; d++) L.red[15 + d + 3 * b][lane] = inner ? ar * (n[d] * dx) : 0.0;
+  }
+  // bounds: the left cell always, the right cell when it is a cell
+  L.red[24][lane] = active ? dmin(wl[0], inner ? wr[0] : 1e100) : 1e100;
+  L.red[25][lane] = active ? dmax(wl[0], inner ? wr[0] : 0.0) : 0.0;
+  L.red[26][lane] = active ? dmin(wl[4], inner ? wr[4] : 1e100) : 1e100;
+  L.red[27][lane] = active ? dmax(wl[4], inner ? wr[4] : 0.0) : 0.0;
+  wsync();
+  for (int p = lane; p < nvw * 24; p += 64) {
+    int vv = p / 24, c = p - vv * 24;
+    L.sum[vv][c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
+  }
+  if (lane < nvw) {
+    const int jb = L.vtab[lane], je = L.vtab[lane + 1];
+    double mm[4] = {1e100, 0.0, 1e100, 0.0};
+    for (int j = jb; j < je; j++) {
+      mm[0] = dmin(mm[0], L.red[24][j]);
+      mm[1] = dmax(mm[1], L.red[25][j]);
+      mm[2] = dmin(mm[2], L.red[26][j]);
+      mm[3] = dmax(mm[3], L.red[27][j]);
+    }
+    double *o = a.VMM + 4L * (vb + lane);
+    for (int q = 0; q < 4; q++) o[q] = mm[q];
+  }
+  wsync();
+  if (lane < nvw) {
+    const double *S = L.sum[lane];
+    double M[9];
+    for (int q = 0; q < 9; q++) M[q] = S[15 + q];
+    pinv3(M);
+    const bool bnd = L.vbound[lane];
+    double *o = a.NG + 15L * (vb + lane);
+    for (int k = 0; k < 5; k++)
+      for (int d = 0; d < 3; d++) {
+        double x = M[d] * S[3 * k] + M[d + 3] * S[3 * k + 1] + M[d + 6] * S[3 * k + 2];
+        o[3 * k + d] = bnd ? 0.0 : x;
+      }
+  }
 }
 
 SFV_DEV double oi3(const double *g) { return sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]); }
@@ -308,12 +348,13 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
   const DevMesh &m = a.m;
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.C) return;
-  const long C = m.C, V = m.V;
+  const long C = m.C;
   double g[15];
   for (int q = 0; q < 15; q++) g[q] = 0.0;
   const int vb = m.e_voff[e], nv = m.e_voff[e + 1] - vb;
+  double *go = a.G + 15L * e;
   if (a.method == 0 || a.method == 1) {
-    for (int q = 0; q < 15; q++) a.G[q * C + e] = 0.0;
+    for (int q = 0; q < 15; q++) go[q] = 0.0;
     return;
   }
   const double eps = 1e-3;
@@ -322,20 +363,20 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
     for (int j = 0; j < nv; j++) {
       int v = m.e_vert[vb + j];
       double vv = m.vvol[v];
-      for (int q = 0; q < 15; q++) g[q] = g[q] + vv * a.NG[q * V + v];
+      const double *ng = a.NG + 15L * v;
+      for (int q = 0; q < 15; q++) g[q] = g[q] + vv * ng[q];
       dvs = dvs + vv;
     }
     for (int q = 0; q < 15; q++) g[q] = g[q] / dvs;
   } else if (a.method == 3) {
     int v = m.e_vert[vb];
-    for (int q = 0; q < 15; q++) g[q] = a.NG[q * V + v];
+    for (int q = 0; q < 15; q++) g[q] = a.NG[15L * v + q];
     for (int j = 1; j < nv; j++) {
       v = m.e_vert[vb + j];
-      for (int k = 0; k < 5; k++) {
-        double ngk[3] = {a.NG[(3 * k) * V + v], a.NG[(3 * k + 1) * V + v], a.NG[(3 * k + 2) * V + v]};
-        if (oi3(ngk) < oi3(&g[3 * k]))
-          for (int d = 0; d < 3; d++) g[3 * k + d] = ngk[d];
-      }
+      const double *ng = a.NG + 15L * v;
+      for (int k = 0; k < 5; k++)
+        if (oi3(&ng[3 * k]) < oi3(&g[3 * k]))
+          for (int d = 0; d < 3; d++) g[3 * k + d] = ng[3 * k + d];
     }
   } else {
     double ws[5] = {0, 0, 0, 0, 0}, cg[15], cws = 0.0;
@@ -344,7 +385,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
       int v = m.e_vert[vb + j];
       double vv = m.vvol[v];
       double ng[15];
-      for (int q = 0; q < 15; q++) ng[q] = a.NG[q * V + v];
+      for (int q = 0; q < 15; q++) ng[q] = a.NG[15L * v + q];
       for (int k = 0; k < 5; k++) {
         double den = eps + oi3(&ng[3 * k]);
         if (a.method != 4) den = den * den;
@@ -369,13 +410,14 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
     for (int k = 0; k < 5; k++)
       for (int d = 0; d < 3; d++) g[3 * k + d] = g[3 * k + d] / ws[k];
   }
-  // physical_slope_limiter
+  // physical_slope_limiter, with the neighbourhood bounds gathered alongside
   const double tol = 1e-8;
   double u[5], w[5];
   for (int k = 0; k < 5; k++) u[k] = a.U[k * C + e];
   to_prim(u, w);
-  const double *xc = &m.ccoord[3L * e];
+  const double xc[3] = {m.ccoord[3L * e], m.ccoord[3L * e + 1], m.ccoord[3L * e + 2]};
   double omega = 1.0;
+  double rmin = 1e100, rmax = 0.0, pmin = 1e100, pmax = 0.0;
   for (int j = 0; j < nv; j++) {
     int v = m.e_vert[vb + j];
     double dx[3];
@@ -384,17 +426,14 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
     double d4 = g[12] * dx[0] + g[13] * dx[1] + g[14] * dx[2];
     if (d0 < 0.0) omega = dmin(omega, dmax(0.0, w[0] / (tol - d0)));
     if (d4 < 0.0) omega = dmin(omega, dmax(0.0, w[4] / (tol - d4)));
+    const double *mm = a.VMM + 4L * v;
+    rmin = dmin(rmin, mm[0]);
+    rmax = dmax(rmax, mm[1]);
+    pmin = dmin(pmin, mm[2]);
+    pmax = dmax(pmax, mm[3]);
   }
   for (int q = 0; q < 15; q++) g[q] = omega * g[q];
-  // relaxed_maximum_principle
-  double rmin = 1e100, rmax = 0.0, pmin = 1e100, pmax = 0.0;
-  for (int j = 0; j < nv; j++) {
-    int v = m.e_vert[vb + j];
-    rmin = dmin(rmin, a.VMM[v]);
-    rmax = dmax(rmax, a.VMM[V + v]);
-    pmin = dmin(pmin, a.VMM[2 * V + v]);
-    pmax = dmax(pmax, a.VMM[3 * V + v]);
-  }
+  // relaxed_maximum_principle on the limited gradient
   pmin = (1.0 - 1e-2) * pmin;
   rmin = (1.0 - 1e-2) * rmin;
   pmax = (1.0 + 1e-2) * pmax;
@@ -415,7 +454,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
     else
       omega = dmin(omega, dmax(0.0, -(w[4] - pmax) / (tol + d4)));
   }
-  for (int q = 0; q < 15; q++) a.G[q * C + e] = omega * g[q];
+  for (int q = 0; q < 15; q++) go[q] = omega * g[q];
 }
 
 }  // namespace sfv

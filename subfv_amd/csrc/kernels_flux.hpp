@@ -4,12 +4,16 @@
 //
 // Mapping: one wavefront owns a run of consecutive vertices whose sub-faces
 // (contiguous, vertex-major) fit in its 64 lanes; lane = sub-face.  Every
-// per-vertex sum (nodal matrix, right-hand side, per-sub-element flux) is
-// formed by one lane walking the vertex's sub-faces in reference order
-// through LDS, so results are deterministic and atomic-free.  The output is
-// one row of partial sums per node-sorted sub-element slot:
-// P[k][slot] = sum of area*flux_k (k < 5) and P[5][slot] = sum of
-// area*max(0, -s) over that vertex's sub-faces of that sub-element.
+// per-vertex sum (nodal matrix and right-hand side, per-sub-element flux) is
+// formed in reference order by one lane reading the vertex's sub-face terms
+// from LDS (loads batched, additions in order), so results are deterministic
+// and atomic-free.  Per-vertex data and the wave's sub-element slots are
+// staged in LDS up front so the dependent global loads happen once.
+//
+// Output: one row of six partial sums per sub-element, stored cell-major so
+// the gather streams it: P[6*se + k] = sum over the vertex's sub-faces of
+// area*flux_k (k < 5) and P[6*se + 5] = sum of area*max(0, -s), where se is
+// the reference's (cell-major) sub-element id.
 #pragma once
 #include "device_mesh.hpp"
 #include "physics.hpp"
@@ -20,8 +24,8 @@ struct FluxArgs {
   DevMesh m;
   BCParams bc;
   const double *U;      // [5][C]
-  const double *G;      // [15][C] cell gradients, (3*k + d) major (second order)
-  double *P;            // [6][SE]
+  const double *G;      // [C][15] cell gradients, 3*k + d fastest (second order)
+  double *P;            // [SE][6] cell-major
   double *vp;           // [3][V] nodal velocity output or null
   int *err;
   const double *t;      // physical time (time-dependent BCs)
@@ -43,22 +47,46 @@ SFV_DEV int xcd_block(int per_xcd) {
   return (b & 7) * per_xcd + (b >> 3);
 }
 
+// In-order sum row[b] + ... + row[e-1] starting from 0.0; loads are issued
+// eight at a time, the additions stay sequential (adding the +0.0 padding
+// leaves every partial sum unchanged).
+SFV_DEV double seg_sum(const double *row, int b, int e) {
+  double acc = 0.0;
+  for (int j0 = b; j0 < e; j0 += 8) {
+    double t[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) t[q] = (j0 + q < e) ? row[j0 + q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc = acc + t[q];
+  }
+  return acc;
+}
+
 constexpr int kFluxWaves = 4;
+constexpr int kWaveSlots = 128;  // sub-element slots staged per wavefront
+
+struct FluxLds {
+  double red[12][kWaveSub];
+  double sum[kWaveVert][9];
+  double vn[kWaveVert][3];
+  double bp[kWaveVert][3];
+  int vtab[kWaveVert + 1];
+  int setab[kWaveVert + 1];
+  int lslot[kWaveSub];
+  int rslot[kWaveSub];
+  int secell[kWaveSlots];
+  int seid[kWaveSlots];
+  uint8_t vbound[kWaveVert];
+};
 
 template <int SCHEME>
 __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
-  __shared__ double s_red[kFluxWaves][12][kWaveSub];
-  __shared__ double s_sum[kFluxWaves][kWaveVert][9];
-  __shared__ double s_vn[kFluxWaves][kWaveVert][3];
-  __shared__ int s_vtab[kFluxWaves][kWaveVert + 1];
-  __shared__ int s_setab[kFluxWaves][kWaveVert + 1];
-  __shared__ int s_lslot[kFluxWaves][kWaveSub];
-  __shared__ int s_rslot[kFluxWaves][kWaveSub];
-
+  __shared__ FluxLds lds_all[kFluxWaves];
   const DevMesh &m = a.m;
   const BCParams &bc = a.bc;
   const int wib = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  FluxLds &L = lds_all[wib];
   const int lb = xcd_block(a.per_xcd);
   if (lb >= a.nblocks) return;
   const int w = lb * kFluxWaves + wib;
@@ -67,16 +95,33 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
   const int vb = m.wave_v[w], ve = m.wave_v[w + 1];
   const int nvw = ve - vb;
   const int sf0 = m.sf_off[vb], se0 = m.se_off[vb];
+  const int nsew_g = m.se_off[ve] - se0;
+  const bool staged = nsew_g <= kWaveSlots;
+  // ---- stage per-vertex data and the wave's sub-element slots
   if (lane <= nvw) {
-    s_vtab[wib][lane] = m.sf_off[vb + lane] - sf0;
-    s_setab[wib][lane] = m.se_off[vb + lane] - se0;
+    L.vtab[lane] = m.sf_off[vb + lane] - sf0;
+    L.setab[lane] = m.se_off[vb + lane] - se0;
+  }
+  if (lane < nvw) {
+    L.vbound[lane] = m.vbound[vb + lane];
+    if (SCHEME == kMultiPoint || SCHEME == kMultiPointIso) {
+      L.bp[lane][0] = m.Bp[vb + lane];
+      L.bp[lane][1] = m.Bp[m.V + vb + lane];
+      L.bp[lane][2] = m.Bp[2 * m.V + vb + lane];
+    }
+  }
+  if (staged) {
+    for (int q = lane; q < nsew_g; q += 64) {
+      L.secell[q] = m.se_cell[se0 + q];
+      L.seid[q] = m.v_seid[se0 + q];
+    }
   }
   wsync();
-  const int nsfw = s_vtab[wib][nvw];
+  const int nsfw = L.vtab[nvw];
   const bool active = lane < nsfw;
   int vl = 0;
   for (int i = 1; i < nvw; i++)
-    if (s_vtab[wib][i] <= lane) vl = i;
+    if (L.vtab[i] <= lane) vl = i;
   const int v = vb + vl;
   // inactive lanes mirror a valid sub-face of the wave so every load is in
   // bounds; their contributions are masked below
@@ -98,20 +143,19 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
     to_prim(u, wl);
     if (a.second_order) {
       double dx[3];
+      const double *g = a.G + 15L * le;
       for (int d = 0; d < 3; d++) dx[d] = m.vcoord[3L * v + d] - m.ccoord[3L * le + d];
-      for (int k = 0; k < 5; k++)
-        wl[k] = wl[k] + (a.G[(3 * k) * C + le] * dx[0] + a.G[(3 * k + 1) * C + le] * dx[1] +
-                         a.G[(3 * k + 2) * C + le] * dx[2]);
+      for (int k = 0; k < 5; k++) wl[k] = wl[k] + (g[3 * k] * dx[0] + g[3 * k + 1] * dx[1] + g[3 * k + 2] * dx[2]);
     }
     if (re >= 0) {
       for (int k = 0; k < 5; k++) u[k] = a.U[k * C + re];
       to_prim(u, wr);
       if (a.second_order) {
         double dx[3];
+        const double *g = a.G + 15L * re;
         for (int d = 0; d < 3; d++) dx[d] = m.vcoord[3L * v + d] - m.ccoord[3L * re + d];
         for (int k = 0; k < 5; k++)
-          wr[k] = wr[k] + (a.G[(3 * k) * C + re] * dx[0] + a.G[(3 * k + 1) * C + re] * dx[1] +
-                           a.G[(3 * k + 2) * C + re] * dx[2]);
+          wr[k] = wr[k] + (g[3 * k] * dx[0] + g[3 * k + 1] * dx[1] + g[3 * k + 2] * dx[2]);
       }
     } else {
       double ul[5], ug[5];
@@ -123,7 +167,7 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
     }
   }
   const bool wall = is_wall(bc, re);
-  const bool excl = bc.excl && m.vbound[v];
+  const bool excl = bc.excl && L.vbound[vl];
 
   // ---- nodal solve
   double lam_l = 0.0, lam_r = 0.0, vbar = 0.0;
@@ -167,37 +211,32 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
         }
       }
       if (!active || excl) addm = addr = false;
-      double c9[9];
-      c9[0] = addm ? cm * (n[0] * n[0]) : 0.0;
-      c9[1] = addm ? cm * (n[0] * n[1]) : 0.0;
-      c9[2] = addm ? cm * (n[0] * n[2]) : 0.0;
-      c9[3] = addm ? cm * (n[1] * n[1]) : 0.0;
-      c9[4] = addm ? cm * (n[1] * n[2]) : 0.0;
-      c9[5] = addm ? cm * (n[2] * n[2]) : 0.0;
-      c9[6] = addr ? cr * n[0] : 0.0;
-      c9[7] = addr ? cr * n[1] : 0.0;
-      c9[8] = addr ? cr * n[2] : 0.0;
-      for (int c = 0; c < 9; c++) s_red[wib][c][lane] = c9[c];
+      L.red[0][lane] = addm ? cm * (n[0] * n[0]) : 0.0;
+      L.red[1][lane] = addm ? cm * (n[0] * n[1]) : 0.0;
+      L.red[2][lane] = addm ? cm * (n[0] * n[2]) : 0.0;
+      L.red[3][lane] = addm ? cm * (n[1] * n[1]) : 0.0;
+      L.red[4][lane] = addm ? cm * (n[1] * n[2]) : 0.0;
+      L.red[5][lane] = addm ? cm * (n[2] * n[2]) : 0.0;
+      L.red[6][lane] = addr ? cr * n[0] : 0.0;
+      L.red[7][lane] = addr ? cr * n[1] : 0.0;
+      L.red[8][lane] = addr ? cr * n[2] : 0.0;
       wsync();
       for (int p = lane; p < nvw * 9; p += 64) {
         int vv = p / 9, c = p - vv * 9;
-        double acc = 0.0;
-        for (int j = s_vtab[wib][vv]; j < s_vtab[wib][vv + 1]; j++) acc = acc + s_red[wib][c][j];
-        s_sum[wib][vv][c] = acc;
+        L.sum[vv][c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
       }
       wsync();
       if (lane < nvw) {
-        const int vv = vb + lane;
         double vn3[3] = {0.0, 0.0, 0.0};
-        if (!(bc.excl && m.vbound[vv])) {
-          const double *S = s_sum[wib][lane];
+        if (!(bc.excl && L.vbound[lane])) {
+          const double *S = L.sum[lane];
           double M[9] = {S[0], S[1], S[2], S[1], S[3], S[4], S[2], S[4], S[5]};
           double R[3] = {S[6], S[7], S[8]};
           if (bc.bc_style == 2) {
-            double B[3] = {m.Bp[vv], m.Bp[m.V + vv], m.Bp[2 * m.V + vv]};
+            double B[3] = {L.bp[lane][0], L.bp[lane][1], L.bp[lane][2]};
             double mb = dmax(dmax(fabs(B[0]), fabs(B[1])), fabs(B[2]));
             if (mb > 1e-8) {
-              pinv3(M);
+              pinv3_sym(S, M);
               double mR[3], mB[3];
               for (int i = 0; i < 3; i++) {
                 mR[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
@@ -213,28 +252,28 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
               lu_solve<3>(M, R, vn3);
             }
           } else {
-            pinv3(M);
+            pinv3_sym(S, M);
             for (int i = 0; i < 3; i++) vn3[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
           }
         }
-        for (int d = 0; d < 3; d++) s_vn[wib][lane][d] = vn3[d];
+        for (int d = 0; d < 3; d++) L.vn[lane][d] = vn3[d];
       }
       wsync();
-      for (int d = 0; d < 3; d++) v_node[d] = s_vn[wib][vl][d];
+      for (int d = 0; d < 3; d++) v_node[d] = L.vn[vl][d];
     }
     if (a.vp && lane < nvw)
-      for (int d = 0; d < 3; d++) a.vp[d * m.V + vb + lane] = s_vn[wib][lane][d];
+      for (int d = 0; d < 3; d++) a.vp[d * m.V + vb + lane] = L.vn[lane][d];
     if (iso) {
       // warea = minval(area)/area over the vertex's sub-faces
-      s_red[wib][0][lane] = active ? area : 1e300;
+      L.red[0][lane] = active ? area : 1e300;
       wsync();
       if (lane < nvw) {
-        double mn = s_red[wib][0][s_vtab[wib][lane]];
-        for (int j = s_vtab[wib][lane] + 1; j < s_vtab[wib][lane + 1]; j++) mn = dmin(mn, s_red[wib][0][j]);
-        s_sum[wib][lane][0] = mn;
+        double mn = L.red[0][L.vtab[lane]];
+        for (int j = L.vtab[lane] + 1; j < L.vtab[lane + 1]; j++) mn = dmin(mn, L.red[0][j]);
+        L.sum[lane][0] = mn;
       }
       wsync();
-      warea = s_sum[wib][vl][0] / area;
+      warea = L.sum[vl][0] / area;
       wsync();
     }
   } else if (SCHEME == kMultiPointPressure || SCHEME == kMultiPointPressurePh) {
@@ -266,19 +305,16 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
         c1 = (1.0 / lam_r + 1.0 / lam_l) * area * vbar;
         c2 = (1.0 / lam_r + 1.0 / lam_l) * area;
       }
-      s_red[wib][0][lane] = active ? c1 : 0.0;
-      s_red[wib][1][lane] = active ? c2 : 0.0;
+      L.red[0][lane] = active ? c1 : 0.0;
+      L.red[1][lane] = active ? c2 : 0.0;
       wsync();
       if (lane < nvw) {
-        double a1 = 0.0, a2 = 0.0;
-        for (int j = s_vtab[wib][lane]; j < s_vtab[wib][lane + 1]; j++) {
-          a1 = a1 + s_red[wib][0][j];
-          a2 = a2 + s_red[wib][1][j];
-        }
-        s_sum[wib][lane][0] = a1 / a2;
+        double a1 = seg_sum(L.red[0], L.vtab[lane], L.vtab[lane + 1]);
+        double a2 = seg_sum(L.red[1], L.vtab[lane], L.vtab[lane + 1]);
+        L.sum[lane][0] = a1 / a2;
       }
       wsync();
-      p_nodal = s_sum[wib][vl][0];
+      p_nodal = L.sum[vl][0];
       wsync();
     }
   }
@@ -313,36 +349,52 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
   // ---- accumulate area*flux per sub-element slot of the vertex
   int lslot = -1, rslot = -1;
   if (active) {
-    const int sb = m.se_off[v], nse = m.se_off[v + 1] - sb;
-    for (int k = 0; k < nse; k++) {
-      int c = m.se_cell[sb + k];
-      if (c == le) lslot = k;
-      if (c == re) rslot = k;
+    const int kb = L.setab[vl], ke = L.setab[vl + 1];
+    for (int k = kb; k < ke; k++) {
+      int c = staged ? L.secell[k] : m.se_cell[se0 + k];
+      if (c == le) lslot = k - kb;
+      if (c == re) rslot = k - kb;
     }
-    if (re < 0) rslot = -1;
   }
   for (int k = 0; k < 5; k++) {
-    s_red[wib][k][lane] = area * fx[k];
-    s_red[wib][6 + k][lane] = area * fx[5 + k];
+    L.red[k][lane] = area * fx[k];
+    L.red[6 + k][lane] = area * fx[5 + k];
   }
-  s_red[wib][5][lane] = area * dmax(0.0, -sl);
-  s_red[wib][11][lane] = area * dmax(0.0, sr);
-  s_lslot[wib][lane] = lslot;
-  s_rslot[wib][lane] = rslot;
+  L.red[5][lane] = area * dmax(0.0, -sl);
+  L.red[11][lane] = area * dmax(0.0, sr);
+  L.lslot[lane] = lslot;
+  L.rslot[lane] = rslot;
   wsync();
-  const int nsew = s_setab[wib][nvw];
-  for (int p = lane; p < 6 * nsew; p += 64) {
-    int comp = p / nsew, q = p - comp * nsew;
+  for (int q = lane; q < nsew_g; q += 64) {
     int vv = 0;
     for (int i = 1; i < nvw; i++)
-      if (s_setab[wib][i] <= q) vv = i;
-    int k = q - s_setab[wib][vv];
-    double acc = 0.0;
-    for (int j = s_vtab[wib][vv]; j < s_vtab[wib][vv + 1]; j++) {
-      if (s_lslot[wib][j] == k) acc = acc + s_red[wib][comp][j];
-      if (s_rslot[wib][j] == k) acc = acc + s_red[wib][6 + comp][j];
+      if (L.setab[i] <= q) vv = i;
+    const int k = q - L.setab[vv];
+    const int jb = L.vtab[vv], je = L.vtab[vv + 1];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j0 = jb; j0 < je; j0 += 4) {
+      int ls[4], rs[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        int j = j0 + t < je ? j0 + t : je - 1;
+        ls[t] = (j0 + t < je) ? L.lslot[j] : -2;
+        rs[t] = (j0 + t < je) ? L.rslot[j] : -2;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        int j = j0 + t < je ? j0 + t : je - 1;
+        if (ls[t] == k)
+#pragma unroll
+          for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[c][j];
+        if (rs[t] == k)
+#pragma unroll
+          for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[6 + c][j];
+      }
     }
-    a.P[(long)comp * m.SE + se0 + q] = acc;
+    const long se = staged ? L.seid[q] : m.v_seid[se0 + q];
+    double *out = a.P + 6 * se;
+#pragma unroll
+    for (int c = 0; c < 6; c++) out[c] = acc[c];
   }
 }
 

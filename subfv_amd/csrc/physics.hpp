@@ -464,4 +464,76 @@ SFV_DEV void pinv3(double *A) {
     }
 }
 
+// Pseudo-inverse of a symmetric 3x3 matrix (a00, a01, a02, a11, a12, a22),
+// written column-major into P: cyclic Jacobi rotations give M = Q L Q^T,
+// the singular values are |L|, and eigenvalues with |l| <= 3e-12 * max |l|
+// are dropped, the cut-off of linear_solver_module.F90:703-711.
+SFV_DEV void pinv3_sym(const double *S6, double *P) {
+  double a00 = S6[0], a01 = S6[1], a02 = S6[2], a11 = S6[3], a12 = S6[4], a22 = S6[5];
+  double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int sweep = 0; sweep < 16; sweep++) {
+    bool rotated = false;
+#pragma unroll
+    for (int pq = 0; pq < 3; pq++) {
+      // (p, q, r) = (0,1,2), (0,2,1), (1,2,0)
+      double app, aqq, apq, arp, arq;
+      if (pq == 0) { app = a00; aqq = a11; apq = a01; arp = a02; arq = a12; }
+      else if (pq == 1) { app = a00; aqq = a22; apq = a02; arp = a01; arq = a12; }
+      else { app = a11; aqq = a22; apq = a12; arp = a01; arq = a02; }
+      if (apq == 0.0) continue;
+      if (fabs(apq) <= 1e-15 * sqrt(fabs(app) * fabs(aqq))) continue;
+      rotated = true;
+      double theta = (aqq - app) / (2.0 * apq);
+      double t;
+      if (fabs(theta) > 1e100)
+        t = 0.5 / theta;
+      else
+        t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+      double c = 1.0 / sqrt(1.0 + t * t);
+      double s = t * c;
+      double npp = app - t * apq, nqq = aqq + t * apq;
+      double nrp = c * arp - s * arq, nrq = s * arp + c * arq;
+      const int p = (pq == 2) ? 1 : 0, qq = (pq == 0) ? 1 : 2;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        double vp = q[i + 3 * p], vq = q[i + 3 * qq];
+        q[i + 3 * p] = c * vp - s * vq;
+        q[i + 3 * qq] = s * vp + c * vq;
+      }
+      if (pq == 0) { a00 = npp; a11 = nqq; a01 = 0.0; a02 = nrp; a12 = nrq; }
+      else if (pq == 1) { a00 = npp; a22 = nqq; a02 = 0.0; a01 = nrp; a12 = nrq; }
+      else { a11 = npp; a22 = nqq; a12 = 0.0; a01 = nrp; a02 = nrq; }
+    }
+    if (!rotated) break;
+  }
+  double l[3] = {a00, a11, a22};
+  // order by decreasing |l| (stable)
+  int o0 = 0, o1 = 1, o2 = 2;
+  if (fabs(l[o0]) < fabs(l[o1])) { int t = o0; o0 = o1; o1 = t; }
+  if (fabs(l[o1]) < fabs(l[o2])) {
+    int t = o1; o1 = o2; o2 = t;
+    if (fabs(l[o0]) < fabs(l[o1])) { t = o0; o0 = o1; o1 = t; }
+  }
+  const int ord[3] = {o0, o1, o2};
+  const double tol = 3.0 * 1e-12 * fabs(l[o0]);
+  double ip[3], qs[9];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int ok = ord[k];
+    const double lk = (ok == 0) ? l[0] : ((ok == 1) ? l[1] : l[2]);
+    ip[k] = (fabs(lk) > tol) ? 1.0 / lk : 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) qs[i + 3 * k] = (ok == 0) ? q[i] : ((ok == 1) ? q[i + 3] : q[i + 6]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) acc = acc + qs[i + 3 * k] * (ip[k] * qs[j + 3 * k]);
+      P[i + 3 * j] = acc;
+    }
+}
+
 }  // namespace sfv

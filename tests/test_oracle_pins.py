@@ -69,6 +69,26 @@ def test_pinv3_matches_numpy_including_cutoff():
         assert (np.abs(got).max() > 1e9) == (s[2] == 1e-11)  # 1e-11 is kept, 1e-13 and 0 dropped
 
 
+def test_pinv3_sym_matches_numpy_including_cutoff():
+    fn = O.lib().om_test_pinv3_sym
+    fn.argtypes = [np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+    rng = np.random.default_rng(1)
+    for k in range(40):
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        lam = np.array([1.0 + rng.random(), 0.2 + rng.random(), [1.0, 1e-13, 1e-11, 0.0][k % 4]])
+        if k % 5 == 0:
+            lam[1] = -lam[1]  # indefinite matrices too
+        A = Q @ np.diag(lam) @ Q.T
+        A = 0.5 * (A + A.T)
+        ref = np.linalg.pinv(A, rcond=3e-12, hermitian=True)
+        cm = np.ascontiguousarray(A.T).copy()
+        fn(cm)
+        got = cm.reshape(3, 3).T
+        kept = np.abs(lam)[np.abs(lam) > 3e-12 * np.abs(lam).max()]
+        rtol = max(1e-12, 50 * 2.2e-16 * kept.max() / kept.min())
+        assert np.allclose(got, ref, rtol=rtol, atol=rtol * np.abs(ref).max()), (k, lam)
+
+
 def test_lu_solve_noswp_matches_numpy():
     fn = O.lib().om_test_lu_solve_noswp
     dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
