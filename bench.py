@@ -85,11 +85,25 @@ def main():
     ap.add_argument("--first-order", action="store_true", help="first-order forward-Euler stage (M1 headline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters against known bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path is single-GPU; multi-GPU sharding is design-only).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}), flush=True)
+        return
 
     from subfv_amd import meshgen as mg
     from subfv_amd.api import Solver, make_config

@@ -145,6 +145,44 @@ std::vector<int> pack_waves(const subfv::HostMesh &hm) {
   return w;
 }
 
+// vertex groups of the node sweep (<= kGrpVert vertices, <= kGrpSub sub-faces,
+// <= kGrpSlots slots) and, per node-sorted slot, the group-local indices of
+// the sub-faces of its vertex that touch its cell, in increasing order (the
+// order compute_rhs_around_vert accumulates them, ns_euler_module.F90:506-517)
+std::string pack_groups(const subfv::HostMesh &hm, const std::vector<int> &secell, std::vector<int> &grp,
+                        std::vector<uint32_t> &src) {
+  grp.clear();
+  src.assign(hm.nse, 0xFFFFFFFFu);
+  int v = 0;
+  while (v < hm.nv) {
+    const int vb = v, sf0 = hm.sf_off[v], se0 = hm.v_seoff[v];
+    while (v < hm.nv && v - vb < kGrpVert && hm.sf_off[v + 1] - sf0 <= kGrpSub &&
+           hm.v_seoff[v + 1] - se0 <= kGrpSlots)
+      v++;
+    if (v == vb) return "vertex " + std::to_string(v + 1) + " has more than " + std::to_string(kGrpSub) +
+                        " sub-faces or " + std::to_string(kGrpSlots) + " sub-elements";
+    const int hdr[kGrpHdr] = {vb, v - vb, sf0, hm.sf_off[v] - sf0, se0, hm.v_seoff[v] - se0, 0, 0};
+    grp.insert(grp.end(), hdr, hdr + kGrpHdr);
+    for (int u = vb; u < v; u++)
+      for (int q = hm.v_seoff[u]; q < hm.v_seoff[u + 1]; q++) {
+        const int cell = secell[q];
+        int cnt = 0;
+        uint32_t code = 0xFFFFFFFFu;
+        for (int s = hm.sf_off[u]; s < hm.sf_off[u + 1]; s++)
+          if (hm.sf_le[s] == cell || hm.sf_re[s] - 1 == cell) {
+            if (cnt == kSlotSrc)
+              return "a cell has more than " + std::to_string(kSlotSrc) + " sub-faces at vertex " +
+                     std::to_string(u + 1);
+            code &= ~(0xFFu << (8 * cnt));
+            code |= (uint32_t)(s - sf0) << (8 * cnt);
+            cnt++;
+          }
+        src[q] = code;
+      }
+  }
+  return "";
+}
+
 int upload_mesh(subfv_ctx *c) {
   const subfv::HostMesh &h = c->hm;
   DevMesh &d = c->dm;
@@ -200,6 +238,15 @@ int upload_mesh(subfv_ctx *c) {
   std::vector<int> waves = pack_waves(h);
   d.nwaves = (int)waves.size() - 1;
   UP(wave_v, waves);
+  {
+    std::vector<int> grp;
+    std::vector<uint32_t> src;
+    std::string e = pack_groups(h, secell, grp, src);
+    if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
+    d.ngroups = (int)(grp.size() / kGrpHdr);
+    UP(grp, grp);
+    UP(se_src, src);
+  }
 #undef UP
   if ((r = dalloc(c, &d.vvol, h.nv))) return r;
   if ((r = dalloc(c, &d.vbound, h.nv))) return r;
@@ -277,10 +324,10 @@ int launch_flux(subfv_ctx *c, const double *U, const double *G, double *P, bool 
   a.err = c->derr;
   a.t = &c->st->t;
   a.second_order = G != nullptr;
-  int nb = (c->dm.nwaves + kFluxWaves - 1) / kFluxWaves;
+  const int nb = c->dm.ngroups;
   a.per_xcd = (nb + 7) / 8;
   a.nblocks = nb;
-  dim3 grid(8 * a.per_xcd), blk(64 * kFluxWaves);
+  dim3 grid(8 * a.per_xcd), blk(kGrpThreads);
   Prof pf(c, G ? "node_flux_o2" : "node_flux");
   switch (c->bc.scheme) {
     case kMultiPoint: hipLaunchKernelGGL(k_node_flux<kMultiPoint>, grid, blk, 0, c->stream, a); break;

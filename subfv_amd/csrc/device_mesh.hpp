@@ -9,9 +9,21 @@ namespace sfv {
 constexpr int kWaveSub = 64;   // sub-faces per wavefront (one lane each)
 constexpr int kWaveVert = 16;  // vertices per wavefront at most
 
+// Vertex groups of the node sweep: one 256-thread workgroup per group of
+// consecutive vertices.  Group-local sub-face indices fit a byte (0xFF ends a
+// slot's contributor list) and the group's vertices fit one wavefront.
+constexpr int kGrpThreads = 256;
+constexpr int kGrpSub = 255;    // sub-faces per group at most
+constexpr int kGrpVert = 64;    // vertices per group at most
+constexpr int kGrpSlots = 256;  // sub-element slots per group at most
+constexpr int kGrpHdr = 8;      // ints per group header: vb, nv, sf0, nsf, se0, nse, -, -
+constexpr int kSlotSrc = 4;     // sub-faces of one cell meeting at a vertex (pyramid apex: 4)
+constexpr int kRedStride = 257; // padded LDS row (doubles) against bank conflicts
+
 struct DevMesh {
   long V = 0, C = 0, F = 0, SF = 0, SE = 0;
   int nwaves = 0;
+  int ngroups = 0;
   // vertices
   double *vcoord = nullptr;   // [V][3]
   double *vvol = nullptr;     // [V]
@@ -22,6 +34,9 @@ struct DevMesh {
   int *v_seid = nullptr;      // [SE] node-sorted slot -> cell-major sub-element id
   double *Bp = nullptr;       // [3][V] wall normal (depends on the BC set)
   int *wave_v = nullptr;      // [nwaves+1] first vertex of each wavefront
+  int *grp = nullptr;         // [ngroups][kGrpHdr] vertex-group headers
+  uint32_t *se_src = nullptr; // [SE] node-sorted slot -> its contributing group-local
+                              // sub-faces, one byte each, increasing, 0xFF = none
   // faces
   int *f_voff = nullptr, *f_vert = nullptr, *f_left = nullptr, *f_right = nullptr;
   int *f_sfoff = nullptr, *f_sf = nullptr;

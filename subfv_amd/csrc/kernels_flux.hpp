@@ -2,18 +2,21 @@
 // (src/ns/ns_euler_module.F90:151-339, 341-519) with the nodal solvers of
 // src/ns/ns_euler_rs_module.F90:288-553, 681-758.
 //
-// Mapping: one wavefront owns a run of consecutive vertices whose sub-faces
-// (contiguous, vertex-major) fit in its 64 lanes; lane = sub-face.  Every
-// per-vertex sum (nodal matrix and right-hand side, per-sub-element flux) is
-// formed in reference order by one lane reading the vertex's sub-face terms
-// from LDS (loads batched, additions in order), so results are deterministic
-// and atomic-free.  Per-vertex data and the wave's sub-element slots are
-// staged in LDS up front so the dependent global loads happen once.
+// Mapping: one 256-thread workgroup owns a group of consecutive vertices
+// whose sub-faces (contiguous, vertex-major) fit in its lanes; thread =
+// sub-face.  Every per-vertex sum (nodal matrix and right-hand side, flux per
+// sub-element) is formed in reference order by one thread reading the
+// vertex's sub-face terms from LDS (loads batched, additions in order), so
+// results are deterministic and atomic-free.  The 3x3 nodal solves of the
+// group run in the first wavefront only (one lane per vertex), so the other
+// wavefronts issue no solve instructions.
 //
 // Output: one row of six partial sums per sub-element, stored cell-major so
 // the gather streams it: P[6*se + k] = sum over the vertex's sub-faces of
 // area*flux_k (k < 5) and P[6*se + 5] = sum of area*max(0, -s), where se is
-// the reference's (cell-major) sub-element id.
+// the reference's (cell-major) sub-element id.  Thread q of the group owns
+// node-sorted slot q; its contributors (the group-local sub-faces touching
+// the slot's cell, in increasing order) come from the static table se_src.
 #pragma once
 #include "device_mesh.hpp"
 #include "physics.hpp"
@@ -31,7 +34,7 @@ struct FluxArgs {
   const double *t;      // physical time (time-dependent BCs)
   int second_order;
   int per_xcd;          // logical blocks per XCD slice
-  int nblocks;          // logical blocks
+  int nblocks;          // logical blocks (= vertex groups)
 };
 
 SFV_DEV void wsync() {
@@ -39,6 +42,8 @@ SFV_DEV void wsync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+constexpr int kFluxWaves = 4;  // wavefronts per workgroup of the wavefront-segment kernels
 
 // blockIdx -> logical block so that each XCD gets a contiguous slice of the
 // (spatially ordered) work and its L2 sees neighbouring cells.
@@ -62,72 +67,81 @@ SFV_DEV double seg_sum(const double *row, int b, int e) {
   return acc;
 }
 
-constexpr int kFluxWaves = 4;
-constexpr int kWaveSlots = 128;  // sub-element slots staged per wavefront
+// largest i in [0, n) with tab[i] <= x (tab increasing, tab[0] = 0 <= x)
+SFV_DEV int seg_of(const int *tab, int n, int x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 struct FluxLds {
-  double red[12][kWaveSub];
-  double sum[kWaveVert][9];
-  double vn[kWaveVert][3];
-  double bp[kWaveVert][3];
-  int vtab[kWaveVert + 1];
-  int setab[kWaveVert + 1];
-  int lslot[kWaveSub];
-  int rslot[kWaveSub];
-  int secell[kWaveSlots];
-  int seid[kWaveSlots];
-  uint8_t vbound[kWaveVert];
+  double red[12][kRedStride];
+  double sum[kGrpVert][9];
+  double vn[kGrpVert][3];
+  double bp[kGrpVert][3];
+  int vtab[kGrpVert + 1];
+  int lc[kGrpThreads];
+  uint8_t vbound[kGrpVert];
 };
 
+// Nodal velocity of one vertex from its summed matrix S (6 distinct entries)
+// and right-hand side R (S[6..8]): ns_euler_rs_module.F90:406-421.
+SFV_DEV void nodal_velocity(const BCParams &bc, const double *S, const double *B, double *vn3) {
+  double M[9] = {S[0], S[1], S[2], S[1], S[3], S[4], S[2], S[4], S[5]};
+  const double R[3] = {S[6], S[7], S[8]};
+  if (bc.bc_style == 2) {
+    const double mb = dmax(dmax(fabs(B[0]), fabs(B[1])), fabs(B[2]));
+    if (mb > 1e-8) {
+      pinv3_sym(S, M);
+      double mR[3], mB[3];
+      for (int i = 0; i < 3; i++) {
+        mR[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
+        mB[i] = M[i] * B[0] + M[i + 3] * B[1] + M[i + 6] * B[2];
+      }
+      const double pb = (mR[0] * B[0] + mR[1] * B[1] + mR[2] * B[2]) / (mB[0] * B[0] + mB[1] * B[1] + mB[2] * B[2]);
+      double rr[3];
+      for (int d = 0; d < 3; d++) rr[d] = R[d] - pb * B[d];
+      for (int i = 0; i < 3; i++) vn3[i] = M[i] * rr[0] + M[i + 3] * rr[1] + M[i + 6] * rr[2];
+      const double sc = (vn3[0] * B[0] + vn3[1] * B[1] + vn3[2] * B[2]) / (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+      for (int d = 0; d < 3; d++) vn3[d] = vn3[d] - sc * B[d];
+    } else {
+      lu_solve<3>(M, R, vn3);
+    }
+  } else {
+    pinv3_sym(S, M);
+    for (int i = 0; i < 3; i++) vn3[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
+  }
+}
+
 template <int SCHEME>
-__global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
-  __shared__ FluxLds lds_all[kFluxWaves];
+__global__ __launch_bounds__(kGrpThreads) void k_node_flux(FluxArgs a) {
+  __shared__ FluxLds L;
   const DevMesh &m = a.m;
   const BCParams &bc = a.bc;
-  const int wib = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  FluxLds &L = lds_all[wib];
   const int lb = xcd_block(a.per_xcd);
-  if (lb >= a.nblocks) return;
-  const int w = lb * kFluxWaves + wib;
-  if (w >= m.nwaves) return;  // whole wavefront leaves; only wave-level sync below
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  const int tid = threadIdx.x;
+  const int *gh = m.grp + kGrpHdr * (long)lb;
+  const int vb = gh[0], nv = gh[1], sf0 = gh[2], nsf = gh[3], se0 = gh[4], nse = gh[5];
 
-  const int vb = m.wave_v[w], ve = m.wave_v[w + 1];
-  const int nvw = ve - vb;
-  const int sf0 = m.sf_off[vb], se0 = m.se_off[vb];
-  const int nsew_g = m.se_off[ve] - se0;
-  const bool staged = nsew_g <= kWaveSlots;
-  // ---- stage per-vertex data and the wave's sub-element slots
-  if (lane <= nvw) {
-    L.vtab[lane] = m.sf_off[vb + lane] - sf0;
-    L.setab[lane] = m.se_off[vb + lane] - se0;
-  }
-  if (lane < nvw) {
-    L.vbound[lane] = m.vbound[vb + lane];
+  // ---- stage per-vertex data; issue the sub-face and slot loads
+  if (tid <= nv) L.vtab[tid] = m.sf_off[vb + tid] - sf0;
+  if (tid < nv) {
+    L.vbound[tid] = m.vbound[vb + tid];
     if (SCHEME == kMultiPoint || SCHEME == kMultiPointIso) {
-      L.bp[lane][0] = m.Bp[vb + lane];
-      L.bp[lane][1] = m.Bp[m.V + vb + lane];
-      L.bp[lane][2] = m.Bp[2 * m.V + vb + lane];
+      L.bp[tid][0] = m.Bp[vb + tid];
+      L.bp[tid][1] = m.Bp[m.V + vb + tid];
+      L.bp[tid][2] = m.Bp[2 * m.V + vb + tid];
     }
   }
-  if (staged) {
-    for (int q = lane; q < nsew_g; q += 64) {
-      L.secell[q] = m.se_cell[se0 + q];
-      L.seid[q] = m.v_seid[se0 + q];
-    }
-  }
-  wsync();
-  const int nsfw = L.vtab[nvw];
-  const bool active = lane < nsfw;
-  int vl = 0;
-  for (int i = 1; i < nvw; i++)
-    if (L.vtab[i] <= lane) vl = i;
-  const int v = vb + vl;
-  // inactive lanes mirror a valid sub-face of the wave so every load is in
+  const bool active = tid < nsf;
+  // inactive threads mirror the group's first sub-face so every load is in
   // bounds; their contributions are masked below
-  const long s = (nsfw > 0) ? sf0 + (active ? lane : 0) : 0;
-
-  // ---- sub-face data and left/right primitive states (reconstruct_lr_w)
+  const long s = sf0 + (active ? tid : 0);
   double n[3], area;
   n[0] = m.sfn[s];
   n[1] = m.sfn[m.SF + s];
@@ -135,6 +149,20 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
   area = m.sfn[3 * m.SF + s];
   const int le = m.sf_le[s];
   const int re = m.sf_re[s];
+  const bool has_slot = tid < nse;
+  uint32_t src = 0xFFFFFFFFu;
+  int slot_cell = -1;
+  long slot_se = 0;
+  if (has_slot) {
+    src = m.se_src[se0 + tid];
+    slot_cell = m.se_cell[se0 + tid];
+    slot_se = m.v_seid[se0 + tid];
+  }
+  __syncthreads();
+  const int vl = seg_of(L.vtab, nv, tid);
+  const int v = vb + vl;
+
+  // ---- left/right primitive states (reconstruct_lr_w)
   const long C = m.C;
   double wl[5], wr[5];
   {
@@ -211,70 +239,45 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
         }
       }
       if (!active || excl) addm = addr = false;
-      L.red[0][lane] = addm ? cm * (n[0] * n[0]) : 0.0;
-      L.red[1][lane] = addm ? cm * (n[0] * n[1]) : 0.0;
-      L.red[2][lane] = addm ? cm * (n[0] * n[2]) : 0.0;
-      L.red[3][lane] = addm ? cm * (n[1] * n[1]) : 0.0;
-      L.red[4][lane] = addm ? cm * (n[1] * n[2]) : 0.0;
-      L.red[5][lane] = addm ? cm * (n[2] * n[2]) : 0.0;
-      L.red[6][lane] = addr ? cr * n[0] : 0.0;
-      L.red[7][lane] = addr ? cr * n[1] : 0.0;
-      L.red[8][lane] = addr ? cr * n[2] : 0.0;
-      wsync();
-      for (int p = lane; p < nvw * 9; p += 64) {
-        int vv = p / 9, c = p - vv * 9;
+      L.red[0][tid] = addm ? cm * (n[0] * n[0]) : 0.0;
+      L.red[1][tid] = addm ? cm * (n[0] * n[1]) : 0.0;
+      L.red[2][tid] = addm ? cm * (n[0] * n[2]) : 0.0;
+      L.red[3][tid] = addm ? cm * (n[1] * n[1]) : 0.0;
+      L.red[4][tid] = addm ? cm * (n[1] * n[2]) : 0.0;
+      L.red[5][tid] = addm ? cm * (n[2] * n[2]) : 0.0;
+      L.red[6][tid] = addr ? cr * n[0] : 0.0;
+      L.red[7][tid] = addr ? cr * n[1] : 0.0;
+      L.red[8][tid] = addr ? cr * n[2] : 0.0;
+      __syncthreads();
+      for (int p = tid; p < nv * 9; p += kGrpThreads) {
+        const int vv = p / 9, c = p - vv * 9;
         L.sum[vv][c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
       }
-      wsync();
-      if (lane < nvw) {
+      __syncthreads();
+      if (tid < nv) {  // first wavefront only (nv <= 64)
         double vn3[3] = {0.0, 0.0, 0.0};
-        if (!(bc.excl && L.vbound[lane])) {
-          const double *S = L.sum[lane];
-          double M[9] = {S[0], S[1], S[2], S[1], S[3], S[4], S[2], S[4], S[5]};
-          double R[3] = {S[6], S[7], S[8]};
-          if (bc.bc_style == 2) {
-            double B[3] = {L.bp[lane][0], L.bp[lane][1], L.bp[lane][2]};
-            double mb = dmax(dmax(fabs(B[0]), fabs(B[1])), fabs(B[2]));
-            if (mb > 1e-8) {
-              pinv3_sym(S, M);
-              double mR[3], mB[3];
-              for (int i = 0; i < 3; i++) {
-                mR[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
-                mB[i] = M[i] * B[0] + M[i + 3] * B[1] + M[i + 6] * B[2];
-              }
-              double pb = (mR[0] * B[0] + mR[1] * B[1] + mR[2] * B[2]) / (mB[0] * B[0] + mB[1] * B[1] + mB[2] * B[2]);
-              double rr[3];
-              for (int d = 0; d < 3; d++) rr[d] = R[d] - pb * B[d];
-              for (int i = 0; i < 3; i++) vn3[i] = M[i] * rr[0] + M[i + 3] * rr[1] + M[i + 6] * rr[2];
-              double sc = (vn3[0] * B[0] + vn3[1] * B[1] + vn3[2] * B[2]) / (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
-              for (int d = 0; d < 3; d++) vn3[d] = vn3[d] - sc * B[d];
-            } else {
-              lu_solve<3>(M, R, vn3);
-            }
-          } else {
-            pinv3_sym(S, M);
-            for (int i = 0; i < 3; i++) vn3[i] = M[i] * R[0] + M[i + 3] * R[1] + M[i + 6] * R[2];
-          }
+        if (!(bc.excl && L.vbound[tid])) {
+          const double B[3] = {L.bp[tid][0], L.bp[tid][1], L.bp[tid][2]};
+          nodal_velocity(bc, L.sum[tid], B, vn3);
         }
-        for (int d = 0; d < 3; d++) L.vn[lane][d] = vn3[d];
+        for (int d = 0; d < 3; d++) L.vn[tid][d] = vn3[d];
       }
-      wsync();
+      __syncthreads();
       for (int d = 0; d < 3; d++) v_node[d] = L.vn[vl][d];
     }
-    if (a.vp && lane < nvw)
-      for (int d = 0; d < 3; d++) a.vp[d * m.V + vb + lane] = L.vn[lane][d];
+    if (a.vp && tid < nv)
+      for (int d = 0; d < 3; d++) a.vp[d * m.V + vb + tid] = L.vn[tid][d];
     if (iso) {
       // warea = minval(area)/area over the vertex's sub-faces
-      L.red[0][lane] = active ? area : 1e300;
-      wsync();
-      if (lane < nvw) {
-        double mn = L.red[0][L.vtab[lane]];
-        for (int j = L.vtab[lane] + 1; j < L.vtab[lane + 1]; j++) mn = dmin(mn, L.red[0][j]);
-        L.sum[lane][0] = mn;
+      L.red[0][tid] = active ? area : 1e300;
+      __syncthreads();
+      if (tid < nv) {
+        double mn = L.red[0][L.vtab[tid]];
+        for (int j = L.vtab[tid] + 1; j < L.vtab[tid + 1]; j++) mn = dmin(mn, L.red[0][j]);
+        L.sum[tid][0] = mn;
       }
-      wsync();
+      __syncthreads();
       warea = L.sum[vl][0] / area;
-      wsync();
     }
   } else if (SCHEME == kMultiPointPressure || SCHEME == kMultiPointPressurePh) {
     const double rhol = wl[0], pl = wl[4], al = sound_speed(wl);
@@ -305,17 +308,16 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
         c1 = (1.0 / lam_r + 1.0 / lam_l) * area * vbar;
         c2 = (1.0 / lam_r + 1.0 / lam_l) * area;
       }
-      L.red[0][lane] = active ? c1 : 0.0;
-      L.red[1][lane] = active ? c2 : 0.0;
-      wsync();
-      if (lane < nvw) {
-        double a1 = seg_sum(L.red[0], L.vtab[lane], L.vtab[lane + 1]);
-        double a2 = seg_sum(L.red[1], L.vtab[lane], L.vtab[lane + 1]);
-        L.sum[lane][0] = a1 / a2;
+      L.red[0][tid] = active ? c1 : 0.0;
+      L.red[1][tid] = active ? c2 : 0.0;
+      __syncthreads();
+      if (tid < nv) {
+        const double s1 = seg_sum(L.red[0], L.vtab[tid], L.vtab[tid + 1]);
+        const double s2 = seg_sum(L.red[1], L.vtab[tid], L.vtab[tid + 1]);
+        L.sum[tid][0] = s1 / s2;
       }
-      wsync();
+      __syncthreads();
       p_nodal = L.sum[vl][0];
-      wsync();
     }
   }
 
@@ -346,53 +348,26 @@ __global__ __launch_bounds__(256) void k_node_flux(FluxArgs a) {
     flux_multi_point_pressure<true>(wl, wr, n, fx, p_nodal, lam_l, lam_r, sl, sr);
   }
 
-  // ---- accumulate area*flux per sub-element slot of the vertex
-  int lslot = -1, rslot = -1;
-  if (active) {
-    const int kb = L.setab[vl], ke = L.setab[vl + 1];
-    for (int k = kb; k < ke; k++) {
-      int c = staged ? L.secell[k] : m.se_cell[se0 + k];
-      if (c == le) lslot = k - kb;
-      if (c == re) rslot = k - kb;
-    }
-  }
+  // ---- area*flux per sub-element slot (ns_euler_module.F90:506-517)
   for (int k = 0; k < 5; k++) {
-    L.red[k][lane] = area * fx[k];
-    L.red[6 + k][lane] = area * fx[5 + k];
+    L.red[k][tid] = area * fx[k];
+    L.red[6 + k][tid] = area * fx[5 + k];
   }
-  L.red[5][lane] = area * dmax(0.0, -sl);
-  L.red[11][lane] = area * dmax(0.0, sr);
-  L.lslot[lane] = lslot;
-  L.rslot[lane] = rslot;
-  wsync();
-  for (int q = lane; q < nsew_g; q += 64) {
-    int vv = 0;
-    for (int i = 1; i < nvw; i++)
-      if (L.setab[i] <= q) vv = i;
-    const int k = q - L.setab[vv];
-    const int jb = L.vtab[vv], je = L.vtab[vv + 1];
+  L.red[5][tid] = area * dmax(0.0, -sl);
+  L.red[11][tid] = area * dmax(0.0, sr);
+  L.lc[tid] = le;
+  __syncthreads();
+  if (has_slot) {
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int j0 = jb; j0 < je; j0 += 4) {
-      int ls[4], rs[4];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        int j = j0 + t < je ? j0 + t : je - 1;
-        ls[t] = (j0 + t < je) ? L.lslot[j] : -2;
-        rs[t] = (j0 + t < je) ? L.rslot[j] : -2;
-      }
+    for (int e = 0; e < kSlotSrc; e++) {
+      const int j = (src >> (8 * e)) & 0xFFu;
+      if (j == 0xFF) break;
+      const int r0 = (L.lc[j] == slot_cell) ? 0 : 6;
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        int j = j0 + t < je ? j0 + t : je - 1;
-        if (ls[t] == k)
-#pragma unroll
-          for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[c][j];
-        if (rs[t] == k)
-#pragma unroll
-          for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[6 + c][j];
-      }
+      for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[r0 + c][j];
     }
-    const long se = staged ? L.seid[q] : m.v_seid[se0 + q];
-    double *out = a.P + 6 * se;
+    double *out = a.P + 6 * slot_se;
 #pragma unroll
     for (int c = 0; c < 6; c++) out[c] = acc[c];
   }
