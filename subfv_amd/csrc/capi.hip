@@ -127,24 +127,6 @@ int check_err(subfv_ctx *c) {
   return 0;
 }
 
-// wave packing: consecutive vertices, <= 64 sub-faces and <= 16 vertices each
-std::vector<int> pack_waves(const subfv::HostMesh &hm) {
-  std::vector<int> w{0};
-  int v = 0;
-  while (v < hm.nv) {
-    int cnt = 0, nvw = 0;
-    while (v < hm.nv && nvw < kWaveVert) {
-      int ns = hm.sf_off[v + 1] - hm.sf_off[v];
-      if (cnt + ns > kWaveSub) break;
-      cnt += ns;
-      nvw++;
-      v++;
-    }
-    w.push_back(v);
-  }
-  return w;
-}
-
 // vertex groups of the node sweep (<= kGrpVert vertices, <= kGrpSub sub-faces,
 // <= kGrpSlots slots) and, per node-sorted slot, the group-local indices of
 // the sub-faces of its vertex that touch its cell, in increasing order (the
@@ -235,9 +217,6 @@ int upload_mesh(subfv_ctx *c) {
     cperm[e] = code;
   }
   UP(c_perm, cperm);
-  std::vector<int> waves = pack_waves(h);
-  d.nwaves = (int)waves.size() - 1;
-  UP(wave_v, waves);
   {
     std::vector<int> grp;
     std::vector<uint32_t> src;
@@ -372,11 +351,11 @@ int launch_grad(subfv_ctx *c, const double *U, int method) {
   g.VMM = c->VMM;
   g.G = c->G;
   g.method = method;
-  int nb = (c->dm.nwaves + kFluxWaves - 1) / kFluxWaves;
+  const int nb = c->dm.ngroups;
   g.per_xcd = (nb + 7) / 8;
   g.nblocks = nb;
   Prof pf(c, "grad");
-  hipLaunchKernelGGL(k_nodal_grad, dim3(8 * g.per_xcd), dim3(64 * kFluxWaves), 0, c->stream, g);
+  hipLaunchKernelGGL(k_nodal_grad, dim3(8 * g.per_xcd), dim3(kGrpThreads), 0, c->stream, g);
   hipLaunchKernelGGL(k_cell_grad, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -455,6 +434,8 @@ int mesh_i32(const subfv::HostMesh &h, const char *name, int32_t *out, int64_t c
   else if (n == "elem_face") list({}, h.e_face, 1);
   else if (n == "elem_sub_elem") for (int i = 0; i < h.nse; i++) v.push_back(i + 1);
   else if (n == "elem_tag") list({}, h.e_tag, 0);
+  else if (n == "elem_n_neigh_by_vert") for (int i = 0; i < h.ne; i++) v.push_back(h.e_nbvoff[i + 1] - h.e_nbvoff[i]);
+  else if (n == "elem_neigh_by_vert") list({}, h.e_nbv, 1);
   else if (n == "face_n_vert") for (int i = 0; i < h.nf; i++) v.push_back(h.f_voff[i + 1] - h.f_voff[i]);
   else if (n == "face_vert") list({}, h.f_vert, 1);
   else if (n == "face_left") list({}, h.f_left, 1);
@@ -598,9 +579,6 @@ int subfv_mesh_build(subfv_ctx *c, int32_t n_vert, const double *coord, int32_t 
   std::string e = subfv::build_host_mesh(c->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
                                          bf_vert, bf_tag);
   if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
-  for (int v = 0; v < c->hm.nv; v++)
-    if (c->hm.sf_off[v + 1] - c->hm.sf_off[v] > kWaveSub)
-      return fail(c, SUBFV_E_INVALID, "a vertex has more than 64 sub-faces");
   c->b2d = boundary_2d != 0;
   int r = upload_mesh(c);
   if (r) return r;
@@ -622,7 +600,7 @@ int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
   if (w == "n_sub_faces") return c->hm.nsf;
   if (w == "n_sub_elems") return c->hm.nse;
   if (w == "n_max_sub_faces") return c->hm.nmax_sfn;
-  if (w == "n_waves") return c->dm.nwaves;
+  if (w == "n_groups") return c->dm.ngroups;
   return -1;
 }
 

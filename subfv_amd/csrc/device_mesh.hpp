@@ -6,9 +6,6 @@
 
 namespace sfv {
 
-constexpr int kWaveSub = 64;   // sub-faces per wavefront (one lane each)
-constexpr int kWaveVert = 16;  // vertices per wavefront at most
-
 // Vertex groups of the node sweep: one 256-thread workgroup per group of
 // consecutive vertices.  Group-local sub-face indices fit a byte (0xFF ends a
 // slot's contributor list) and the group's vertices fit one wavefront.
@@ -22,7 +19,6 @@ constexpr int kRedStride = 257; // padded LDS row (doubles) against bank conflic
 
 struct DevMesh {
   long V = 0, C = 0, F = 0, SF = 0, SE = 0;
-  int nwaves = 0;
   int ngroups = 0;
   // vertices
   double *vcoord = nullptr;   // [V][3]
@@ -33,7 +29,6 @@ struct DevMesh {
   int *v_eoff = nullptr, *v_elem = nullptr;  // vertex -> cells (reference order)
   int *v_seid = nullptr;      // [SE] node-sorted slot -> cell-major sub-element id
   double *Bp = nullptr;       // [3][V] wall normal (depends on the BC set)
-  int *wave_v = nullptr;      // [nwaves+1] first vertex of each wavefront
   int *grp = nullptr;         // [ngroups][kGrpHdr] vertex-group headers
   uint32_t *se_src = nullptr; // [SE] node-sorted slot -> its contributing group-local
                               // sub-faces, one byte each, increasing, 0xFF = none

@@ -253,35 +253,30 @@ struct GradArgs {
 };
 
 struct GradLds {
-  double red[28][kWaveSub];
-  double sum[kWaveVert][24];
-  int vtab[kWaveVert + 1];
-  uint8_t vbound[kWaveVert];
+  double red[16][kRedStride];
+  double sum[kGrpVert][28];
+  int vtab[kGrpVert + 1];
+  uint8_t vbound[kGrpVert];
 };
 
-// compute_nodal_grad (ns_euler_recon_module.F90:453-490), lane = sub-face,
-// and the per-vertex bounds of relaxed_maximum_principle (:82-97) taken over
-// the cells on either side of the vertex's sub-faces (which are exactly the
-// cells around the vertex; min and max do not depend on the order).
-__global__ __launch_bounds__(256) void k_nodal_grad(GradArgs a) {
-  __shared__ GradLds lds_all[kFluxWaves];
+// compute_nodal_grad (ns_euler_recon_module.F90:453-490) over the vertex
+// groups of the node sweep, thread = sub-face, and the per-vertex bounds of
+// relaxed_maximum_principle (:82-97) taken over the cells on either side of
+// the vertex's sub-faces (exactly the cells around the vertex; min and max do
+// not depend on the order).  Sums run in sub-face order (seg_sum); the 3x3
+// pseudo-inverses of the group run in the first wavefront.
+__global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
+  __shared__ GradLds L;
   const DevMesh &m = a.m;
-  const int wib = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  GradLds &L = lds_all[wib];
   const int lb = xcd_block(a.per_xcd);
-  if (lb >= a.nblocks) return;
-  const int w = lb * kFluxWaves + wib;
-  if (w >= m.nwaves) return;
-  const int vb = m.wave_v[w], ve = m.wave_v[w + 1];
-  const int nvw = ve - vb;
-  const int sf0 = m.sf_off[vb];
-  if (lane <= nvw) L.vtab[lane] = m.sf_off[vb + lane] - sf0;
-  if (lane < nvw) L.vbound[lane] = m.vbound[vb + lane];
-  wsync();
-  const int nsfw = L.vtab[nvw];
-  const bool active = lane < nsfw;
-  const long s = (nsfw > 0) ? sf0 + (active ? lane : 0) : 0;
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  const int tid = threadIdx.x;
+  const int *gh = m.grp + kGrpHdr * (long)lb;
+  const int vb = gh[0], nv = gh[1], sf0 = gh[2], nsf = gh[3];
+  if (tid <= nv) L.vtab[tid] = m.sf_off[vb + tid] - sf0;
+  if (tid < nv) L.vbound[tid] = m.vbound[vb + tid];
+  const bool active = tid < nsf;
+  const long s = sf0 + (active ? tid : 0);
   const long C = m.C, SF = m.SF;
   const int le = m.sf_le[s], re = m.sf_re[s];
   const bool inner = active && re >= 0;
@@ -294,44 +289,53 @@ __global__ __launch_bounds__(256) void k_nodal_grad(GradArgs a) {
   to_prim(ur, wr);
   const double n[3] = {m.sfn[s], m.sfn[SF + s], m.sfn[2 * SF + s]};
   const double ar = m.sfn[3 * SF + s];
-  for (int k = 0; k < 5; k++) {
-    double dw = wr[k] - wl[k];
-    for (int d = 0; d < 3; d++) L.red[3 * k + d][lane] = inner ? ar * (n[d] * dw) : 0.0;
-  }
+  // pass 1: sum a n (x) (x_R - x_L) and the bounds
   for (int b = 0; b < 3; b++) {
     double dx = inner ? m.ccoord[3L * re + b] - m.ccoord[3L * le + b] : 0.0;
-    for (int d = 0; d < 3; d++) L.red[15 + d + 3 * b][lane] = inner ? ar * (n[d] * dx) : 0.0;
+    for (int d = 0; d < 3; d++) L.red[d + 3 * b][tid] = inner ? ar * (n[d] * dx) : 0.0;
   }
   // bounds: the left cell always, the right cell when it is a cell
-  L.red[24][lane] = active ? dmin(wl[0], inner ? wr[0] : 1e100) : 1e100;
-  L.red[25][lane] = active ? dmax(wl[0], inner ? wr[0] : 0.0) : 0.0;
-  L.red[26][lane] = active ? dmin(wl[4], inner ? wr[4] : 1e100) : 1e100;
-  L.red[27][lane] = active ? dmax(wl[4], inner ? wr[4] : 0.0) : 0.0;
-  wsync();
-  for (int p = lane; p < nvw * 24; p += 64) {
-    int vv = p / 24, c = p - vv * 24;
+  L.red[9][tid] = active ? dmin(wl[0], inner ? wr[0] : 1e100) : 1e100;
+  L.red[10][tid] = active ? dmax(wl[0], inner ? wr[0] : 0.0) : 0.0;
+  L.red[11][tid] = active ? dmin(wl[4], inner ? wr[4] : 1e100) : 1e100;
+  L.red[12][tid] = active ? dmax(wl[4], inner ? wr[4] : 0.0) : 0.0;
+  __syncthreads();
+  for (int p = tid; p < nv * 9; p += kGrpThreads) {
+    const int vv = p / 9, c = p - vv * 9;
+    L.sum[vv][15 + c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
+  }
+  for (int p = tid; p < nv * 4; p += kGrpThreads) {
+    const int vv = p >> 2, c = p & 3;
+    const int jb = L.vtab[vv], je = L.vtab[vv + 1];
+    const double *row = L.red[9 + c];
+    double mm = row[jb];
+    if (c & 1)
+      for (int j = jb + 1; j < je; j++) mm = dmax(mm, row[j]);
+    else
+      for (int j = jb + 1; j < je; j++) mm = dmin(mm, row[j]);
+    // the bounds start from 1e100 / 0.0 (ns_euler_recon_module.F90:83-86)
+    mm = (c & 1) ? dmax(0.0, mm) : dmin(1e100, mm);
+    a.VMM[4L * (vb + vv) + c] = mm;
+  }
+  __syncthreads();
+  // pass 2: sum a n (x) (W_R - W_L)
+  for (int k = 0; k < 5; k++) {
+    double dw = wr[k] - wl[k];
+    for (int d = 0; d < 3; d++) L.red[3 * k + d][tid] = inner ? ar * (n[d] * dw) : 0.0;
+  }
+  __syncthreads();
+  for (int p = tid; p < nv * 15; p += kGrpThreads) {
+    const int vv = p / 15, c = p - vv * 15;
     L.sum[vv][c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
   }
-  if (lane < nvw) {
-    const int jb = L.vtab[lane], je = L.vtab[lane + 1];
-    double mm[4] = {1e100, 0.0, 1e100, 0.0};
-    for (int j = jb; j < je; j++) {
-      mm[0] = dmin(mm[0], L.red[24][j]);
-      mm[1] = dmax(mm[1], L.red[25][j]);
-      mm[2] = dmin(mm[2], L.red[26][j]);
-      mm[3] = dmax(mm[3], L.red[27][j]);
-    }
-    double *o = a.VMM + 4L * (vb + lane);
-    for (int q = 0; q < 4; q++) o[q] = mm[q];
-  }
-  wsync();
-  if (lane < nvw) {
-    const double *S = L.sum[lane];
+  __syncthreads();
+  if (tid < nv) {  // first wavefront only (nv <= 64)
+    const double *S = L.sum[tid];
     double M[9];
     for (int q = 0; q < 9; q++) M[q] = S[15 + q];
     pinv3(M);
-    const bool bnd = L.vbound[lane];
-    double *o = a.NG + 15L * (vb + lane);
+    const bool bnd = L.vbound[tid];
+    double *o = a.NG + 15L * (vb + tid);
     for (int k = 0; k < 5; k++)
       for (int d = 0; d < 3; d++) {
         double x = M[d] * S[3 * k] + M[d + 3] * S[3 * k + 1] + M[d + 6] * S[3 * k + 2];

@@ -390,6 +390,26 @@ std::string build_host_mesh(HostMesh &m, int nv, const double *coord, int ne, co
       int e = m.f_left[f];
       if (m.e_tag[e] < 0 || m.e_tag[e] > tag) m.e_tag[e] = tag;
     }
+  // ---- compute_neigh_by_vert (:595-654): the cells around the cell's
+  // vertices, vertex by vertex in the cell's order, first occurrence kept
+  m.e_nbvoff.assign(ne + 1, 0);
+  m.e_nbv.clear();
+  {
+    std::vector<int> seen(ne, -1);
+    for (int e = 0; e < ne; e++) {
+      for (int j = m.e_voff[e]; j < m.e_voff[e + 1]; j++) {
+        const int v = m.e_vert[j];
+        for (int k = m.v_eoff[v]; k < m.v_eoff[v + 1]; k++) {
+          const int o = m.v_elem[k];
+          if (o != e && seen[o] != e) {
+            seen[o] = e;
+            m.e_nbv.push_back(o);
+          }
+        }
+      }
+      m.e_nbvoff[e + 1] = (int)m.e_nbv.size();
+    }
+  }
   m.nmax_sfn = 0;
   m.nmax_sen = 0;
   for (int v = 0; v < nv; v++) {

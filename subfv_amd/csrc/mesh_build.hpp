@@ -33,6 +33,8 @@ struct HostMesh {
   std::vector<int> v_seoff, v_se;
   // element tags (smallest BC tag on its boundary faces, -1 if none)
   std::vector<int> e_tag;
+  // cells sharing a vertex with the cell (neigh_by_vert)
+  std::vector<int> e_nbvoff, e_nbv;
 };
 
 // kind: gmsh kinds (4 tet, 5 hex, 6 prism, 7 pyramid); elem_vert: 1-based
