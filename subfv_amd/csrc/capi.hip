@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kCellThreads) void k_flux_limiting(DevMesh m, const
   for (int k = 0; k < 5; k++) {
     r[k] = rhs[k * C + c];
     r2[k] = rhs2[k * C + c];
-    x[k] = U[k * C + c] + r[k] / cf;
+    x[k] = U[5 * c + k] + r[k] / cf;
     y[k] = (r2[k] - r[k]) / cf;
   }
   double om = max_omega_pos(x, y);
@@ -683,12 +683,12 @@ int subfv_set_time(subfv_ctx *c, double t) {
 
 int subfv_state_put(subfv_ctx *c, const double *sol) {
   if (!c || !c->has_mesh || !sol) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
-  return host_to_dev_soa(c, sol, 5, c->dm.C, c->U[c->cur]);
+  return host_to_dev_soa(c, sol, 1, 5 * c->dm.C, c->U[c->cur]);
 }
 
 int subfv_state_get(subfv_ctx *c, double *sol) {
   if (!c || !c->has_mesh || !sol) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
-  return dev_to_host_soa(c, c->U[c->cur], 5, c->dm.C, sol);
+  return dev_to_host_soa(c, c->U[c->cur], 1, 5 * c->dm.C, sol);
 }
 
 int64_t subfv_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) {
@@ -696,7 +696,7 @@ int64_t subfv_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) 
   std::string n(name);
   const long C = c->dm.C, V = c->dm.V;
   struct Item { const char *nm; double *p; int nc; long n; };
-  Item items[] = {{"sol", c->U[c->cur], 5, C},   {"rhs", c->rhs, 5, C},      {"rhs_o2", c->rhs2, 5, C},
+  Item items[] = {{"sol", c->U[c->cur], 1, 5 * C}, {"rhs", c->rhs, 5, C},      {"rhs_o2", c->rhs2, 5, C},
                   {"sum_lambda", c->sl, 1, C},   {"sum_lambda_o2", c->sl2, 1, C},
                   // gradients are stored in the reference layout already
                   {"cell_grad", c->G, 1, 15 * C}, {"nodal_grad", c->NG, 1, 15 * V}, {"vp", c->vp, 3, V}};
@@ -713,7 +713,7 @@ int64_t subfv_put_f64(subfv_ctx *c, const char *name, const double *in, int64_t 
   std::string n(name);
   const long C = c->dm.C;
   struct Item { const char *nm; double *p; int nc; long n; };
-  Item items[] = {{"sol", c->U[c->cur], 5, C}, {"rhs", c->rhs, 5, C}, {"rhs_o2", c->rhs2, 5, C},
+  Item items[] = {{"sol", c->U[c->cur], 1, 5 * C}, {"rhs", c->rhs, 5, C}, {"rhs_o2", c->rhs2, 5, C},
                   {"sum_lambda", c->sl, 1, C}, {"cell_grad", c->G, 1, 15 * C}};
   for (auto &it : items)
     if (n == it.nm) {

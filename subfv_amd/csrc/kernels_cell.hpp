@@ -143,9 +143,9 @@ __global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
 
 struct UpdateArgs {
   DevMesh m;
-  const double *U;    // base state [5][C]
+  const double *U;    // base state [C][5]
   const double *U0;   // combination partner (w0 != 0) or null
-  double *Uout;       // [5][C]
+  double *Uout;       // [C][5]
   const double *rhs, *rhs2, *sl;
   const double *diag; // [C][25] column-major 5x5 blocks, or null (c*I diagonal)
   double *blk;
@@ -164,7 +164,7 @@ SFV_DEV double stage_cell(const UpdateArgs &a, long c, double *u, double *delta)
   const StepState *st = a.st;
   double r[5];
   for (int k = 0; k < 5; k++) {
-    u[k] = a.U[k * C + c];
+    u[k] = a.U[5 * c + k];
     r[k] = a.rhs[k * C + c];
   }
   const double vol = a.m.cvol[c];
@@ -205,8 +205,8 @@ SFV_DEV void store_stage(const UpdateArgs &a, long c, const double *u, const dou
   const long C = a.m.C;
   for (int k = 0; k < 5; k++) {
     double x = u[k] + a.a2 * omega * delta[k];
-    if (a.U0) x = a.w0 * a.U0[k * C + c] + a.w1 * x;
-    a.Uout[k * C + c] = x;
+    if (a.U0) x = a.w0 * a.U0[5 * c + k] + a.w1 * x;
+    a.Uout[5 * c + k] = x;
   }
 }
 
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kCellThreads) void k_update(UpdateArgs a) {
   StepState *st = a.st;
   if (st->done || st->err) {  // uniform over the grid: the state is frozen
     if (c < a.m.C)
-      for (int k = 0; k < 5; k++) a.Uout[k * a.m.C + c] = a.U[k * a.m.C + c];
+      for (int k = 0; k < 5; k++) a.Uout[5 * c + k] = a.U[5 * c + k];
     return;
   }
   double om_c = 1.0;
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kCellThreads) void k_update_fixup(UpdateArgs a) {
 // ---------------------------------------------------------------- gradients
 struct GradArgs {
   DevMesh m;
-  const double *U;   // [5][C]
+  const double *U;   // [C][5]
   double *NG;        // [V][15] nodal gradients, 3*k + d fastest
   double *VMM;       // [V][4] rho_min, rho_max, p_min, p_max over the vertex's cells
   double *G;         // [C][15]
@@ -282,8 +282,8 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   const bool inner = active && re >= 0;
   double ul[5], ur[5], wl[5], wr[5];
   for (int k = 0; k < 5; k++) {
-    ul[k] = a.U[k * C + le];
-    ur[k] = a.U[k * C + (re >= 0 ? re : le)];
+    ul[k] = a.U[5L * le + k];
+    ur[k] = a.U[5L * (re >= 0 ? re : le) + k];
   }
   to_prim(ul, wl);
   to_prim(ur, wr);
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
   // physical_slope_limiter, with the neighbourhood bounds gathered alongside
   const double tol = 1e-8;
   double u[5], w[5];
-  for (int k = 0; k < 5; k++) u[k] = a.U[k * C + e];
+  for (int k = 0; k < 5; k++) u[k] = a.U[5 * e + k];
   to_prim(u, w);
   const double xc[3] = {m.ccoord[3L * e], m.ccoord[3L * e + 1], m.ccoord[3L * e + 2]};
   double omega = 1.0;

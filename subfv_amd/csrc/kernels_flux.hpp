@@ -26,7 +26,7 @@ namespace sfv {
 struct FluxArgs {
   DevMesh m;
   BCParams bc;
-  const double *U;      // [5][C]
+  const double *U;      // [C][5] conservative state, sol(5, n_elems)
   const double *G;      // [C][15] cell gradients, 3*k + d fastest (second order)
   double *P;            // [SE][6] cell-major
   double *vp;           // [3][V] nodal velocity output or null
@@ -118,7 +118,7 @@ SFV_DEV void nodal_velocity(const BCParams &bc, const double *S, const double *B
 }
 
 template <int SCHEME>
-__global__ __launch_bounds__(kGrpThreads) void k_node_flux(FluxArgs a) {
+__global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
   __shared__ FluxLds L;
   const DevMesh &m = a.m;
   const BCParams &bc = a.bc;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kGrpThreads) void k_node_flux(FluxArgs a) {
   double wl[5], wr[5];
   {
     double u[5];
-    for (int k = 0; k < 5; k++) u[k] = a.U[k * C + le];
+    for (int k = 0; k < 5; k++) u[k] = a.U[5L * le + k];
     to_prim(u, wl);
     if (a.second_order) {
       double dx[3];
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kGrpThreads) void k_node_flux(FluxArgs a) {
       for (int k = 0; k < 5; k++) wl[k] = wl[k] + (g[3 * k] * dx[0] + g[3 * k + 1] * dx[1] + g[3 * k + 2] * dx[2]);
     }
     if (re >= 0) {
-      for (int k = 0; k < 5; k++) u[k] = a.U[k * C + re];
+      for (int k = 0; k < 5; k++) u[k] = a.U[5L * re + k];
       to_prim(u, wr);
       if (a.second_order) {
         double dx[3];
