@@ -205,18 +205,21 @@ int upload_mesh(subfv_ctx *c) {
   UP(e_foff, h.e_foff);
   UP(e_face, h.e_face);
   // each cell's sub-elements in increasing vertex id (the scatter order of
-  // compute_euler_flux, ns_euler_module.F90:321-336), 3 bits per entry
-  std::vector<uint32_t> cperm(h.ne);
-  for (int e = 0; e < h.ne; e++) {
-    int b = h.e_voff[e], n = h.e_voff[e + 1] - b;
-    std::pair<int, int> pv[8];
-    for (int j = 0; j < n; j++) pv[j] = {h.se_vert[b + j], j};
-    std::sort(pv, pv + n);
-    uint32_t code = 0;
-    for (int j = 0; j < n; j++) code |= (uint32_t)pv[j].second << (3 * j);
-    cperm[e] = code;
+  // compute_euler_flux, ns_euler_module.F90:321-336), as node-sorted slots:
+  // the gather reads the node sweep's per-slot partials through this table
+  {
+    std::vector<int> slot_of(h.nse);
+    for (int q = 0; q < h.nse; q++) slot_of[h.v_se[q]] = q;
+    std::vector<int> cslot(h.nse);
+    for (int e = 0; e < h.ne; e++) {
+      int b = h.e_voff[e], n = h.e_voff[e + 1] - b;
+      std::pair<int, int> pv[8];
+      for (int j = 0; j < n; j++) pv[j] = {h.se_vert[b + j], slot_of[b + j]};
+      std::sort(pv, pv + n);
+      for (int j = 0; j < n; j++) cslot[b + j] = pv[j].second;
+    }
+    UP(c_slot, cslot);
   }
-  UP(c_perm, cperm);
   {
     std::vector<int> grp;
     std::vector<uint32_t> src;

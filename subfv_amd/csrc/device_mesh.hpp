@@ -49,7 +49,7 @@ struct DevMesh {
   int *se_cell = nullptr;     // [SE] node-sorted slot -> cell
   // cells
   int *e_voff = nullptr, *e_vert = nullptr, *e_foff = nullptr, *e_face = nullptr;
-  uint32_t *c_perm = nullptr; // [C] local sub-element order by increasing vertex id, 3 bits each
+  int *c_slot = nullptr;      // [SE] cell-major: the cell's node-sorted slots in increasing vertex id
   double *ccoord = nullptr;   // [C][3]
   double *cvol = nullptr;     // [C]
 };

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kCellThreads) void k_finalize_omega(const double *b
 
 struct GatherArgs {
   DevMesh m;
-  const double *P;   // first-order partials [SE][6], cell-major
+  const double *P;   // first-order partials [SE][6], node-sorted slots
   const double *P2;  // second-order partials or null
   double *rhs;       // [5][C]
   double *rhs2;      // [5][C] (second order)
@@ -109,8 +109,9 @@ struct GatherArgs {
 };
 
 // rhs(:,c) = -sum of the partials of c's vertices in increasing vertex id,
-// sum_lambda(c) = 1e-12 + ... (ns_euler_module.F90:181-183, 330-335), then
-// dt = cfl*min(vol/sum_lambda) (:546-560) via the last block.
+// sum_lambda(c) = 1e-12 + ... (ns_euler_module.F90:181-183, 330-335), and
+// the per-block minima of vol/sum_lambda that k_finalize_dt turns into
+// dt = cfl*min(vol/sum_lambda) (:546-560).
 __global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
   const DevMesh &m = a.m;
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,11 +119,10 @@ __global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
   if (c < m.C) {
     const long C = m.C;
     const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
-    const uint32_t perm = m.c_perm[c];
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double lam = 1e-12;
-    for (int q = 0; q < nv; q++) {
-      const double *p = a.P + 6L * (b + ((perm >> (3 * q)) & 7u));
+    for (int j = 0; j < nv; j++) {
+      const double *p = a.P + 6L * m.c_slot[b + j];
       for (int k = 0; k < 5; k++) r[k] = r[k] - p[k];
       lam = lam + p[5];
     }
@@ -130,8 +130,8 @@ __global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
     a.sl[c] = lam;
     if (a.P2) {
       double r2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int q = 0; q < nv; q++) {
-        const double *p = a.P2 + 6L * (b + ((perm >> (3 * q)) & 7u));
+      for (int j = 0; j < nv; j++) {
+        const double *p = a.P2 + 6L * m.c_slot[b + j];
         for (int k = 0; k < 5; k++) r2[k] = r2[k] - p[k];
       }
       for (int k = 0; k < 5; k++) a.rhs2[k * C + c] = r2[k];

@@ -11,12 +11,12 @@
 // group run in the first wavefront only (one lane per vertex), so the other
 // wavefronts issue no solve instructions.
 //
-// Output: one row of six partial sums per sub-element, stored cell-major so
-// the gather streams it: P[6*se + k] = sum over the vertex's sub-faces of
-// area*flux_k (k < 5) and P[6*se + 5] = sum of area*max(0, -s), where se is
-// the reference's (cell-major) sub-element id.  Thread q of the group owns
-// node-sorted slot q; its contributors (the group-local sub-faces touching
-// the slot's cell, in increasing order) come from the static table se_src.
+// Output: one row of six partial sums per node-sorted sub-element slot q
+// (vertex-major, so a group's rows are one contiguous run): P[6*q + k] = sum
+// over the vertex's sub-faces of area*flux_k (k < 5) and P[6*q + 5] = sum of
+// area*max(0, -s).  Thread q of the group owns slot q; its contributors (the
+// group-local sub-faces touching the slot's cell, in increasing order) come
+// from the static table se_src.
 #pragma once
 #include "device_mesh.hpp"
 #include "physics.hpp"
@@ -28,7 +28,7 @@ struct FluxArgs {
   BCParams bc;
   const double *U;      // [C][5] conservative state, sol(5, n_elems)
   const double *G;      // [C][15] cell gradients, 3*k + d fastest (second order)
-  double *P;            // [SE][6] cell-major
+  double *P;            // [SE][6] node-sorted slots
   double *vp;           // [3][V] nodal velocity output or null
   int *err;
   const double *t;      // physical time (time-dependent BCs)
@@ -152,11 +152,9 @@ __global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
   const bool has_slot = tid < nse;
   uint32_t src = 0xFFFFFFFFu;
   int slot_cell = -1;
-  long slot_se = 0;
   if (has_slot) {
     src = m.se_src[se0 + tid];
     slot_cell = m.se_cell[se0 + tid];
-    slot_se = m.v_seid[se0 + tid];
   }
   __syncthreads();
   const int vl = seg_of(L.vtab, nv, tid);
@@ -367,7 +365,7 @@ __global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
 #pragma unroll
       for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[r0 + c][j];
     }
-    double *out = a.P + 6 * slot_se;
+    double *out = a.P + 6L * (se0 + tid);
 #pragma unroll
     for (int c = 0; c < 6; c++) out[c] = acc[c];
   }
