@@ -213,3 +213,74 @@ class Solver:
         dt = C.c_double(0.0)
         self._check(_lib.lib().subfv_run(self.h, n_steps, time_scheme, C.byref(t), C.byref(dt)))
         return t.value, dt.value
+
+
+class Bcsr:
+    """bcsr_matrix_type resident on the device (sparse_bcsr_linear_module.F90:6-11):
+    m block rows of b x b blocks, 1-based row(m+1) and col(nnzb), v(b,b,nnzb)
+    column-major blocks."""
+
+    def __init__(self, solver: "Solver", m: int, b: int, row, col, v):
+        self.m, self.b = int(m), int(b)
+        self._row = _lib.i32(row)
+        self._col = _lib.i32(col)
+        self._v = _lib.f64(v).ravel()
+        self.nnzb = int(self._col.size)
+        if self._row.size != self.m + 1 or self._v.size != self.nnzb * self.b * self.b:
+            raise ValueError("inconsistent BCSR arrays")
+        h = C.c_void_p()
+        st = _lib.lib().subfv_bcsr_create(solver.h, C.byref(h), self.m, self.b, self.nnzb, _lib.iptr(self._row),
+                                          _lib.iptr(self._col), _lib.dptr(self._v))
+        if st:
+            raise SubfvError(st, _lib.lib().subfv_error(solver.h).decode())
+        self.h = h
+        self.solver = solver
+
+    @property
+    def n(self) -> int:
+        return self.m * self.b
+
+    def _vec(self, x):
+        x = _lib.f64(x).ravel()
+        if x.size != self.n:
+            raise ValueError(f"vector of length {x.size}, expected {self.n}")
+        return x
+
+    def matmul(self, x):
+        """y = A x (bcsr_matmul)."""
+        x = self._vec(x)
+        y = np.zeros(self.n)
+        self.solver._check(_lib.lib().subfv_bcsr_matmul(self.h, _lib.dptr(x), _lib.dptr(y)))
+        return y
+
+    def diag(self, rhs):
+        """x = D^-1 rhs, block-Jacobi with lu_solve_noswp (bcsr_diag)."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n)
+        self.solver._check(_lib.lib().subfv_bcsr_diag(self.h, _lib.dptr(x), _lib.dptr(rhs)))
+        return x
+
+    def gmres_diag(self, rhs, tol: float, restart: int = 30, precision: int = 0):
+        """bcsr_gmres_diag from x = 0 with absolute tolerance tol; precision 1
+        runs the SpMV and block-Jacobi of each cycle in FP32.  Returns
+        (x, conv, {"restarts", "rho", "spmv"})."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n)
+        conv = C.c_int32(0)
+        stats = np.zeros(3)
+        st = _lib.lib().subfv_bcsr_gmres_diag(self.h, _lib.dptr(x), _lib.dptr(rhs), tol, restart, precision,
+                                              C.byref(conv), _lib.dptr(stats))
+        if st not in (0, _lib.E_NOT_CONVERGED):
+            self.solver._check(st)
+        return x, bool(conv.value), {"restarts": int(stats[0]), "rho": float(stats[1]), "spmv": int(stats[2])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().subfv_bcsr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import os
+import shutil
 import subprocess
 import sys
 
@@ -25,6 +26,50 @@ HIP_FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FORTRAN, "build")
+# the Fortran host: iso_c_binding module, the reference-named procedures and
+# the subfvns driver; FP contraction off as for the kernels
+F_FLAGS = ["-O2", "-ffp-contract=off"]
+F_MODULES = ["subfv_c.f90", "subfv_ns_gpu.f90"]
+F_PROGRAMS = ["subfvns_gpu.f90", "mesh_check.f90"]
+
+
+def find_flang():
+    for c in ("flang", "amdflang", "/opt/rocm/bin/amdflang", "/opt/rocm/lib/llvm/bin/flang"):
+        p = shutil.which(c)
+        if p:
+            return p
+    return None
+
+
+def build_fortran(verbose: bool = False):
+    """Compiles the Fortran host against libsubfv_hip.so; returns the
+    executables built (none when no Fortran compiler is installed)."""
+    fc = find_flang()
+    if fc is None:
+        return []
+    os.makedirs(FBUILD, exist_ok=True)
+    objs = []
+    for m in F_MODULES:
+        o = os.path.join(FBUILD, m.replace(".f90", ".o"))
+        cmd = [fc, *F_FLAGS, "-J", FBUILD, "-c", os.path.join(FORTRAN, m), "-o", o]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(o)
+    exes = []
+    for p in F_PROGRAMS:
+        exe = os.path.join(FBUILD, p.replace(".f90", ""))
+        cmd = [fc, *F_FLAGS, "-J", FBUILD, os.path.join(FORTRAN, p), *objs, "-o", exe, "-L", HERE,
+               "-lsubfv_hip", "-Wl,-rpath,$ORIGIN/../.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        exes.append(exe)
+    return exes
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, "capi.hip"), os.path.join(CSRC, "mesh_build.cpp")]
     tmp = LIB + ".tmp"
@@ -33,6 +78,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -1,0 +1,172 @@
+! subfvns on one MI355X: the driver of src/ns/ns_main.F90 over the device
+! path.  It reads the reference's namelist file (&INPUT_PARAM,
+! ns_global_data_module.F90:177-213; every entry is accepted, the ones the
+! explicit path uses are honoured) and its msh 2.2 mesh, builds the mesh on
+! the device, sets the initial state (init_uniform or init_1drp,
+! ns_euler_module.F90:20-70), runs the time loop of ns_main.F90:83-107 with
+! the state resident in HBM, prints the final time and the wall time of the
+! loop (as ns_main.F90:113-126 does) and writes the final sol(5, n_elems) to
+! sol_final.bin (stream: int32 n_elems, then the doubles).
+!
+! Usage: subfvns_gpu [namelist file, default input_data.f]
+program subfvns_gpu
+  use iso_c_binding
+  use subfv_c
+  use subfv_ns_gpu
+  implicit none
+
+  integer, parameter :: mnbc = 10
+  ! ---- &INPUT_PARAM entries with the reference's names, types and defaults
+  real(DP) :: Cv_p = 720.19471_DP, mu_p = 1.329e-5_DP, mu0 = 1.716e-5_DP, T0 = 273.15_DP
+  logical :: use_sutherland = .false.
+  real(DP) :: cfl = 1e5_DP, tmax = 1.0_DP
+  character(len=255) :: meshfile_path = "", meshfile = ""
+  logical :: rescale = .false., periodic_mesh = .false., use_cylinder_map = .false.
+  real(DP) :: rescale_factor = 1.0_DP
+  integer :: dim_cylinder_map = 2
+  integer :: n_bc = 0
+  character(len=255) :: bc_name(mnbc) = "", bc_type(mnbc) = "", bc_type_V(mnbc) = "", bc_type_T(mnbc) = ""
+  real(DP) :: bc_val(5, mnbc) = 0.0_DP, bc_val_V(3, mnbc) = 0.0_DP, bc_val_T(mnbc) = 0.0_DP
+  integer :: n_iter_print = 1, n_iter_write_sol = 100, method = 4, bc_style = 1
+  logical :: second_order = .false., exclude_bound_vert = .false., activate_diffusion = .false.
+  logical :: local_time_step = .true., boundary_2d = .false.
+  character(len=255) :: scheme = ""
+  logical :: write_residual = .true.
+  integer :: n_iter_residual = 100, n_max_iter = 10000
+  real(DP) :: final_res = 1e-4_DP
+  logical :: init_uniform = .false., init_sedov = .false., init_isentropic_vortex = .false.
+  logical :: init_gresho = .false., init_potential_flow_2d = .false., init_potential_flow_3d = .false.
+  logical :: init_kelvin = .false., init_double_mach = .false., init_1drp = .false.
+  real(DP) :: sol_uniform(5) = 0.0_DP, r_sedov = 0.0_DP, x1drp = 0.5_DP
+  real(DP) :: sol_w_1drp_l(5) = 0.0_DP, sol_w_1drp_r(5) = 0.0_DP
+  logical :: error_2d = .false., thermal_couette = .false.
+  real(DP) :: error_2d_h = 0.025_DP
+  logical :: init_restart = .false.
+  character(len=255) :: restart_file = ""
+  integer :: id_vtk_restart = 0
+  logical :: compute_coeffs = .false.
+  real(DP) :: pinf = 0.0_DP, rhoinf = 0.0_DP, vinf = 0.0_DP
+  integer :: coeffs_surf = 0
+  logical :: plot_solution_dat = .false.
+  real(DP) :: xmin_dat = 0, xmax_dat = 0, ymin_dat = 0, ymax_dat = 0, zmin_dat = 0, zmax_dat = 0
+  logical :: write_cell_size = .false.
+  real(DP) :: delta_mach_imp = 1.0_DP
+
+  namelist /INPUT_PARAM/ &
+    meshfile_path, meshfile, rescale, rescale_factor, periodic_mesh, &
+    use_cylinder_map, dim_cylinder_map, &
+    n_bc, bc_name, bc_type, bc_val, bc_type_V, bc_val_V, bc_type_T, bc_val_T, &
+    n_iter_print, n_iter_write_sol, second_order, method, &
+    scheme, exclude_bound_vert, activate_diffusion, local_time_step, &
+    bc_style, boundary_2d, write_residual, n_iter_residual, &
+    final_res, n_max_iter, init_uniform, sol_uniform, init_sedov, r_sedov, &
+    init_isentropic_vortex, init_gresho, init_potential_flow_2d, init_potential_flow_3d, &
+    init_kelvin, init_double_mach, init_1drp, sol_w_1drp_l, sol_w_1drp_r, x1drp, &
+    error_2d, error_2d_h, init_restart, restart_file, id_vtk_restart, &
+    compute_coeffs, pinf, rhoinf, vinf, coeffs_surf, plot_solution_dat, &
+    xmin_dat, xmax_dat, ymin_dat, ymax_dat, zmin_dat, zmax_dat, &
+    write_cell_size, delta_mach_imp, mu_p, Cv_p, use_sutherland, mu0, T0, &
+    cfl, tmax, thermal_couette
+
+  real(DP), parameter :: gamma = 1.4_DP
+  character(len=512) :: nlfile
+  integer :: funit, i, n_elems, n_vert
+  type(c_ptr) :: ctx
+  type(raw_mesh) :: mesh
+  type(subfv_config) :: cfg
+  real(DP), allocatable :: sol(:, :), xc(:, :), vol(:)
+  real(DP) :: t, dt, w(5), wall_time
+  integer(8) :: start_count, end_count, count_rate
+  integer(c_int) :: st
+
+  nlfile = "input_data.f"
+  if (command_argument_count() >= 1) call get_command_argument(1, nlfile)
+  open (newunit=funit, file=trim(nlfile), status="old")
+  read (funit, nml=INPUT_PARAM)
+  close (funit)
+
+  ! ---- init_flags (ns_global_data_module.F90:220-330), the explicit-path subset
+  select case (trim(adjustl(scheme)))
+  case ("multi_point"); cfg%scheme_id = 1
+  case ("multi_point_iso"); cfg%scheme_id = 2
+  case ("three_wave"); cfg%scheme_id = 3
+  case ("two_wave"); cfg%scheme_id = 4
+  case ("modified_three_wave"); cfg%scheme_id = 5
+  case ("multi_point_pressure"); cfg%scheme_id = 6
+  case ("multi_point_pressure_ph"); cfg%scheme_id = 8
+  case default
+    print *, "SCHEME NOT RECOGNIZED !"
+    error stop
+  end select
+  if (n_bc > mnbc) error stop "n_bc > mnbc"
+  cfg%n_bc = n_bc
+  do i = 1, n_bc
+    select case (trim(adjustl(bc_type(i))))
+    case ("wall", ""); cfg%bc_euler_id(i) = 1
+    case ("adherence_wall"); cfg%bc_euler_id(i) = 2
+    case ("freestream"); cfg%bc_euler_id(i) = 3
+    case ("outflowsupersonic"); cfg%bc_euler_id(i) = 4
+    case ("inflow_pond"); cfg%bc_euler_id(i) = 5
+    case ("inout_double_mach"); cfg%bc_euler_id(i) = 6
+    case ("double_mach_bottom"); cfg%bc_euler_id(i) = 7
+    case default
+      print *, "BC TYPE NOT RECOGNIZED !"
+      error stop
+    end select
+    cfg%bc_val(:, i) = bc_val(:, i)
+  end do
+  cfg%bc_style = bc_style
+  cfg%boundary_2d = merge(1, 0, boundary_2d)
+  cfg%exclude_bound_vert = merge(1, 0, exclude_bound_vert)
+  cfg%local_time_step = merge(1, 0, local_time_step)
+  cfg%second_order = merge(1, 0, second_order)
+  cfg%method = method
+  cfg%cfl = cfl
+  cfg%tmax = tmax
+  if (activate_diffusion) print *, "note: activate_diffusion is ignored by the explicit device path"
+  if (periodic_mesh .or. rescale .or. use_cylinder_map) error stop "periodic/rescaled/mapped meshes are not supported"
+
+  ! ---- mesh and initial state
+  call check(c_null_ptr, subfv_create(ctx, 0))
+  call read_mesh_msh_2(mesh, meshfile_path, meshfile, n_bc, bc_name)
+  call build_mesh(ctx, mesh, boundary_2d)
+  call check(ctx, subfv_set_config(ctx, cfg))
+  n_elems = mesh_count(ctx, "n_elems")
+  n_vert = mesh_count(ctx, "n_vert")
+  allocate (sol(5, n_elems), xc(3, n_elems), vol(n_elems))
+  call elem_geometry(ctx, n_elems, xc, vol)
+  do i = 1, n_elems
+    if (init_uniform) then
+      w = sol_uniform
+    else if (init_1drp) then
+      w = merge(sol_w_1drp_l, sol_w_1drp_r, xc(1, i) < x1drp)
+    else
+      error stop "only init_uniform and init_1drp are supported"
+    end if
+    sol(1, i) = w(1)
+    sol(2:4, i) = w(1) * w(2:4)
+    sol(5, i) = w(5) / (gamma - 1.0_DP) + 0.5_DP * w(1) * (w(2)**2 + w(3)**2 + w(4)**2)
+  end do
+  call check(ctx, subfv_state_put(ctx, sol))
+  call check(ctx, subfv_set_time(ctx, 0.0_DP))
+  if (final_res > 0.0_DP) print *, "note: the device loop runs n_max_iter+1 steps or to tmax (final_res unused)"
+
+  ! ---- time loop (do while iter <= n_max_iter .and. t < tmax)
+  call system_clock(count_rate=count_rate)
+  call system_clock(start_count)
+  st = subfv_run(ctx, int(n_max_iter + 1, c_int32_t), 0, t, dt)
+  call check(ctx, st)
+  call check(ctx, subfv_state_get(ctx, sol))
+  call system_clock(end_count)
+
+  wall_time = real(end_count - start_count, DP) / real(count_rate, DP)
+  print *, "  --------------------------------------------------------"
+  print *, "  Final time                  =", t
+  print *, "  Total CPU time (wall)       =", wall_time
+  print *, "  --------------------------------------------------------"
+  open (newunit=funit, file="sol_final.bin", access="stream", form="unformatted", status="replace")
+  write (funit) int(n_elems, c_int32_t)
+  write (funit) sol
+  close (funit)
+  call subfv_destroy(ctx)
+end program subfvns_gpu

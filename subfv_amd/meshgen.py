@@ -271,3 +271,45 @@ def blast_state(centroids: np.ndarray, volumes: np.ndarray, h: float, seed: int 
     vdep = volumes[sel].sum()
     u[sel, 4] += energy / vdep
     return u
+
+
+def write_msh(mesh: Mesh, path: str, bc_names=()) -> None:
+    """Writes the mesh as a gmsh 2.2 ASCII file in the form
+    read_mesh_msh_2 reads (mesh_reading_module.F90:989-1324): boundary face
+    j-tags become physical groups named bc_names[j-1] (tag 0 faces get an
+    unnamed group, i.e. the default wall); coordinates are written with 17
+    significant digits so they read back bit-identical."""
+    nb = int(mesh.bf_nvert.size)
+    phys = {}
+    for j, nm in enumerate(bc_names, start=1):
+        phys[j] = (100 + j, nm)
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat"]
+    lines += ["$PhysicalNames", str(len(phys) + 2), '3 1 "volume"', '2 99 "unnamed_boundary"']
+    for j, (tag, nm) in phys.items():
+        lines.append(f'2 {tag} "{nm}"')
+    lines.append("$EndPhysicalNames")
+    lines += ["$Nodes", str(mesh.n_vert)]
+    lines += [f"{i + 1} {x:.17g} {y:.17g} {z:.17g}" for i, (x, y, z) in enumerate(mesh.coord)]
+    lines.append("$EndNodes")
+    items = []
+    off = 0
+    for f in range(nb):
+        nv = int(mesh.bf_nvert[f])
+        verts = mesh.bf_vert[off:off + nv]
+        off += nv
+        t = int(mesh.bf_tag[f])
+        ptag = phys[t][0] if t in phys else 99
+        items.append((2 if nv == 3 else 3, ptag, verts))
+    nvk = {4: 4, 5: 8, 6: 6, 7: 5}
+    off = 0
+    for e in range(mesh.n_elems):
+        k = int(mesh.kind[e])
+        verts = mesh.elem_vert[off:off + nvk[k]]
+        off += nvk[k]
+        items.append((k, 1, verts))
+    lines += ["$Elements", str(len(items))]
+    for i, (k, ptag, verts) in enumerate(items):
+        lines.append(f"{i + 1} {k} 2 {ptag} {ptag} " + " ".join(str(int(v)) for v in verts))
+    lines.append("$EndElements")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
