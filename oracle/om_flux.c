@@ -893,7 +893,7 @@ int om_add_timedisc_and_solve(const om_mesh *m, const om_config *cfg, double *so
   return OM_OK;
 }
 
-/* compute_rhs_ns without diffusion, ns_main.F90:130-179 */
+/* compute_rhs_ns, ns_main.F90:130-179 (diffusion: om_diffusion.c) */
 int om_compute_rhs_ns(const om_mesh *m, const om_config *cfg, const double *sol, double *rhs, double *diag,
                       double *sum_lambda, double *cell_grad, double *nodal_grad, double *vp) {
   long ne = m->ne;
@@ -911,6 +911,7 @@ int om_compute_rhs_ns(const om_mesh *m, const om_config *cfg, const double *sol,
     free(rhs_o2);
     free(sl_o2);
   }
+  if (!st) st = om_add_diffusion(m, cfg, sol, rhs, diag);
   return st;
 }
 

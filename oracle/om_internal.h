@@ -65,6 +65,14 @@ typedef struct {
   double cfl;
   double tmax;
   double t;                          /* physical time seen by time-dependent BCs */
+  /* viscous terms (compute_rhs_ns with activate_diffusion, ns_main.F90:170-175);
+   * constant viscosity (use_sutherland = .false.) */
+  int activate_diffusion;
+  double mu_p, Cv_p, prandtl;
+  int bc_V_id[OM_MNBC + 1];          /* 1 neumann, 2 dirichlet (ns_global_data_module.F90:40-41) */
+  int bc_T_id[OM_MNBC + 1];
+  double bc_val_V[OM_MNBC + 1][3];
+  double bc_val_T[OM_MNBC + 1];
 } om_config;
 
 typedef struct {
@@ -107,5 +115,8 @@ typedef struct {
 } om_mesh;
 
 void *om_arena_alloc(om_mesh *m, size_t bytes);
+
+/* viscous block of compute_rhs_ns (om_diffusion.c) */
+int om_add_diffusion(const om_mesh *m, const om_config *cfg, const double *sol, double *rhs, double *diag);
 
 #endif

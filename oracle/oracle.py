@@ -54,13 +54,23 @@ class Config(C.Structure):
         ("cfl", C.c_double),
         ("tmax", C.c_double),
         ("t", C.c_double),
+        ("activate_diffusion", C.c_int),
+        ("mu_p", C.c_double),
+        ("Cv_p", C.c_double),
+        ("prandtl", C.c_double),
+        ("bc_V_id", C.c_int * (MNBC + 1)),
+        ("bc_T_id", C.c_int * (MNBC + 1)),
+        ("bc_val_V", (C.c_double * 3) * (MNBC + 1)),
+        ("bc_val_T", C.c_double * (MNBC + 1)),
     ]
 
 
 def make_config(scheme="multi_point", bc_style=2, boundary_2d=True, exclude_bound_vert=False,
                 local_time_step=False, second_order=False, method=4, bcs=(), cfl=0.4, tmax=1e30,
-                t=0.0) -> Config:
-    """bcs: sequence of (bc_type, (rho,u,v,w,p)) in BC index order 1..n_bc."""
+                t=0.0, diffusion=False, mu=1e-3, cv=720.19471, prandtl=0.71, bcs_v=(), bcs_t=()) -> Config:
+    """bcs: sequence of (bc_type, (rho,u,v,w,p)) in BC index order 1..n_bc;
+    bcs_v / bcs_t: ("neumann"|"dirichlet", value) per BC index for the
+    viscous terms (velocity (3,) / temperature), default neumann."""
     c = Config()
     c.scheme_id = SCHEMES[scheme]
     c.bc_style = bc_style
@@ -77,6 +87,20 @@ def make_config(scheme="multi_point", bc_style=2, boundary_2d=True, exclude_boun
     c.cfl = cfl
     c.tmax = tmax
     c.t = t
+    c.activate_diffusion = int(diffusion)
+    c.mu_p = mu
+    c.Cv_p = cv
+    c.prandtl = prandtl
+    for i in range(1, MNBC + 1):
+        c.bc_V_id[i] = 1
+        c.bc_T_id[i] = 1
+    for i, (typ, val) in enumerate(bcs_v, start=1):
+        c.bc_V_id[i] = 2 if typ == "dirichlet" else 1
+        for k in range(3):
+            c.bc_val_V[i][k] = float(val[k])
+    for i, (typ, val) in enumerate(bcs_t, start=1):
+        c.bc_T_id[i] = 2 if typ == "dirichlet" else 1
+        c.bc_val_T[i] = float(val)
     return c
 
 
@@ -115,6 +139,8 @@ def lib():
         L.om_bcsr_matmul.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp]
         L.om_bcsr_diag.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp]
         L.om_bcsr_gmres_diag.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.om_add_diffusion.argtypes = [P, cp, dp, dp, dp]
+        L.om_test_inv.argtypes = [C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -210,6 +236,19 @@ class OracleMesh:
             raise RuntimeError(f"oracle compute_euler_flux status {st}")
         return (rhs, sl, vp) if want_vp else (rhs, sl)
 
+    def add_diffusion(self, cfg: Config, sol: np.ndarray, rhs=None, diag=None):
+        """The viscous block of compute_rhs_ns (momentum, then heat) added to
+        rhs (n_elems, 5) and diag (n_elems, 5, 5 column-major blocks, i.e.
+        diag[c].T is the 5x5 matrix); zeros if not given."""
+        ne = self.n_elems
+        sol = np.ascontiguousarray(sol, np.float64).reshape(ne, 5)
+        rhs = np.zeros((ne, 5)) if rhs is None else np.ascontiguousarray(rhs, np.float64).copy()
+        diag = np.zeros((ne, 5, 5)) if diag is None else np.ascontiguousarray(diag, np.float64).copy()
+        st = lib().om_add_diffusion(self.h, C.byref(cfg), sol.ravel(), rhs.reshape(-1), diag.reshape(-1))
+        if st:
+            raise RuntimeError(f"oracle add_diffusion status {st}")
+        return rhs, diag
+
     def compute_cell_grad(self, sol: np.ndarray, method: int = 4):
         ne, nv = self.n_elems, self.n_vert
         grad = np.zeros((ne, 5, 3))
@@ -278,3 +317,12 @@ def bcsr_gmres_diag(m, b, row, col, v, rhs, tol, restart=30):
                                     np.ascontiguousarray(v, np.float64).ravel(), x,
                                     np.ascontiguousarray(rhs, np.float64), tol, restart, info)
     return x, bool(conv), dict(restarts=int(info[0]), rho=info[1])
+
+
+def inv(A: np.ndarray) -> np.ndarray:
+    """inv_lapack restatement (LU + unblocked dgetri) of a square matrix."""
+    A = np.asarray(A, np.float64)
+    n = A.shape[0]
+    out = np.zeros(n * n)
+    lib().om_test_inv(n, np.ascontiguousarray(A.T).ravel(), out)
+    return out.reshape(n, n).T
