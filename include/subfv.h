@@ -22,7 +22,9 @@
  *   subfv_compute_cell_grad     compute_cell_grad_from_nodal_grad (src/ns/ns_euler_recon_module.F90:8-59)
  *   subfv_flux_limiting         flux_limiting (src/ns/ns_euler_recon_module.F90:560-601)
  *   subfv_add_timedisc_and_solve add_timedisc_and_solve (src/ns/ns_euler_module.F90:521-609)
- *   subfv_compute_rhs_ns        compute_rhs_ns (src/ns/ns_main.F90:130-179, without diffusion)
+ *   subfv_compute_rhs_ns        compute_rhs_ns (src/ns/ns_main.F90:130-179)
+ *   subfv_set_diffusion         activate_diffusion and the viscous namelist entries
+ *                               (src/ns/ns_global_data_module.F90:8-12,177-213)
  *   subfv_run                   the time loop of src/ns/ns_main.F90:83-107
  *   subfv_bcsr_*                bcsr_matrix_type, bcsr_matmul, bcsr_diag, bcsr_gmres_diag
  *                               (src/linear_algebra/sparse_bcsr_linear_module.F90:6-29,1414-1436,1592-1691)
@@ -127,7 +129,9 @@ int subfv_state_put(subfv_ctx *ctx, const double *sol);
 int subfv_state_get(subfv_ctx *ctx, double *sol);
 
 /* Device arrays in reference layout, by name: "sol", "rhs", "rhs_o2",
- * "sum_lambda", "sum_lambda_o2", "cell_grad", "nodal_grad", "vp", "diag". */
+ * "sum_lambda", "sum_lambda_o2", "cell_grad", "nodal_grad", "vp", and with
+ * the viscous terms "diag" (5, 5, n_elems) and "rhs_diff" (4, n_elems: the
+ * viscous rhs of rho u, rho v, rho w, E). */
 int64_t subfv_get_f64(subfv_ctx *ctx, const char *name, double *out, int64_t cap);
 int64_t subfv_put_f64(subfv_ctx *ctx, const char *name, const double *in, int64_t n);
 
@@ -144,8 +148,22 @@ int subfv_flux_limiting(subfv_ctx *ctx, double cfl);
  * *omega the positivity factor. */
 int subfv_add_timedisc_and_solve(subfv_ctx *ctx, double a1, double a2, double cfl, double t, double *dt,
                                  double *omega);
-/* compute_rhs_ns without diffusion */
+/* compute_rhs_ns: Euler fluxes, second-order reconstruction and flux
+ * limiting, then the viscous block when subfv_set_diffusion activated it */
 int subfv_compute_rhs_ns(subfv_ctx *ctx);
+
+/* Viscous terms of compute_rhs_ns (activate_diffusion, ns_main.F90:170-175):
+ * build_momentum_diff_mat_into_global_mat_diag (ns_vectorial_diffusion_module.F90:200-320)
+ * and build_diff_mat_into_global_mat_diag (ns_diffusion_module.F90:112-184) with a
+ * constant viscosity mu_p (compute_mu without Sutherland, :322-347), Cv_p and the
+ * Prandtl number (a compile-time 0.71 in the reference).  Per BC index
+ * i = 1..n_bc (arrays 0-based): bc_v_id / bc_t_id 1 = neumann, 2 = dirichlet
+ * (bc_type_V / bc_type_T), bc_val_v(3, n_bc), bc_val_t(n_bc).  Applies to
+ * subfv_compute_rhs_ns (rhs and "diag") and to the stages of subfv_run.
+ * Vertices may have at most 16 sub-faces and 16 sub-elements. */
+int subfv_set_diffusion(subfv_ctx *ctx, int32_t activate, double mu_p, double cv_p, double prandtl,
+                        int32_t n_bc, const int32_t *bc_v_id, const double *bc_val_v, const int32_t *bc_t_id,
+                        const double *bc_val_t);
 
 /* The time loop on device-resident state without host synchronisation.
  * time_scheme 0: the reference loop (forward Euler, or its two-stage scheme

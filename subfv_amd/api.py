@@ -144,7 +144,7 @@ class Solver:
         return s
 
     _SHAPES = {"sol": (5,), "rhs": (5,), "rhs_o2": (5,), "sum_lambda": (), "sum_lambda_o2": (),
-               "cell_grad": (5, 3), "nodal_grad": (5, 3), "vp": (3,)}
+               "cell_grad": (5, 3), "nodal_grad": (5, 3), "vp": (3,), "diag": (5, 5), "rhs_diff": (4,)}
     _PER_VERT = {"nodal_grad", "vp"}
 
     def get(self, name: str) -> np.ndarray:
@@ -192,6 +192,25 @@ class Solver:
         om = C.c_double(0.0)
         self._check(_lib.lib().subfv_add_timedisc_and_solve(self.h, a1, a2, cfl, t, C.byref(dt), C.byref(om)))
         return dt.value, om.value
+
+    def set_diffusion(self, activate: bool = True, mu: float = 1e-3, cv: float = 720.19471,
+                      prandtl: float = 0.71, bcs_v=(), bcs_t=()):
+        """activate_diffusion with constant viscosity mu (mu_p), Cv_p and the
+        Prandtl number; bcs_v / bcs_t: ("neumann"|"dirichlet", value) per BC
+        index 1..n (velocity (3,) / temperature)."""
+        n = max(len(bcs_v), len(bcs_t))
+        vid = np.ones(max(n, 1), np.int32)
+        tid = np.ones(max(n, 1), np.int32)
+        vv = np.zeros(3 * max(n, 1))
+        tv = np.zeros(max(n, 1))
+        for i, (typ, val) in enumerate(bcs_v):
+            vid[i] = 2 if typ == "dirichlet" else 1
+            vv[3 * i:3 * i + 3] = val
+        for i, (typ, val) in enumerate(bcs_t):
+            tid[i] = 2 if typ == "dirichlet" else 1
+            tv[i] = val
+        self._check(_lib.lib().subfv_set_diffusion(self.h, int(activate), mu, cv, prandtl, n, _lib.iptr(vid),
+                                                   _lib.dptr(vv), _lib.iptr(tid), _lib.dptr(tv)))
 
     def compute_rhs_ns(self):
         self._check(_lib.lib().subfv_compute_rhs_ns(self.h))

@@ -12,6 +12,7 @@
 #include "device_mesh.hpp"
 #include "kernels_bcsr.hpp"
 #include "kernels_cell.hpp"
+#include "kernels_diff.hpp"
 #include "kernels_flux.hpp"
 #include "kernels_geom.hpp"
 #include "mesh_build.hpp"
@@ -40,6 +41,11 @@ struct subfv_ctx {
   double *rhs = nullptr, *rhs2 = nullptr, *sl = nullptr, *sl2 = nullptr;
   double *NG = nullptr, *VMM = nullptr, *G = nullptr, *vp = nullptr, *diag = nullptr;
   double *blk = nullptr;
+  // viscous terms (constant viscosity): per-vertex precomputed blocks
+  bool diff_on = false, diff_ready = false;
+  DiffParams dparams{};
+  long *doff = nullptr;
+  double *pre = nullptr, *PD = nullptr, *rdiff = nullptr;
   int *derr = nullptr;
   StepState *st = nullptr;
   double host_t = 0.0;
@@ -89,6 +95,9 @@ void free_mesh(subfv_ctx *c) {
   for (auto &u : c->U) u = nullptr;
   c->P = c->P2 = c->rhs = c->rhs2 = c->sl = c->sl2 = nullptr;
   c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = nullptr;
+  c->doff = nullptr;
+  c->pre = c->PD = c->rdiff = nullptr;
+  c->diff_ready = false;
 }
 
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -365,7 +374,7 @@ int launch_grad(subfv_ctx *c, const double *U, int method) {
 }
 
 int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout, double a1, double a2, double w0,
-                  double w1, int flux_limit, int advance_t) {
+                  double w1, int flux_limit, int advance_t, bool with_diff = false) {
   UpdateArgs u;
   u.m = c->dm;
   u.U = U;
@@ -374,7 +383,8 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
   u.rhs = c->rhs;
   u.rhs2 = c->rhs2;
   u.sl = c->sl;
-  u.diag = c->diag;
+  u.diag = c->diff_on ? c->diag : nullptr;
+  u.rdiff = (c->diff_on && with_diff) ? c->rdiff : nullptr;
   u.blk = c->blk;
   u.st = c->st;
   u.a1 = a1;
@@ -385,7 +395,7 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
   u.advance_t = advance_t;
   dim3 g(nblk(c->dm.C, kCellThreads)), b(kCellThreads);
   Prof pf(c, "update");
-  if (c->diag) {
+  if (u.diag) {
     hipLaunchKernelGGL(k_update<true>, g, b, 0, c->stream, u);
     hipLaunchKernelGGL(k_finalize_omega, dim3(1), b, 0, c->stream, (const double *)c->blk, (int)g.x, c->st, advance_t);
     hipLaunchKernelGGL(k_update_fixup<true>, g, b, 0, c->stream, u);
@@ -396,6 +406,76 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
   }
   HIPCHK(c, hipGetLastError());
   return 0;
+}
+
+// ------------------------------------------------------------- diffusion
+// Once per mesh and parameter set: offsets of the packed per-vertex blocks,
+// buffers, and the setup kernel (nodal systems, inverses, products).
+int ensure_diffusion(subfv_ctx *c) {
+  if (c->diff_ready) return 0;
+  const subfv::HostMesh &h = c->hm;
+  std::vector<long> off(h.nv + 1, 0);
+  for (int v = 0; v < h.nv; v++) {
+    const int nf = h.sf_off[v + 1] - h.sf_off[v], ns = h.v_seoff[v + 1] - h.v_seoff[v];
+    if (nf > kDiffMaxSub || ns > kDiffMaxSub)
+      return fail(c, SUBFV_E_INVALID, "viscous terms: vertex " + std::to_string(v + 1) + " has more than " +
+                                          std::to_string(kDiffMaxSub) + " sub-faces or sub-elements");
+    off[v + 1] = off[v] + diff_block_size(nf, ns);
+  }
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+  const size_t need = sizeof(double) * ((size_t)off[h.nv] + (size_t)kDiffSlot * h.nse + 29ull * h.ne);
+  if (need + (size_t)1 << 30 > free_b)
+    return fail(c, SUBFV_E_DEVICE, "viscous terms: " + std::to_string(need >> 20) + " MiB needed, " +
+                                       std::to_string(free_b >> 20) + " MiB free");
+  int r;
+  if ((r = upload(c, &c->doff, off))) return r;
+  if ((r = dalloc(c, &c->pre, (size_t)off[h.nv]))) return r;
+  if ((r = dalloc(c, &c->PD, (size_t)kDiffSlot * h.nse))) return r;
+  if ((r = dalloc(c, &c->rdiff, 4ull * h.ne))) return r;
+  if (!c->diag && (r = dalloc(c, &c->diag, 25ull * h.ne))) return r;
+  HIPCHK(c, hipMemsetAsync(c->diag, 0, sizeof(double) * 25 * h.ne, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->rdiff, 0, sizeof(double) * 4 * h.ne, c->stream));
+  DiffArgs a;
+  a.m = c->dm;
+  a.p = c->dparams;
+  a.doff = c->doff;
+  a.pre = c->pre;
+  a.U = nullptr;
+  a.PD = c->PD;
+  a.rdiff = c->rdiff;
+  a.diag = c->diag;
+  hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)h.nv), dim3(64), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  c->diff_ready = true;
+  return 0;
+}
+
+// rdiff and diag of the current state (build_momentum_diff_mat_into_global_mat_diag
+// and build_diff_mat_into_global_mat_diag)
+int launch_diffusion(subfv_ctx *c, const double *U) {
+  int r = ensure_diffusion(c);
+  if (r) return r;
+  DiffArgs a;
+  a.m = c->dm;
+  a.p = c->dparams;
+  a.doff = c->doff;
+  a.pre = c->pre;
+  a.U = U;
+  a.PD = c->PD;
+  a.rdiff = c->rdiff;
+  a.diag = c->diag;
+  Prof pf(c, "diffusion");
+  hipLaunchKernelGGL(k_diff_apply, dim3(nblk(c->dm.V, kDiffWaves)), dim3(64 * kDiffWaves), 0, c->stream, a);
+  hipLaunchKernelGGL(k_diff_gather, dim3(nblk(c->dm.C, 256)), dim3(256), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+__global__ void k_add_rdiff(long C, double *rhs, const double *rdiff) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  for (int k = 1; k < 5; k++) rhs[k * C + c] = rhs[k * C + c] + rdiff[4 * c + k - 1];
 }
 
 int sync_state_scalars(subfv_ctx *c, StepState *hs) {
@@ -702,9 +782,12 @@ int64_t subfv_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) 
   Item items[] = {{"sol", c->U[c->cur], 1, 5 * C}, {"rhs", c->rhs, 5, C},      {"rhs_o2", c->rhs2, 5, C},
                   {"sum_lambda", c->sl, 1, C},   {"sum_lambda_o2", c->sl2, 1, C},
                   // gradients are stored in the reference layout already
-                  {"cell_grad", c->G, 1, 15 * C}, {"nodal_grad", c->NG, 1, 15 * V}, {"vp", c->vp, 3, V}};
+                  {"cell_grad", c->G, 1, 15 * C}, {"nodal_grad", c->NG, 1, 15 * V}, {"vp", c->vp, 3, V},
+                  // viscous block: diag(5,5,C) and the viscous rhs of components 2..5
+                  {"diag", c->diag, 1, 25 * C}, {"rhs_diff", c->rdiff, 1, 4 * C}};
   for (auto &it : items)
     if (n == it.nm) {
+      if (!it.p) return -1;
       if ((int64_t)it.nc * it.n > cap) return -1;
       return dev_to_host_soa(c, it.p, it.nc, it.n, out) ? -1 : (int64_t)it.nc * it.n;
     }
@@ -808,6 +891,14 @@ int subfv_compute_rhs_ns(subfv_ctx *c) {
     r = subfv_flux_limiting(c, c->cfg.cfl);
     if (r) return r;
   }
+  if (c->diff_on) {
+    // the diffusion block (ns_main.F90:170-175): rhs += viscous terms, diag = their block
+    r = launch_diffusion(c, c->U[c->cur]);
+    if (r) return r;
+    hipLaunchKernelGGL(k_add_rdiff, dim3(nblk(c->dm.C, 256)), dim3(256), 0, c->stream, (long)c->dm.C, c->rhs,
+                       (const double *)c->rdiff);
+    HIPCHK(c, hipGetLastError());
+  }
   return 0;
 }
 
@@ -815,6 +906,7 @@ int subfv_compute_rhs_ns(subfv_ctx *c) {
 // order), with dt from the first-order sum_lambda.
 static int residual(subfv_ctx *c, const double *U, int fixed_dt) {
   int r;
+  if (c->diff_on && (r = launch_diffusion(c, U))) return r;
   if (c->cfg.second_order) {
     if ((r = launch_grad(c, U, c->cfg.method))) return r;
     if ((r = launch_flux(c, U, nullptr, c->P, false))) return r;
@@ -854,6 +946,35 @@ int subfv_profile_read(subfv_ctx *c, const char *phase, double *ms_total, int64_
   return 0;
 }
 
+int subfv_set_diffusion(subfv_ctx *c, int32_t activate, double mu_p, double cv_p, double prandtl, int32_t n_bc,
+                        const int32_t *bc_v_id, const double *bc_val_v, const int32_t *bc_t_id,
+                        const double *bc_val_t) {
+  if (!c) return SUBFV_E_INVALID;
+  if (n_bc < 0 || n_bc > SUBFV_MNBC) return fail(c, SUBFV_E_INVALID, "n_bc out of range");
+  if (activate && !(mu_p > 0.0 && cv_p > 0.0 && prandtl > 0.0))
+    return fail(c, SUBFV_E_INVALID, "viscous terms need mu_p, Cv_p and Prandtl > 0");
+  DiffParams p{};
+  p.mu = mu_p;
+  p.mu0 = mu_p;  // compute_mu without Sutherland: mu0corr = mu_p
+  p.cv = cv_p;
+  p.kap = mu_p * cv_p * kGamma / prandtl;  // kappa (ns_diffusion_module.F90:186-194)
+  for (int i = 0; i < 11; i++) {
+    p.bc_V_id[i] = 1;
+    p.bc_T_id[i] = 1;
+  }
+  for (int i = 0; i < n_bc; i++) {  // BC index i + 1, as the right-neighbour codes count them
+    p.bc_V_id[i + 1] = bc_v_id ? bc_v_id[i] : 1;
+    p.bc_T_id[i + 1] = bc_t_id ? bc_t_id[i] : 1;
+    for (int d = 0; d < 3; d++) p.bc_val_V[i + 1][d] = bc_val_v ? bc_val_v[3 * i + d] : 0.0;
+    p.bc_val_T[i + 1] = bc_val_t ? bc_val_t[i] : 0.0;
+  }
+  const bool changed = std::memcmp(&p, &c->dparams, sizeof(p)) != 0;
+  c->dparams = p;
+  c->diff_on = activate != 0;
+  if (changed) c->diff_ready = false;
+  return 0;
+}
+
 int subfv_run(subfv_ctx *c, int32_t n_steps, int32_t time_scheme, double *t, double *dt) {
   if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
   if (time_scheme == 1 && c->cfg.local_time_step)
@@ -866,21 +987,21 @@ int subfv_run(subfv_ctx *c, int32_t n_steps, int32_t time_scheme, double *t, dou
       if ((r = residual(c, c->U[a], 0))) return r;
       if (c->cfg.second_order) {
         // ns_main.F90:88-95: half step into U*, residual at U*, full step from U^n
-        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 2.0, 1.0, 0.0, 1.0, fl, 0))) return r;
+        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 2.0, 1.0, 0.0, 1.0, fl, 0, true))) return r;
         if ((r = residual(c, c->U[b], 0))) return r;
-        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, fl, 1))) return r;
+        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, fl, 1, true))) return r;
       } else {
-        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, 0, 1))) return r;
+        if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, 0, 1, true))) return r;
       }
       c->cur = b;
     } else {
       // SSP-RK3: U1 = L(Un) -> b; U2 = 3/4 Un + 1/4 L(U1) -> q; Un+1 = 1/3 Un + 2/3 L(U2) -> b
       if ((r = residual(c, c->U[a], 0))) return r;
-      if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, fl, 0))) return r;
+      if ((r = launch_update(c, c->U[a], nullptr, c->U[b], 1.0, 1.0, 0.0, 1.0, fl, 0, true))) return r;
       if ((r = residual(c, c->U[b], 1))) return r;
-      if ((r = launch_update(c, c->U[b], c->U[a], c->U[q], 1.0, 1.0, 0.75, 0.25, fl, 0))) return r;
+      if ((r = launch_update(c, c->U[b], c->U[a], c->U[q], 1.0, 1.0, 0.75, 0.25, fl, 0, true))) return r;
       if ((r = residual(c, c->U[q], 1))) return r;
-      if ((r = launch_update(c, c->U[q], c->U[a], c->U[b], 1.0, 1.0, 1.0 / 3.0, 2.0 / 3.0, fl, 2))) return r;
+      if ((r = launch_update(c, c->U[q], c->U[a], c->U[b], 1.0, 1.0, 1.0 / 3.0, 2.0 / 3.0, fl, 2, true))) return r;
       c->cur = b;
     }
   }

@@ -148,6 +148,7 @@ struct UpdateArgs {
   double *Uout;       // [C][5]
   const double *rhs, *rhs2, *sl;
   const double *diag; // [C][25] column-major 5x5 blocks, or null (c*I diagonal)
+  const double *rdiff;// [C][4] viscous rhs of components 2..5 (added after flux limiting), or null
   double *blk;
   StepState *st;
   double a1, a2, w0, w1;
@@ -180,6 +181,8 @@ SFV_DEV double stage_cell(const UpdateArgs &a, long c, double *u, double *delta)
     double om = max_omega_pos(x, y);
     for (int k = 0; k < 5; k++) r[k] = r[k] + dmin(1.0, om) * (r2[k] - r[k]);
   }
+  if (DIAG && a.rdiff)  // the diffusion block follows flux limiting (ns_main.F90:170-175)
+    for (int k = 1; k < 5; k++) r[k] = r[k] + a.rdiff[4 * c + k - 1];
   double cc = a.a1 * (vol / st->dt + (st->lts ? lam : 0.0));
   double nw[5];
   if (DIAG) {

@@ -1,0 +1,503 @@
+// Viscous terms of compute_rhs_ns (src/ns/ns_main.F90:170-175) with a
+// constant viscosity: the nodal momentum system of
+// ns_vectorial_diffusion_module.F90:7-320 and the nodal heat system of
+// ns_diffusion_module.F90:7-206.
+//
+// With mu constant in space and time (use_sutherland = .false.) the nodal
+// matrices N, S, S~^T and B depend on the mesh only, so the setup kernel
+// forms, once per mesh, every product the residual needs (one wavefront per
+// vertex, matrices in LDS, the inverse in the order of LU + unblocked
+// dgetri, products summed over k in increasing order):
+//   momentum: SNS = (S~^T N^-1) S, SB = S~^T (N^-1 B), the sub-element sums of
+//             S~^T, and N^-1 S, N^-1 B (the face velocities Vf = N^-1 (S Ve + B)
+//             are formed as N^-1 S Ve + N^-1 B);
+//   heat:     SNS_h, SB_h and the row sums of S~^T_h.
+// Each step, one wavefront per vertex evaluates the reference's scatter
+// terms from the cell states and writes one row of 18 partial sums per
+// node-sorted sub-element slot; k_diff_gather adds a cell's rows in
+// increasing vertex id into rdiff[C][4] (rhs of rho u, rho v, rho w, E) and
+// diag[C][25] (the point-implicit block).
+#pragma once
+#include "device_mesh.hpp"
+#include "physics.hpp"
+
+namespace sfv {
+
+constexpr int kDiffMaxSub = 16;             // sub-faces (and sub-elements) per vertex
+constexpr int kDiffMaxF = 3 * kDiffMaxSub;  // 48
+constexpr int kDiffSlot = 18;               // partials per slot
+
+struct DiffParams {
+  double mu, mu0, kap, cv;
+  int bc_V_id[11], bc_T_id[11];  // 1 neumann, 2 dirichlet, by BC index (1-based)
+  double bc_val_V[11][3], bc_val_T[11];
+};
+
+// packed per-vertex block: sizes for nf sub-faces and ns sub-elements
+SFV_DEV __host__ inline long diff_block_size(int nf, int ns) {
+  const long F = 3 * nf, E = 3 * ns;
+  return E * E + E + 9L * ns + F * E + F + (long)ns * ns + 2L * ns;
+}
+
+struct DiffSetupLds {
+  double N[kDiffMaxF * kDiffMaxF];
+  double S[kDiffMaxF * kDiffMaxF];
+  double T[kDiffMaxF * kDiffMaxF];  // S~^T
+  double W[kDiffMaxF * kDiffMaxF];  // products
+  double B[kDiffMaxF], x[kDiffMaxF], work[kDiffMaxF];
+  int piv[kDiffMaxF];
+  int lse[kDiffMaxSub], rse[kDiffMaxSub];  // local slot of each sub-face's left/right cell (-1 none)
+};
+
+struct DiffArgs {
+  DevMesh m;
+  DiffParams p;
+  const long *doff;   // [V+1] offsets of the packed vertex blocks
+  double *pre;        // packed blocks
+  const double *U;    // [C][5]
+  double *PD;         // [SE][18] node-sorted slot partials
+  double *rdiff;      // [C][4]
+  double *diag;       // [C][25]
+};
+
+SFV_DEV void wsync_d() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A := A^-1 (n x n column-major in LDS, ld n): LU with partial pivoting in
+// lu_solve's order, then inv(U) (dtrti2) and inv(A) L = inv(U) (dgetri).
+SFV_DEV void wave_inverse(double *A, int n, int *piv, double *work, int lane) {
+#define AA(i, j) A[(i) + n * (j)]
+  for (int k = 0; k < n; k++) {
+    if (lane == 0) {
+      int p = k;
+      double amax = fabs(AA(k, k));
+      for (int i = k + 1; i < n; i++)
+        if (fabs(AA(i, k)) > amax) {
+          amax = fabs(AA(i, k));
+          p = i;
+        }
+      piv[k] = p;
+    }
+    wsync_d();
+    const int p = piv[k];
+    if (AA(p, k) != 0.0) {
+      if (p != k)
+        for (int j = lane; j < n; j += 64) {
+          double t = AA(k, j);
+          AA(k, j) = AA(p, j);
+          AA(p, j) = t;
+        }
+      wsync_d();
+      const double pv = AA(k, k);
+      if (fabs(pv) >= 2.2250738585072014e-308) {
+        const double r = 1.0 / pv;
+        for (int i = k + 1 + lane; i < n; i += 64) AA(i, k) = AA(i, k) * r;
+      } else {
+        for (int i = k + 1 + lane; i < n; i += 64) AA(i, k) = AA(i, k) / pv;
+      }
+    }
+    wsync_d();
+    const int m = n - k - 1;
+    for (int q = lane; q < m * m; q += 64) {
+      const int i = k + 1 + q % m, j = k + 1 + q / m;
+      AA(i, j) = AA(i, j) - AA(i, k) * AA(k, j);
+    }
+    wsync_d();
+  }
+  // dtrti2 (upper, non-unit)
+  for (int j = 0; j < n; j++) {
+    if (lane == 0) AA(j, j) = 1.0 / AA(j, j);
+    wsync_d();
+    const double ajj = -AA(j, j);
+    for (int jj = 0; jj < j; jj++) {
+      const double temp = AA(jj, j);
+      if (temp != 0.0) {
+        for (int i = lane; i < jj; i += 64) AA(i, j) = AA(i, j) + temp * AA(i, jj);
+        wsync_d();
+        if (lane == 0) AA(jj, j) = AA(jj, j) * AA(jj, jj);
+      }
+      wsync_d();
+    }
+    for (int i = lane; i < j; i += 64) AA(i, j) = ajj * AA(i, j);
+    wsync_d();
+  }
+  // inv(A) L = inv(U), columns right to left
+  for (int j = n - 1; j >= 0; j--) {
+    for (int i = j + 1 + lane; i < n; i += 64) {
+      work[i] = AA(i, j);
+      AA(i, j) = 0.0;
+    }
+    wsync_d();
+    for (int jj = j + 1; jj < n; jj++) {
+      const double temp = -1.0 * work[jj];
+      for (int i = lane; i < n; i += 64) AA(i, j) = AA(i, j) + temp * AA(i, jj);
+    }
+    wsync_d();
+  }
+  for (int j = n - 2; j >= 0; j--) {
+    const int jp = piv[j];
+    if (jp != j)
+      for (int i = lane; i < n; i += 64) {
+        double t = AA(i, j);
+        AA(i, j) = AA(i, jp);
+        AA(i, jp) = t;
+      }
+    wsync_d();
+  }
+#undef AA
+}
+
+// C(m,n) = A(m,k) B(k,n), each entry summed over k in order from zero
+SFV_DEV void wave_gemm(int m, int n, int k, const double *A, const double *B, double *C, int lane) {
+  for (int q = lane; q < m * n; q += 64) {
+    const int i = q % m, j = q / m;
+    double acc = 0.0;
+    for (int l = 0; l < k; l++) acc = acc + B[l + k * j] * A[i + m * l];
+    C[i + m * j] = acc;
+  }
+}
+
+// y(m) = A(m,n) x(n)
+SFV_DEV void wave_gemv(int m, int n, const double *A, const double *x, double *y, int lane) {
+  for (int i = lane; i < m; i += 64) {
+    double acc = 0.0;
+    for (int j = 0; j < n; j++) acc = acc + x[j] * A[i + m * j];
+    y[i] = acc;
+  }
+}
+
+SFV_DEV void sf_normal(const DevMesh &m, long s, double sign, double *n) {
+  n[0] = sign * m.sfn[s];
+  n[1] = sign * m.sfn[m.SF + s];
+  n[2] = sign * m.sfn[2 * m.SF + s];
+}
+
+// blc_3 of the constant-viscosity form (ns_vectorial_diffusion_module.F90:92-96),
+// entry (i, j): s (mu (nl.np) d_ij + (mu - mu0) np_i nl_j + (mu0 - 2/3 mu) np_j nl_i)
+SFV_DEV double blc_mu_ij(double s, double mu, double mu0, const double *nl, const double *np, int i, int j) {
+  const double d = nl[0] * np[0] + nl[1] * np[1] + nl[2] * np[2];
+  return s * (mu * d * (i == j ? 1.0 : 0.0) + (mu - mu0) * (np[i] * nl[j]) + (mu0 - (2.0 / 3.0) * mu) * (np[j] * nl[i]));
+}
+
+// One 64-thread workgroup per vertex.
+__global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
+  __shared__ DiffSetupLds L;
+  const DevMesh &m = a.m;
+  const DiffParams &P = a.p;
+  const long v = blockIdx.x;
+  if (v >= m.V) return;
+  const int lane = threadIdx.x;
+  const int sf0 = m.sf_off[v], nf = m.sf_off[v + 1] - sf0;
+  const int se0 = m.se_off[v], ns = m.se_off[v + 1] - se0;
+  const int F = 3 * nf, E = 3 * ns;
+  double *out = a.pre + a.doff[v];
+  // local slots of the left / right cells of each sub-face
+  if (lane < nf) {
+    const long s = sf0 + lane;
+    const int le = m.sf_le[s], re = m.sf_re[s];
+    int ls = -1, rs = -1;
+    for (int q = 0; q < ns; q++) {
+      const int c = m.se_cell[se0 + q];
+      if (c == le) ls = q;
+      if (re >= 0 && c == re) rs = q;
+    }
+    L.lse[lane] = ls;
+    L.rse[lane] = rs;
+  }
+  for (int q = lane; q < kDiffMaxF * kDiffMaxF; q += 64) L.N[q] = L.S[q] = L.T[q] = 0.0;
+  for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
+  wsync_d();
+
+  // ---------------- heat (scalar entries, one lane, reference loop order)
+  if (lane == 0) {
+    for (int l = 0; l < nf; l++) {
+      const long s = sf0 + l;
+      const double al = m.sfn[3 * m.SF + s];
+      for (int side = 0; side < 2; side++) {
+        const int so = side == 0 ? L.lse[l] : L.rse[l];
+        if (side == 1 && so < 0) {
+          const int re = m.sf_re[s];
+          if (re <= -2 && P.bc_T_id[-re - 1] == 2) {
+            L.B[l] = al * P.bc_val_T[-re - 1];
+            for (int c = 0; c < ns; c++) L.S[l + nf * c] = 0.0;
+            for (int c = 0; c < nf; c++) L.N[l + nf * c] = 0.0;
+            L.N[l + nf * l] = al;
+          }
+          continue;
+        }
+        double nl[3];
+        sf_normal(m, s, side == 0 ? 1.0 : -1.0, nl);
+        const long se = m.v_seid[se0 + so];
+        const int cell = m.se_cell[se0 + so];
+        const double vol = m.se_vol[se];
+        for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
+          const long sp = m.se_sf[pp];
+          const int po = (int)(sp - sf0);
+          double np[3];
+          sf_normal(m, sp, m.sf_le[sp] == cell ? 1.0 : -1.0, np);
+          const double blc = P.kap * al * m.sfn[3 * m.SF + sp] * (1.0 / vol) * (np[0] * nl[0] + np[1] * nl[1] + np[2] * nl[2]);
+          L.N[l + nf * po] = L.N[l + nf * po] + blc;
+          L.S[l + nf * so] = L.S[l + nf * so] + blc;
+          L.T[so + ns * l] = L.T[so + ns * l] + blc;
+        }
+      }
+    }
+  }
+  wsync_d();
+  wave_inverse(L.N, nf, L.piv, L.work, lane);
+  // SNS_h = (T N^-1) S ; SB_h = T (N^-1 B) ; b1 = row sums of T
+  double *o_snsh = out + (long)E * E + E + 9L * ns + (long)F * E + F;
+  double *o_sbh = o_snsh + (long)ns * ns;
+  double *o_b1 = o_sbh + ns;
+  wave_gemm(ns, nf, nf, L.T, L.N, L.W, lane);
+  wsync_d();
+  wave_gemm(ns, ns, nf, L.W, L.S, o_snsh, lane);
+  wave_gemv(nf, nf, L.N, L.B, L.x, lane);
+  wsync_d();
+  wave_gemv(ns, nf, L.T, L.x, o_sbh, lane);
+  for (int l = lane; l < ns; l += 64) {
+    double acc = 0.0;
+    for (int c = 0; c < nf; c++) acc = acc + L.T[l + ns * c];
+    o_b1[l] = acc;
+  }
+  wsync_d();
+
+  // ---------------- momentum (3x3 blocks, lane (i, j) < 9 owns entry (i, j))
+  for (int q = lane; q < kDiffMaxF * kDiffMaxF; q += 64) L.N[q] = L.S[q] = L.T[q] = 0.0;
+  for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
+  wsync_d();
+  if (lane < 9) {
+    const int i = lane % 3, j = lane / 3;
+    const double mu = P.mu, mu0 = P.mu0;
+    for (int l = 0; l < nf; l++) {
+      const long s = sf0 + l;
+      const int re = m.sf_re[s];
+      const double al = m.sfn[3 * m.SF + s];
+      bool neumann, dirichlet;
+      if (re >= 0) {
+        neumann = dirichlet = false;
+      } else if (re <= -2) {
+        dirichlet = P.bc_V_id[-re - 1] == 2;
+        neumann = P.bc_V_id[-re - 1] == 1;
+      } else {
+        neumann = true;
+        dirichlet = false;
+      }
+      const int fl0 = 3 * l;
+      const int nsides = (!neumann && !dirichlet) ? 2 : 1;
+      for (int side = 0; side < nsides; side++) {
+        const int so = side == 0 ? L.lse[l] : L.rse[l];
+        const int se0l = 3 * so;
+        double nl[3];
+        sf_normal(m, s, side == 0 ? 1.0 : -1.0, nl);
+        const long se = m.v_seid[se0 + so];
+        const int cell = m.se_cell[se0 + so];
+        const double vol = m.se_vol[se];
+        for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
+          const long sp = m.se_sf[pp];
+          const int fp0 = 3 * (int)(sp - sf0);
+          double np[3];
+          sf_normal(m, sp, m.sf_le[sp] == cell ? 1.0 : -1.0, np);
+          const double sc = al * m.sfn[3 * m.SF + sp] * (1.0 / vol);
+          const double bmu = blc_mu_ij(sc, mu, mu0, nl, np, i, j);
+          const double bmu_t = blc_mu_ij(sc, mu, mu0, nl, np, j, i);
+          if (!neumann && !dirichlet) {
+            L.T[(se0l + i) + E * (fl0 + j)] = L.T[(se0l + i) + E * (fl0 + j)] + bmu_t;
+            L.N[(fl0 + i) + F * (fp0 + j)] = L.N[(fl0 + i) + F * (fp0 + j)] + bmu;
+            L.S[(fl0 + i) + F * (se0l + j)] = L.S[(fl0 + i) + F * (se0l + j)] + bmu;
+          } else if (neumann) {
+            const double dd = nl[0] * np[0] + nl[1] * np[1] + nl[2] * np[2];
+            const double bn = sc * (dd * (i == j ? 1.0 : 0.0) + (np[j] * nl[i]) / 3.0);
+            L.N[(fl0 + i) + F * (fp0 + j)] = L.N[(fl0 + i) + F * (fp0 + j)] + bn;
+            L.S[(fl0 + i) + F * (se0l + j)] = L.S[(fl0 + i) + F * (se0l + j)] + bn;
+            L.T[(se0l + i) + E * (fl0 + j)] = L.T[(se0l + i) + E * (fl0 + j)] + bmu_t;
+          } else {
+            L.T[(se0l + i) + E * (fl0 + j)] = L.T[(se0l + i) + E * (fl0 + j)] + bmu_t;
+          }
+        }
+      }
+    }
+  }
+  wsync_d();
+  // Dirichlet rows (each sub-face's rows only receive its own terms above)
+  if (lane == 0) {
+    for (int l = 0; l < nf; l++) {
+      const long s = sf0 + l;
+      const int re = m.sf_re[s];
+      if (re <= -2 && P.bc_V_id[-re - 1] == 2) {
+        const double al = m.sfn[3 * m.SF + s];
+        for (int d = 0; d < 3; d++) {
+          const int r = 3 * l + d;
+          L.B[r] = al * P.bc_val_V[-re - 1][d];
+          for (int c = 0; c < E; c++) L.S[r + F * c] = 0.0;
+          for (int c = 0; c < F; c++) L.N[r + F * c] = 0.0;
+        }
+        for (int d = 0; d < 3; d++) L.N[(3 * l + d) + F * (3 * l + d)] = al * 1.0;
+      }
+    }
+  }
+  wsync_d();
+  double *o_sns = out;
+  double *o_sb = o_sns + (long)E * E;
+  double *o_blcs = o_sb + E;
+  double *o_ninvs = o_blcs + 9L * ns;
+  double *o_ninvb = o_ninvs + (long)F * E;
+  // sub-element sums of S~^T blocks (before anything overwrites T)
+  for (int q = lane; q < 9 * ns; q += 64) {
+    const int p = q / 9, ij = q % 9, i = ij % 3, j = ij / 3;
+    double acc = 0.0;
+    for (int l = 0; l < nf; l++) acc = acc + L.T[(3 * p + i) + E * (3 * l + j)];
+    o_blcs[q] = acc;
+  }
+  wave_inverse(L.N, F, L.piv, L.work, lane);
+  wave_gemm(E, F, F, L.T, L.N, L.W, lane);
+  wsync_d();
+  wave_gemm(E, E, F, L.W, L.S, o_sns, lane);
+  wave_gemv(F, F, L.N, L.B, L.x, lane);
+  wsync_d();
+  wave_gemv(E, F, L.T, L.x, o_sb, lane);
+  for (int q = lane; q < F; q += 64) o_ninvb[q] = L.x[q];
+  wave_gemm(F, E, F, L.N, L.S, o_ninvs, lane);
+}
+
+// temp_u and dtheta_dU (ns_euler_primitives_module.F90:52-63, ns_diffusion_module.F90:196-206)
+SFV_DEV double temp_u(const double *u, double cv) {
+  const double tau = 1.0 / u[0];
+  return (u[4] * tau - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * (tau * tau)) / cv;
+}
+SFV_DEV void dtheta_du(const double *u, double cv, double *P) {
+  const double v1 = u[1] / u[0], v2 = u[2] / u[0], v3 = u[3] / u[0];
+  const double nrm = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+  P[0] = (nrm * nrm - u[4] / u[0]) / (u[0] * cv);
+  for (int k = 1; k < 4; k++) P[k] = -u[k] / ((u[0] * u[0]) * cv);
+  P[4] = 1.0 / (u[0] * cv);
+}
+
+constexpr int kDiffWaves = 4;
+
+struct DiffApplyLds {
+  double Ve[kDiffMaxF], Vf[kDiffMaxF], SVe[kDiffMaxF];
+  double th[kDiffMaxSub], P[kDiffMaxSub][5], rho[kDiffMaxSub];
+};
+
+// One wavefront per vertex: the scatter terms of
+// build_momentum_diff_mat_into_global_mat_diag (:231-317) and
+// build_diff_mat_into_global_mat_diag (:139-182) per sub-element slot.
+// Row of PD: [0..2] rhs(2:4), [3] rhs(5), [4..12] diag(2:4, 2:4) column-major,
+// [13..17] diag(5, 1:5).
+__global__ __launch_bounds__(64 * kDiffWaves) void k_diff_apply(DiffArgs a) {
+  __shared__ DiffApplyLds lds_all[kDiffWaves];
+  const DevMesh &m = a.m;
+  const int wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long v = (long)blockIdx.x * kDiffWaves + wib;
+  if (v >= m.V) return;  // whole wavefront
+  DiffApplyLds &L = lds_all[wib];
+  const int sf0 = m.sf_off[v], nf = m.sf_off[v + 1] - sf0;
+  const int se0 = m.se_off[v], ns = m.se_off[v + 1] - se0;
+  const int F = 3 * nf, E = 3 * ns;
+  const double cv = a.p.cv;
+  const double *pre = a.pre + a.doff[v];
+  const double *sns = pre, *sb = sns + (long)E * E, *blcs = sb + E, *ninvs = blcs + 9L * ns;
+  const double *ninvb = ninvs + (long)F * E, *snsh = ninvb + F, *sbh = snsh + (long)ns * ns, *b1 = sbh + ns;
+  if (lane < ns) {
+    const int c = m.se_cell[se0 + lane];
+    double u[5];
+    for (int k = 0; k < 5; k++) u[k] = a.U[5L * c + k];
+    for (int k = 0; k < 3; k++) L.Ve[3 * lane + k] = u[k + 1] / u[0];
+    L.rho[lane] = u[0];
+    L.th[lane] = temp_u(u, cv);
+    dtheta_du(u, cv, L.P[lane]);
+  }
+  wsync_d();
+  if (lane < F) {
+    double acc = 0.0;
+    for (int j = 0; j < E; j++) acc = acc + L.Ve[j] * ninvs[lane + (long)F * j];
+    L.Vf[lane] = acc + ninvb[lane];
+  }
+  if (lane < E) {
+    double acc = 0.0;
+    for (int j = 0; j < E; j++) acc = acc + L.Ve[j] * sns[lane + (long)E * j];
+    L.SVe[lane] = acc;
+  }
+  wsync_d();
+  if (lane < ns) {
+    const int l = lane;
+    const int cell = m.se_cell[se0 + l];
+    const long se = m.v_seid[se0 + l];
+    const double rho = L.rho[l];
+    double out[kDiffSlot];
+    // momentum rhs and diagonal block
+    for (int i = 0; i < 3; i++) {
+      double mv = 0.0;
+      for (int j = 0; j < 3; j++) mv = mv + sns[(3 * l + i) + (long)E * (3 * l + j)] * L.Ve[3 * l + j];
+      out[i] = (L.SVe[3 * l + i] - mv) + sb[3 * l + i];
+    }
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++)
+        out[4 + i + 3 * j] = -sns[(3 * l + i) + (long)E * (3 * l + j)] / rho + blcs[9 * l + j + 3 * i] / rho;
+    // viscous work of the sub-element
+    double Spc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
+      const long s = m.se_sf[pp];
+      const int lo = (int)(s - sf0);
+      double nl[3], vm[3];
+      sf_normal(m, s, m.sf_le[s] == cell ? 1.0 : -1.0, nl);
+      for (int d = 0; d < 3; d++) vm[d] = L.Vf[3 * lo + d] - L.Ve[3 * l + d];
+      const double dv = vm[0] * nl[0] + vm[1] * nl[1] + vm[2] * nl[2];
+      const double ar = m.sfn[3 * m.SF + s];
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++)
+          Spc[i + 3 * j] = Spc[i + 3 * j] + ar * (vm[i] * nl[j] + vm[j] * nl[i] - (2.0 / 3.0) * dv * (i == j ? 1.0 : 0.0));
+    }
+    for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * a.p.mu / m.se_vol[se];
+    double work = 0.0;
+    for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
+      const long s = m.se_sf[pp];
+      const int lo = (int)(s - sf0);
+      double nl[3];
+      sf_normal(m, s, m.sf_le[s] == cell ? 1.0 : -1.0, nl);
+      double sv[3];
+      for (int i = 0; i < 3; i++)
+        sv[i] = Spc[i] * L.Vf[3 * lo] + Spc[i + 3] * L.Vf[3 * lo + 1] + Spc[i + 6] * L.Vf[3 * lo + 2];
+      work = work + m.sfn[3 * m.SF + s] * (sv[0] * nl[0] + sv[1] * nl[1] + sv[2] * nl[2]);
+    }
+    // heat
+    double h = 0.0;
+    for (int p = 0; p < ns; p++) h = h + snsh[l + (long)ns * p] * L.th[p];
+    const double sll = snsh[l + (long)ns * l];
+    double pu = 0.0;
+    for (int k = 0; k < 5; k++) pu = pu + L.P[l][k] * a.U[5L * cell + k];
+    h = h - sll * pu + (b1[l] - sll) * pu - b1[l] * L.th[l] + sbh[l];
+    out[3] = work + h;
+    for (int k = 0; k < 5; k++) out[13 + k] = (b1[l] - sll) * L.P[l][k];
+    double *o = a.PD + (long)kDiffSlot * (se0 + l);
+    for (int k = 0; k < kDiffSlot; k++) o[k] = out[k];
+  }
+}
+
+// rdiff and diag of each cell: its slots' rows in increasing vertex id
+__global__ __launch_bounds__(256) void k_diff_gather(DiffArgs a) {
+  const DevMesh &m = a.m;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
+  double r[4] = {0, 0, 0, 0}, d[14];
+  for (int k = 0; k < 14; k++) d[k] = 0.0;
+  for (int j = 0; j < nv; j++) {
+    const double *p = a.PD + (long)kDiffSlot * m.c_slot[b + j];
+    for (int k = 0; k < 4; k++) r[k] = r[k] + p[k];
+    for (int k = 0; k < 14; k++) d[k] = d[k] + p[4 + k];
+  }
+  for (int k = 0; k < 4; k++) a.rdiff[4 * c + k] = r[k];
+  double D[25];
+  for (int q = 0; q < 25; q++) D[q] = 0.0;
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) D[(1 + i) + 5 * (1 + j)] = d[i + 3 * j];
+  for (int k = 0; k < 5; k++) D[4 + 5 * k] = D[4 + 5 * k] + d[9 + k];
+  for (int q = 0; q < 25; q++) a.diag[25 * c + q] = D[q];
+}
+
+}  // namespace sfv

@@ -1,0 +1,106 @@
+"""Viscous terms on the device (kernels_diff.hpp) against the oracle
+(om_diffusion.c).  The device forms the nodal products once per mesh and
+adds each cell's terms as one sum per vertex, where the reference adds them
+one by one; results agree to round-off (relative 1e-11 of the terms' scale,
+stated per test), not bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from subfv_amd import meshgen as mg
+from subfv_amd.api import Solver, SubfvError, make_config
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = Solver(0)
+    yield s
+    s.close()
+
+
+def smooth_state(xc, seed=5):
+    rng = np.random.default_rng(seed)
+    x, y, z = xc[:, 0], xc[:, 1], xc[:, 2]
+    w = np.stack([1.0 + 0.2 * np.sin(3 * x + 1) * np.cos(2 * y) + 0.02 * rng.random(x.size),
+                  0.3 * np.sin(2 * y + 0.5), -0.2 * np.cos(3 * x), 0.05 * np.sin(4 * z + x),
+                  1.0 + 0.3 * np.cos(2 * x + y) + 0.02 * rng.random(x.size)], 1)
+    return mg.primit_to_conserv(w)
+
+
+CASES = {
+    "quad9": (lambda: mg.quad_mesh(9), dict(), dict()),
+    "tri7": (lambda: mg.tri_mesh(7), dict(), dict()),
+    "quad8lr_dirichlet": (lambda: mg.quad_mesh(8, tags="lr"),
+                          dict(bcs=[("freestream", (1.0, 0.2, 0.0, 0.0, 1.0)), ("outflowsupersonic", (0, 0, 0, 0, 0))]),
+                          dict(bcs_v=[("dirichlet", (0.1, -0.05, 0.0)), ("neumann", (0, 0, 0))],
+                               bcs_t=[("dirichlet", 1.5e-3), ("neumann", 0.0)])),
+}
+
+
+@pytest.mark.parametrize("second", [False, True])
+@pytest.mark.parametrize("name", list(CASES))
+def test_rhs_and_diag_match_oracle(solver, name, second):
+    print(f"[gpu] diffusion rhs {name} second={second}", flush=True)
+    mk, cfgkw, dkw = CASES[name]
+    mesh = mk()
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    sol = smooth_state(om.elem_coord)
+    mu = 2e-3
+    solver.set_config(make_config("multi_point", second_order=second, method=4, **cfgkw))
+    solver.set_diffusion(True, mu=mu, **dkw)
+    solver.put_sol(sol)
+    solver.compute_rhs_ns()
+    rhs = solver.get("rhs")
+    diag = solver.get("diag")
+    ocfg = O.make_config("multi_point", second_order=second, method=4, diffusion=True, mu=mu, **cfgkw, **dkw)
+    r0, sl = om.compute_euler_flux(ocfg, sol)
+    if second:
+        g = om.compute_cell_grad(sol, 4)[0]
+        r2, _ = om.compute_euler_flux(ocfg, sol, grad=g, second_order=True)
+        r0 = om.flux_limiting(sol, r0, r2, sl, ocfg.cfl)
+    rd, dd = om.add_diffusion(ocfg, sol)
+    ref_rhs = r0 + rd
+    scale_r = np.abs(rd).max(axis=0) + 1e-300
+    scale_d = np.abs(dd).max()
+    # Euler part bit-exact; the viscous part to 1e-11 of its scale
+    assert np.all(np.abs(rhs - ref_rhs) <= 1e-11 * scale_r[None, :] + 1e-14 * np.abs(ref_rhs)), \
+        np.abs(rhs - ref_rhs).max(axis=0) / scale_r
+    assert np.abs(diag - dd).max() <= 1e-11 * scale_d
+    solver.set_diffusion(False)
+
+
+@pytest.mark.parametrize("scheme,second,ts", [("multi_point", False, 0), ("multi_point", True, 0),
+                                              ("two_wave", True, 1)])
+def test_viscous_time_loop_matches_oracle(solver, scheme, second, ts):
+    print(f"[gpu] diffusion run {scheme} second={second} ts={ts}", flush=True)
+    mesh = mg.tri_mesh(8)
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    sol = smooth_state(om.elem_coord, 9)
+    mu = 5e-3
+    solver.set_config(make_config(scheme, second_order=second, method=4, cfl=0.3))
+    solver.set_diffusion(True, mu=mu)
+    solver.set_time(0.0)
+    solver.put_sol(sol)
+    solver.run(10, ts)
+    got = solver.get_sol()
+    st, ref, info = om.run(O.make_config(scheme, second_order=second, method=4, cfl=0.3, diffusion=True, mu=mu),
+                           sol, 10, ts)
+    assert st == 0
+    rel = np.abs(got - ref).max() / np.abs(ref).max()
+    assert rel <= 1e-11, rel
+    solver.set_diffusion(False)
+
+
+def test_vertices_beyond_the_limit_are_rejected(solver):
+    print("[gpu] diffusion limit", flush=True)
+    solver.build_mesh(mg.tet_mesh(3))  # interior vertices have 36 sub-faces
+    solver.set_config(make_config("multi_point", boundary_2d=False))
+    solver.set_diffusion(True, mu=1e-3)
+    solver.put_sol(mg.primit_to_conserv(np.tile([1.0, 0, 0, 0, 1.0], (solver.n_elems, 1))))
+    with pytest.raises(SubfvError):
+        solver.compute_rhs_ns()
+    solver.set_diffusion(False)
