@@ -2,8 +2,10 @@
 
 Default workload: 1024 x 1024 jittered quads split along seeded diagonals
 (2,097,152 one-layer prisms), FP64, second-order reconstruction with
-limited gradients (method 4), multi_point nodal-solver fluxes, SSP-RK3, a
-seeded vortex initial state.  A "step" is one SSP-RK3 step (3 stages).
+limited gradients (method 4), multi_point nodal-solver fluxes, viscous terms
+(mu = 1e-3, Pr = 0.72, point-implicit nodal momentum and heat systems),
+SSP-RK3, a seeded vortex initial state.  A "step" is one SSP-RK3 step
+(3 stages).
 ``value`` = cell updates per RK stage per second over the timed steps, with
 the state resident in HBM.  One JSON line is printed.
 
@@ -57,8 +59,9 @@ def read_traffic(kernel_tag):
     return None
 
 
-def cpu_baseline(n=512, steps=1):
-    """Oracle (single core) on an n x n tri mesh, second order, SSP-RK3."""
+def cpu_baseline(n=192, steps=1, viscous=True):
+    """Oracle (single core) on an n x n tri mesh, second order, SSP-RK3, with
+    the same viscous terms as the timed workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -66,14 +69,42 @@ def cpu_baseline(n=512, steps=1):
 
     om = O.OracleMesh(mg.tri_mesh(n))
     sol = mg.vortex_state(om.elem_coord)
-    cfg = O.make_config("multi_point", second_order=True, method=4, cfl=0.4)
+    cfg = O.make_config("multi_point", second_order=True, method=4, cfl=0.4, diffusion=viscous, mu=1e-3,
+                        prandtl=0.72)
     t0 = time.perf_counter()
     st, _, _ = om.run(cfg, sol, steps, 1)
     dt = time.perf_counter() - t0
     if st:
         raise RuntimeError(f"oracle status {st}")
     return {"value": om.n_elems * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"tri {n}x{n} ({om.n_elems} prisms), {steps} SSP-RK3 step(s), second order, multi_point"}
+            "sample": f"tri {n}x{n} ({om.n_elems} prisms), {steps} SSP-RK3 step(s), second order, multi_point"
+                      + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid")}
+
+
+def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24):
+    """Served mixed-size first-order solves (quad/tri 50/50, 32^2..512^2
+    cells log-uniform, seeded discontinuities, `steps` steps each, Poisson
+    arrivals at `rate`/s): continuous batching with autotuned parameters
+    against the same requests solved one at a time.  Mesh connectivity is
+    prepared on the client before the arrivals (outside the timed region);
+    upload, geometry, initial state, the steps and the read-back are timed."""
+    from subfv_amd import serve as S
+
+    reqs = S.prepare(S.make_requests(n_req, rate=rate, steps=steps))
+    calib = S.prepare(S.make_requests(calib_n, rate=1e9, steps=30, seed=S.SEED + 1))
+    best, trials = S.autotune(calib)
+    plain = S.summarize(*S.serve_plain(reqs))
+    served = S.summarize(*S.serve(reqs, best))
+    cells = sum(r.cells for r in reqs)
+    return {
+        "requests": n_req, "steps_per_request": steps, "arrival_rate_per_s": rate,
+        "mean_cells": cells / n_req, "cell_steps": cells * steps,
+        "served": served, "plain": plain,
+        "throughput_vs_plain": served["solves_per_s"] / plain["solves_per_s"],
+        "autotuned": {"chunk": best.chunk, "max_batch": best.max_batch, "use_graph": best.use_graph},
+        "autotune_trials": trials,
+        "note": "subsample of the 4096-request stream; same requests and arrival times for both",
+    }
 
 
 def main():
@@ -82,9 +113,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024, help="quads per side before the diagonal split")
-    ap.add_argument("--first-order", action="store_true", help="first-order forward-Euler stage (M1 headline)")
+    ap.add_argument("--first-order", action="store_true",
+                    help="inviscid first-order forward-Euler stage (the M1 configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-n", type=int, default=192)
+    ap.add_argument("--inviscid", action="store_true", help="drop the viscous terms (Euler only)")
+    ap.add_argument("--served-requests", type=int, default=192,
+                    help="requests in the served-solves measurement (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against known bytes) and exit")
@@ -116,6 +151,9 @@ def main():
     SF, SE = s.n_sub_faces, s.n_sub_elems
     o2 = not args.first_order
     s.set_config(make_config("multi_point", second_order=o2, method=4, cfl=0.4))
+    viscous = not (args.inviscid or args.first_order)  # M1 is the inviscid first-order stage
+    if viscous:
+        s.set_diffusion(True, mu=1e-3, prandtl=0.72)
     s.set_time(0.0)
     s.put_sol(mg.vortex_state(s.elem_coord))
     setup_s = time.perf_counter() - t0
@@ -131,7 +169,7 @@ def main():
     tf, dtf = s.run(args.steps, scheme)  # ends with a stream synchronisation
     elapsed = time.perf_counter() - t1
     phases = {}
-    for ph in ("node_flux", "node_flux_o2", "grad", "gather", "update"):
+    for ph in ("node_flux", "node_flux_o2", "grad", "gather", "diffusion", "update"):
         ms, cnt = s.profile_read(ph)
         if cnt:
             phases[ph] = {"ms_total": ms, "launches": cnt, "us_avg": 1e3 * ms / cnt}
@@ -166,8 +204,10 @@ def main():
         "config": {
             "workload": (f"tri {args.n}x{args.n} jittered, {C} prisms, multi_point, "
                          + ("first order, forward Euler" if args.first_order else
-                            "second order (method 4 limited gradients), SSP-RK3")),
-            "cells": C, "stages_per_step": stages, "inviscid": True,
+                            "second order (method 4 limited gradients), SSP-RK3")
+                         + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid")),
+            "cells": C, "stages_per_step": stages, "viscous": viscous, "mu": 1e-3 if viscous else 0.0,
+            "prandtl": 0.72 if viscous else None,
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_node_flux<multi_point>", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -183,8 +223,10 @@ def main():
         "t_final": tf,
     }
     s.close()
+    if args.served_requests > 0 and not args.first_order:
+        out["served"] = served_bench(args.served_requests)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, viscous=viscous)
     print(json.dumps(out), flush=True)
 
 

@@ -104,7 +104,10 @@ int subfv_mesh_build(subfv_ctx *ctx, int32_t n_vert, const double *coord, int32_
                      const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
                      const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag,
                      int32_t boundary_2d);
-/* "n_vert", "n_elems", "n_faces", "n_sub_faces", "n_sub_elems", "n_max_sub_faces" */
+/* The same from a connectivity built beforehand on the host
+ * (subfv_host_mesh_build): upload, device geometry and BC data only. */
+int subfv_mesh_upload(subfv_ctx *ctx, const subfv_host_mesh *hm, int32_t boundary_2d);
+/* "n_vert", "n_elems", "n_faces", "n_sub_faces", "n_sub_elems", "n_max_sub_faces", "n_groups" */
 int64_t subfv_mesh_count(const subfv_ctx *ctx, const char *what);
 /* Connectivity in the reference numbering (same names as the mesh_type
  * components, e.g. "vert_sub_face", "sub_face_left_elem"); returns the number
@@ -197,6 +200,24 @@ int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs);
  * stats[2] = SpMV count. Returns SUBFV_E_NOT_CONVERGED (soft) if not conv. */
 int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                           int32_t precision, int32_t *conv, double *stats);
+
+/* ------------------------------------------------------- served batches */
+/* No counterpart in the reference (it runs one solve per process): a batch
+ * advances many independent contexts together, one multi-instance launch per
+ * kernel of a step instead of one launch per context.  Each context keeps
+ * its own mesh, state, dt and omega, so its results are bit-identical to
+ * subfv_run(ctx, n_steps, 0, ...) alone.  Members must share the device and
+ * the scheme and run the first-order inviscid forward-Euler loop.
+ * subfv_batch_run enqueues n_steps steps on the batch's stream and returns;
+ * with use_graph, whole 3-step cycles replay one captured hipGraph (re-captured
+ * when the membership changes).  Call subfv_batch_wait before using a member
+ * context again. */
+typedef struct subfv_batch subfv_batch;
+int subfv_batch_create(subfv_batch **b, int device);
+void subfv_batch_destroy(subfv_batch *b);
+const char *subfv_batch_error(const subfv_batch *b);
+int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps, int32_t use_graph);
+int subfv_batch_wait(subfv_batch *b);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,7 @@ SIGNATURES = {
     "subfv_destroy": (None, [_P]),
     "subfv_error": (C.c_char_p, [_P]),
     "subfv_mesh_build": (C.c_int, [_P, _i32, _dp, _i32, _ip, _ip, _i32, _ip, _ip, _ip, _i32]),
+    "subfv_mesh_upload": (C.c_int, [_P, _P, _i32]),
     "subfv_mesh_count": (_i64, [_P, C.c_char_p]),
     "subfv_mesh_get_i32": (_i64, [_P, C.c_char_p, _ip, _i64]),
     "subfv_mesh_get_f64": (_i64, [_P, C.c_char_p, _dp, _i64]),
@@ -117,6 +118,11 @@ SIGNATURES = {
     "subfv_run": (C.c_int, [_P, _i32, _i32, _dp, _dp]),
     "subfv_profile_enable": (C.c_int, [_P, _i32]),
     "subfv_profile_read": (C.c_int, [_P, C.c_char_p, _dp, C.POINTER(C.c_int64)]),
+    "subfv_batch_create": (C.c_int, [_PP, C.c_int]),
+    "subfv_batch_destroy": (None, [_P]),
+    "subfv_batch_error": (C.c_char_p, [_P]),
+    "subfv_batch_run": (C.c_int, [_P, C.POINTER(C.c_void_p), _i32, _i32, _i32]),
+    "subfv_batch_wait": (C.c_int, [_P]),
     "subfv_bcsr_create": (C.c_int, [_P, _PP, _i32, _i32, _i32, _ip, _ip, _dp]),
     "subfv_bcsr_destroy": (None, [_P]),
     "subfv_bcsr_matmul": (C.c_int, [_P, _dp, _dp]),

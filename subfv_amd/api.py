@@ -95,6 +95,16 @@ class Solver:
         self._check(L.subfv_mesh_build(self.h, mesh.n_vert, _lib.dptr(coord), mesh.n_elems, _lib.iptr(kind),
                                        _lib.iptr(ev), int(bn.size), _lib.iptr(bn), _lib.iptr(bv), _lib.iptr(bt),
                                        int(mesh.boundary_2d)))
+        self._counts()
+
+    def upload_mesh(self, host_mesh: "HostMesh", boundary_2d: bool):
+        """Device side of build_mesh for a connectivity built beforehand on the
+        host (HostMesh): upload, geometry and BC data."""
+        self._check(_lib.lib().subfv_mesh_upload(self.h, host_mesh.h, int(boundary_2d)))
+        self._counts()
+
+    def _counts(self):
+        L = _lib.lib()
         self.n_vert = int(L.subfv_mesh_count(self.h, b"n_vert"))
         self.n_elems = int(L.subfv_mesh_count(self.h, b"n_elems"))
         self.n_faces = int(L.subfv_mesh_count(self.h, b"n_faces"))

@@ -10,6 +10,7 @@
 
 #include "../../include/subfv.h"
 #include "device_mesh.hpp"
+#include "kernels_batch.hpp"
 #include "kernels_bcsr.hpp"
 #include "kernels_cell.hpp"
 #include "kernels_diff.hpp"
@@ -21,6 +22,21 @@ using namespace sfv;
 
 struct subfv_host_mesh {
   subfv::HostMesh hm;
+};
+
+// A batch of contexts advanced together by multi-instance launches.
+struct subfv_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<subfv_ctx *> members;
+  int scheme = 0, total_g = 0, total_c = 0;
+  sfv::InstDesc *d_desc = nullptr;
+  int *d_gtab = nullptr, *d_ctab = nullptr;
+  size_t cap_desc = 0, cap_gtab = 0, cap_ctab = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ok = false;
 };
 
 struct subfv_ctx {
@@ -300,6 +316,21 @@ int apply_bc(subfv_ctx *c) {
   DevMesh &d = c->dm;
   hipLaunchKernelGGL(k_wall_normal, dim3(nblk(d.V, 256)), dim3(256), 0, c->stream, d, c->bc);
   HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// upload of a built host connectivity, device geometry and BC data
+int finish_mesh(subfv_ctx *c, int32_t boundary_2d) {
+  c->b2d = boundary_2d != 0;
+  int r = upload_mesh(c);
+  if (r) return r;
+  r = geometry(c);
+  if (r) return r;
+  c->has_mesh = true;
+  c->diff_ready = false;
+  r = apply_bc(c);
+  if (r) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -662,16 +693,15 @@ int subfv_mesh_build(subfv_ctx *c, int32_t n_vert, const double *coord, int32_t 
   std::string e = subfv::build_host_mesh(c->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
                                          bf_vert, bf_tag);
   if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
-  c->b2d = boundary_2d != 0;
-  int r = upload_mesh(c);
-  if (r) return r;
-  r = geometry(c);
-  if (r) return r;
-  c->has_mesh = true;
-  r = apply_bc(c);
-  if (r) return r;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return 0;
+  return finish_mesh(c, boundary_2d);
+}
+
+int subfv_mesh_upload(subfv_ctx *c, const subfv_host_mesh *hm, int32_t boundary_2d) {
+  if (!c || !hm) return fail(c, SUBFV_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  free_mesh(c);
+  c->hm = hm->hm;
+  return finish_mesh(c, boundary_2d);
 }
 
 int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
@@ -1328,6 +1358,211 @@ int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x_out, const double *rhs, doubl
     stats[2] = (double)nspmv;
   }
   return conv ? 0 : fail(c, SUBFV_E_NOT_CONVERGED, "GMRES did not converge");
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- batches
+namespace {
+
+void batch_drop_graph(subfv_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->graph_ok = false;
+}
+
+int bfail(subfv_batch *b, int code, const std::string &m) {
+  b->err = m;
+  return code;
+}
+
+#define BHIP(b, expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) return bfail((b), SUBFV_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <class T>
+int grow(subfv_batch *b, T **p, size_t *cap, size_t n) {
+  if (n <= *cap && *p) return 0;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  size_t c = std::max<size_t>(n, 64) * 2;
+  BHIP(b, hipMalloc((void **)p, c * sizeof(T)));
+  *cap = c;
+  b->graph_ok = false;  // captured kernel arguments point at the old buffer
+  return 0;
+}
+
+// the argument blocks of one instance, as launch_flux / launch_gather /
+// launch_update set them for a first-order forward-Euler step
+void fill_desc(subfv_ctx *c, InstDesc &d) {
+  std::memset(&d, 0, sizeof(d));
+  d.fa.m = c->dm;
+  d.fa.bc = c->bc;
+  d.fa.U = nullptr;
+  d.fa.G = nullptr;
+  d.fa.P = c->P;
+  d.fa.vp = nullptr;
+  d.fa.err = c->derr;
+  d.fa.t = &c->st->t;
+  d.fa.second_order = 0;
+  d.fa.nblocks = c->dm.ngroups;
+  d.ga.m = c->dm;
+  d.ga.P = c->P;
+  d.ga.P2 = nullptr;
+  d.ga.rhs = c->rhs;
+  d.ga.rhs2 = nullptr;
+  d.ga.sl = c->sl;
+  d.ga.blk = c->blk;
+  d.ga.st = c->st;
+  d.ga.fixed_dt = 0;
+  d.ua.m = c->dm;
+  d.ua.U = nullptr;
+  d.ua.U0 = nullptr;
+  d.ua.Uout = nullptr;
+  d.ua.rhs = c->rhs;
+  d.ua.rhs2 = nullptr;
+  d.ua.sl = c->sl;
+  d.ua.diag = nullptr;
+  d.ua.rdiff = nullptr;
+  d.ua.blk = c->blk;
+  d.ua.st = c->st;
+  d.ua.a1 = 1.0;
+  d.ua.a2 = 1.0;
+  d.ua.w0 = 0.0;
+  d.ua.w1 = 1.0;
+  d.ua.flux_limit = 0;
+  d.ua.advance_t = 1;
+  for (int i = 0; i < 3; i++) d.U[i] = c->U[i];
+  d.cur0 = c->cur;
+  d.ngroups = c->dm.ngroups;
+  d.ncblk = (int)nblk(c->dm.C, kCellThreads);
+}
+
+template <int S>
+void launch_phase(subfv_batch *b, int n, int phase) {
+  const int per_xcd = (b->total_g + 7) / 8;
+  hipLaunchKernelGGL(k_node_flux_multi<S>, dim3(8 * per_xcd), dim3(kGrpThreads), 0, b->stream,
+                     (const InstDesc *)b->d_desc, (const int *)b->d_gtab, b->total_g, per_xcd, phase);
+  hipLaunchKernelGGL(k_gather_multi, dim3(b->total_c), dim3(kCellThreads), 0, b->stream, (const InstDesc *)b->d_desc,
+                     (const int *)b->d_ctab, b->total_c);
+  hipLaunchKernelGGL(k_finalize_dt_multi, dim3(n), dim3(kCellThreads), 0, b->stream, (const InstDesc *)b->d_desc);
+  hipLaunchKernelGGL(k_update_multi, dim3(b->total_c), dim3(kCellThreads), 0, b->stream, (const InstDesc *)b->d_desc,
+                     (const int *)b->d_ctab, b->total_c, phase);
+  hipLaunchKernelGGL(k_finalize_omega_multi, dim3(n), dim3(kCellThreads), 0, b->stream, (const InstDesc *)b->d_desc);
+  hipLaunchKernelGGL(k_update_fixup_multi, dim3(b->total_c), dim3(kCellThreads), 0, b->stream,
+                     (const InstDesc *)b->d_desc, (const int *)b->d_ctab, b->total_c, phase);
+}
+
+void launch_phase_any(subfv_batch *b, int n, int phase) {
+  switch (b->scheme) {
+    case kMultiPoint: launch_phase<kMultiPoint>(b, n, phase); break;
+    case kMultiPointIso: launch_phase<kMultiPointIso>(b, n, phase); break;
+    case kThreeWave: launch_phase<kThreeWave>(b, n, phase); break;
+    case kTwoWave: launch_phase<kTwoWave>(b, n, phase); break;
+    case kModifiedThreeWave: launch_phase<kModifiedThreeWave>(b, n, phase); break;
+    case kMultiPointPressure: launch_phase<kMultiPointPressure>(b, n, phase); break;
+    default: launch_phase<kMultiPointPressurePh>(b, n, phase); break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int subfv_batch_create(subfv_batch **out, int device) {
+  if (!out) return SUBFV_E_INVALID;
+  *out = nullptr;
+  subfv_batch *b = new subfv_batch();
+  b->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return SUBFV_E_DEVICE;
+  }
+  *out = b;
+  return 0;
+}
+
+void subfv_batch_destroy(subfv_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_gtab) (void)hipFree(b->d_gtab);
+  if (b->d_ctab) (void)hipFree(b->d_ctab);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *subfv_batch_error(const subfv_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps, int32_t use_graph) {
+  if (!b || !ctxs || n < 1 || n_steps < 0) return SUBFV_E_INVALID;
+  if (n > (1 << (31 - kTabShift)) - 1) return bfail(b, SUBFV_E_INVALID, "too many instances in one batch");
+  BHIP(b, hipSetDevice(b->device));
+  const int scheme = ctxs[0] ? ctxs[0]->bc.scheme : 0;
+  for (int i = 0; i < n; i++) {
+    subfv_ctx *c = ctxs[i];
+    if (!c || !c->has_mesh) return bfail(b, SUBFV_E_INVALID, "batch member without a mesh");
+    if (c->device != b->device) return bfail(b, SUBFV_E_INVALID, "batch members must share the device");
+    if (c->cfg.second_order || c->diff_on)
+      return bfail(b, SUBFV_E_INVALID, "batches run the first-order inviscid forward-Euler loop");
+    if (c->bc.scheme != scheme) return bfail(b, SUBFV_E_INVALID, "batch members must share the scheme");
+    if (c->dm.ngroups > kTabMask || (long)nblk(c->dm.C, kCellThreads) > kTabMask)
+      return bfail(b, SUBFV_E_INVALID, "instance too large for a batch");
+  }
+  std::vector<subfv_ctx *> mem(ctxs, ctxs + n);
+  if (mem != b->members || scheme != b->scheme) {
+    // pending uploads and state puts of the members complete first
+    for (auto *c : mem) BHIP(b, hipStreamSynchronize(c->stream));
+    b->members = mem;
+    b->scheme = scheme;
+    std::vector<int> gt, ct;
+    for (int i = 0; i < n; i++) {
+      for (int g = 0; g < ctxs[i]->dm.ngroups; g++) gt.push_back((i << kTabShift) | g);
+      const int nc = (int)nblk(ctxs[i]->dm.C, kCellThreads);
+      for (int q = 0; q < nc; q++) ct.push_back((i << kTabShift) | q);
+    }
+    int r;
+    if ((r = grow(b, &b->d_gtab, &b->cap_gtab, gt.size()))) return r;
+    if ((r = grow(b, &b->d_ctab, &b->cap_ctab, ct.size()))) return r;
+    BHIP(b, hipMemcpyAsync(b->d_gtab, gt.data(), gt.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_ctab, ct.data(), ct.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    b->total_g = (int)gt.size();
+    b->total_c = (int)ct.size();
+    batch_drop_graph(b);
+  }
+  int r;
+  if ((r = grow(b, &b->d_desc, &b->cap_desc, (size_t)n))) return r;
+  std::vector<InstDesc> hd(n);
+  for (int i = 0; i < n; i++) fill_desc(ctxs[i], hd[i]);
+  BHIP(b, hipMemcpyAsync(b->d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, b->stream));
+  int s = 0;
+  if (use_graph && n_steps >= 3) {
+    if (!b->graph_ok) {
+      batch_drop_graph(b);
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int ph = 0; ph < 3; ph++) launch_phase_any(b, n, ph);
+      BHIP(b, hipStreamEndCapture(b->stream, &b->graph));
+      BHIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_ok = true;
+    }
+    for (; s + 3 <= n_steps; s += 3) BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+  }
+  for (; s < n_steps; s++) launch_phase_any(b, n, s % 3);
+  BHIP(b, hipGetLastError());
+  for (auto *c : mem) c->cur = (c->cur + n_steps) % 3;
+  return 0;
+}
+
+int subfv_batch_wait(subfv_batch *b) {
+  if (!b) return SUBFV_E_INVALID;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return 0;
 }
 
 }  // extern "C"

@@ -31,7 +31,7 @@ constexpr int kCellThreads = 256;
 // reduces them.  A device-wide fence per block would write back the whole
 // L2 of the issuing XCD, so none is used.  min is order-independent, so the
 // result is exact.
-SFV_DEV void block_min_store(double v, double *blk) {
+SFV_DEV void block_min_store(double v, double *blk, int bidx) {
   __shared__ double s_m[kCellThreads];
   s_m[threadIdx.x] = v;
   __syncthreads();
@@ -39,8 +39,9 @@ SFV_DEV void block_min_store(double v, double *blk) {
     if (threadIdx.x < o) s_m[threadIdx.x] = dmin(s_m[threadIdx.x], s_m[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) blk[blockIdx.x] = s_m[0];
+  if (threadIdx.x == 0) blk[bidx] = s_m[0];
 }
+SFV_DEV void block_min_store(double v, double *blk) { block_min_store(v, blk, blockIdx.x); }
 
 SFV_DEV double reduce_min_one_block(const double *blk, int nblk) {
   __shared__ double s_m[kCellThreads];
@@ -57,8 +58,7 @@ SFV_DEV double reduce_min_one_block(const double *blk, int nblk) {
 
 // dt = cfl*min(vol/sum_lambda) (ns_euler_module.F90:546-560; the flux
 // limiter's own dt, ns_euler_recon_module.F90:575-581, has no tmax clamp)
-__global__ __launch_bounds__(kCellThreads) void k_finalize_dt(const double *blk, int nblk, StepState *st,
-                                                              int fixed_dt) {
+SFV_DEV void finalize_dt(const double *blk, int nblk, StepState *st, int fixed_dt) {
   double mn = reduce_min_one_block(blk, nblk);
   if (threadIdx.x != 0) return;
   double d = dmin(1e10, mn);
@@ -70,10 +70,13 @@ __global__ __launch_bounds__(kCellThreads) void k_finalize_dt(const double *blk,
     st->dt = dt;
   }
 }
+__global__ __launch_bounds__(kCellThreads) void k_finalize_dt(const double *blk, int nblk, StepState *st,
+                                                              int fixed_dt) {
+  finalize_dt(blk, nblk, st, fixed_dt);
+}
 
 // omega checks and time advance of add_timedisc_and_solve (:595-608)
-__global__ __launch_bounds__(kCellThreads) void k_finalize_omega(const double *blk, int nblk, StepState *st,
-                                                                 int advance_t) {
+SFV_DEV void finalize_omega(const double *blk, int nblk, StepState *st, int advance_t) {
   double om = reduce_min_one_block(blk, nblk);
   if (threadIdx.x != 0) return;
   if (st->done || st->err) return;
@@ -95,6 +98,10 @@ __global__ __launch_bounds__(kCellThreads) void k_finalize_omega(const double *b
     if (!st->lts && st->t >= st->tmax) st->done = 1;
   }
 }
+__global__ __launch_bounds__(kCellThreads) void k_finalize_omega(const double *blk, int nblk, StepState *st,
+                                                                 int advance_t) {
+  finalize_omega(blk, nblk, st, advance_t);
+}
 
 struct GatherArgs {
   DevMesh m;
@@ -112,9 +119,9 @@ struct GatherArgs {
 // sum_lambda(c) = 1e-12 + ... (ns_euler_module.F90:181-183, 330-335), and
 // the per-block minima of vol/sum_lambda that k_finalize_dt turns into
 // dt = cfl*min(vol/sum_lambda) (:546-560).
-__global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
+SFV_DEV void gather_block(const GatherArgs &a, long bidx) {
   const DevMesh &m = a.m;
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = bidx * kCellThreads + threadIdx.x;
   double local = 1e10;
   if (c < m.C) {
     const long C = m.C;
@@ -138,8 +145,9 @@ __global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
     }
     local = m.cvol[c] / lam;
   }
-  block_min_store(local, a.blk);
+  block_min_store(local, a.blk, (int)bidx);
 }
+__global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) { gather_block(a, blockIdx.x); }
 
 struct UpdateArgs {
   DevMesh m;
@@ -160,12 +168,12 @@ struct UpdateArgs {
 // add_timedisc_and_solve (ns_euler_module.F90:565-580).  Returns the cell's
 // omega; u and delta receive the state and the increment.
 template <bool DIAG>
-SFV_DEV double stage_cell(const UpdateArgs &a, long c, double *u, double *delta) {
+SFV_DEV double stage_cell(const UpdateArgs &a, const double *U, long c, double *u, double *delta) {
   const long C = a.m.C;
   const StepState *st = a.st;
   double r[5];
   for (int k = 0; k < 5; k++) {
-    u[k] = a.U[5 * c + k];
+    u[k] = U[5 * c + k];
     r[k] = a.rhs[k * C + c];
   }
   const double vol = a.m.cvol[c];
@@ -204,44 +212,54 @@ SFV_DEV double stage_cell(const UpdateArgs &a, long c, double *u, double *delta)
   return max_omega_pos(u, ad);
 }
 
-SFV_DEV void store_stage(const UpdateArgs &a, long c, const double *u, const double *delta, double omega) {
-  const long C = a.m.C;
+SFV_DEV void store_stage(const UpdateArgs &a, double *Uout, long c, const double *u, const double *delta,
+                         double omega) {
   for (int k = 0; k < 5; k++) {
     double x = u[k] + a.a2 * omega * delta[k];
     if (a.U0) x = a.w0 * a.U0[5 * c + k] + a.w1 * x;
-    a.Uout[5 * c + k] = x;
+    Uout[5 * c + k] = x;
   }
 }
 
 template <bool DIAG>
-__global__ __launch_bounds__(kCellThreads) void k_update(UpdateArgs a) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+SFV_DEV void update_block(const UpdateArgs &a, const double *U, double *Uout, long bidx) {
+  long c = bidx * kCellThreads + threadIdx.x;
   StepState *st = a.st;
   if (st->done || st->err) {  // uniform over the grid: the state is frozen
     if (c < a.m.C)
-      for (int k = 0; k < 5; k++) a.Uout[5 * c + k] = a.U[5 * c + k];
+      for (int k = 0; k < 5; k++) Uout[5 * c + k] = U[5 * c + k];
     return;
   }
   double om_c = 1.0;
   if (c < a.m.C) {
     double u[5], d[5];
-    om_c = stage_cell<DIAG>(a, c, u, d);
-    store_stage(a, c, u, d, 1.0);
+    om_c = stage_cell<DIAG>(a, U, c, u, d);
+    store_stage(a, Uout, c, u, d, 1.0);
   }
-  block_min_store(om_c, a.blk);
+  block_min_store(om_c, a.blk, (int)bidx);
+}
+
+template <bool DIAG>
+__global__ __launch_bounds__(kCellThreads) void k_update(UpdateArgs a) {
+  update_block<DIAG>(a, a.U, a.Uout, blockIdx.x);
 }
 
 // Correction when the global omega came out below one: redo the cell with
 // sol = sol + a2*omega*delta (ns_euler_module.F90:608).
 template <bool DIAG>
-__global__ __launch_bounds__(kCellThreads) void k_update_fixup(UpdateArgs a) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+SFV_DEV void update_fixup_block(const UpdateArgs &a, const double *U, double *Uout, long bidx) {
+  long c = bidx * kCellThreads + threadIdx.x;
   const StepState *st = a.st;
   if (!st->fixup || st->err) return;
   if (c >= a.m.C) return;
   double u[5], d[5];
-  stage_cell<DIAG>(a, c, u, d);
-  store_stage(a, c, u, d, st->omega);
+  stage_cell<DIAG>(a, U, c, u, d);
+  store_stage(a, Uout, c, u, d, st->omega);
+}
+
+template <bool DIAG>
+__global__ __launch_bounds__(kCellThreads) void k_update_fixup(UpdateArgs a) {
+  update_fixup_block<DIAG>(a, a.U, a.Uout, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- gradients

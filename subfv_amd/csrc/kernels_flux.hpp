@@ -117,13 +117,11 @@ SFV_DEV void nodal_velocity(const BCParams &bc, const double *S, const double *B
   }
 }
 
+// The sweep of one vertex group lb of the mesh in a, state U.
 template <int SCHEME>
-__global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
-  __shared__ FluxLds L;
+SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, int lb, FluxLds &L) {
   const DevMesh &m = a.m;
   const BCParams &bc = a.bc;
-  const int lb = xcd_block(a.per_xcd);
-  if (lb >= a.nblocks) return;  // uniform over the workgroup
   const int tid = threadIdx.x;
   const int *gh = m.grp + kGrpHdr * (long)lb;
   const int vb = gh[0], nv = gh[1], sf0 = gh[2], nsf = gh[3], se0 = gh[4], nse = gh[5];
@@ -165,7 +163,7 @@ __global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
   double wl[5], wr[5];
   {
     double u[5];
-    for (int k = 0; k < 5; k++) u[k] = a.U[5L * le + k];
+    for (int k = 0; k < 5; k++) u[k] = U[5L * le + k];
     to_prim(u, wl);
     if (a.second_order) {
       double dx[3];
@@ -174,7 +172,7 @@ __global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
       for (int k = 0; k < 5; k++) wl[k] = wl[k] + (g[3 * k] * dx[0] + g[3 * k + 1] * dx[1] + g[3 * k + 2] * dx[2]);
     }
     if (re >= 0) {
-      for (int k = 0; k < 5; k++) u[k] = a.U[5L * re + k];
+      for (int k = 0; k < 5; k++) u[k] = U[5L * re + k];
       to_prim(u, wr);
       if (a.second_order) {
         double dx[3];
@@ -369,6 +367,14 @@ __global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
 #pragma unroll
     for (int c = 0; c < 6; c++) out[c] = acc[c];
   }
+}
+
+template <int SCHEME>
+__global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
+  __shared__ FluxLds L;
+  const int lb = xcd_block(a.per_xcd);
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  node_flux_group<SCHEME>(a, a.U, lb, L);
 }
 
 }  // namespace sfv

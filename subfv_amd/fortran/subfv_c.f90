@@ -150,6 +150,15 @@ module subfv_c
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function subfv_set_diffusion(ctx, activate, mu_p, cv_p, prandtl, n_bc, bc_v_id, bc_val_v, &
+        bc_t_id, bc_val_t) bind(C, name="subfv_set_diffusion")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: activate, n_bc
+      real(c_double), value :: mu_p, cv_p, prandtl
+      integer(c_int32_t), intent(in) :: bc_v_id(*), bc_t_id(*)
+      real(c_double), intent(in) :: bc_val_v(*), bc_val_t(*)
+    end function
     integer(c_int) function subfv_run(ctx, n_steps, time_scheme, t, dt) bind(C, name="subfv_run")
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx

@@ -1,7 +1,8 @@
 ! subfvns on one MI355X: the driver of src/ns/ns_main.F90 over the device
 ! path.  It reads the reference's namelist file (&INPUT_PARAM,
 ! ns_global_data_module.F90:177-213; every entry is accepted, the ones the
-! explicit path uses are honoured) and its msh 2.2 mesh, builds the mesh on
+! explicit path and its viscous terms use are honoured) and its msh 2.2 mesh,
+! builds the mesh on
 ! the device, sets the initial state (init_uniform or init_1drp,
 ! ns_euler_module.F90:20-70), runs the time loop of ns_main.F90:83-107 with
 ! the state resident in HBM, prints the final time and the wall time of the
@@ -78,6 +79,7 @@ program subfvns_gpu
   real(DP) :: t, dt, w(5), wall_time
   integer(8) :: start_count, end_count, count_rate
   integer(c_int) :: st
+  integer(c_int32_t) :: bc_v_id(mnbc) = 1, bc_t_id(mnbc) = 1
 
   nlfile = "input_data.f"
   if (command_argument_count() >= 1) call get_command_argument(1, nlfile)
@@ -123,7 +125,6 @@ program subfvns_gpu
   cfg%method = method
   cfg%cfl = cfl
   cfg%tmax = tmax
-  if (activate_diffusion) print *, "note: activate_diffusion is ignored by the explicit device path"
   if (periodic_mesh .or. rescale .or. use_cylinder_map) error stop "periodic/rescaled/mapped meshes are not supported"
 
   ! ---- mesh and initial state
@@ -131,6 +132,17 @@ program subfvns_gpu
   call read_mesh_msh_2(mesh, meshfile_path, meshfile, n_bc, bc_name)
   call build_mesh(ctx, mesh, boundary_2d)
   call check(ctx, subfv_set_config(ctx, cfg))
+  if (activate_diffusion) then
+    ! viscous terms with constant viscosity; Prandtl is 0.71 in the reference
+    ! (ns_global_data_module.F90:10); bc_type_V / bc_type_T per BC index
+    do i = 1, n_bc
+      bc_v_id(i) = merge(2, 1, trim(adjustl(bc_type_V(i))) == "dirichlet")
+      bc_t_id(i) = merge(2, 1, trim(adjustl(bc_type_T(i))) == "dirichlet")
+    end do
+    if (use_sutherland) error stop "use_sutherland is not supported on the device path"
+    call check(ctx, subfv_set_diffusion(ctx, 1, mu_p, Cv_p, 0.71_DP, int(n_bc, c_int32_t), bc_v_id, bc_val_V, &
+      bc_t_id, bc_val_T))
+  end if
   n_elems = mesh_count(ctx, "n_elems")
   n_vert = mesh_count(ctx, "n_vert")
   allocate (sol(5, n_elems), xc(3, n_elems), vol(n_elems))

@@ -1,0 +1,254 @@
+"""Served mixed-size explicit solves on one GPU.
+
+A request carries a mesh (quad or prism grid of a seeded size, connectivity
+already built on the client with HostMesh, as a mesh file would be read),
+a seeded two-state initial condition (init_1drp, ns_euler_module.F90:36-43)
+and a step count.  The server admits arriving requests while the HBM budget
+and the batch capacity allow, keeps every admitted state resident, and
+advances all of them together with the multi-instance launches of
+subfv_batch_run (continuous batching: requests that arrive while others run
+are admitted between chunks of steps, and finished ones leave).  Each
+instance's arithmetic is that of a solve alone, so results are bit-identical
+to subfv_run on the same request.
+
+`serve_plain` is the baseline: the same requests one at a time, in arrival
+order, each with its own launches (subfv_run).  `autotune` picks the chunk
+length, the batch capacity and graph replay by timing candidates on a
+calibration sample.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import meshgen as mg
+from .api import HostMesh, Solver, SubfvError, make_config
+
+SEED = 20261202
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str           # "quad" or "tri"
+    n: int              # grid size (tri: 2 n^2 prisms)
+    arrival: float      # seconds after the start
+    wl: np.ndarray      # primitive left state
+    wr: np.ndarray      # primitive right state
+    x_split: float
+    steps: int
+    host: HostMesh | None = None
+    boundary_2d: bool = True
+    cells: int = 0
+
+
+def make_requests(n_req: int = 4096, rate: float = 200.0, steps: int = 300, seed: int = SEED,
+                  cmin: int = 32 ** 2, cmax: int = 512 ** 2) -> list[Request]:
+    """Seeded request stream: Poisson arrivals at `rate` per second, quad or
+    tri 50/50, cell count log-uniform in [cmin, cmax], left/right states with
+    rho, p in [0.1, 1] and |u| <= 0.5, split position in [0.3, 0.7]."""
+    ids = np.arange(n_req, dtype=np.uint64)
+    u = [mg.uniform(seed, 100 + k, ids) for k in range(12)]
+    t = np.cumsum(-np.log1p(-u[0]) / rate)
+    out = []
+    for i in range(n_req):
+        kind = "quad" if u[1][i] < 0.5 else "tri"
+        cells = math.exp(math.log(cmin) + u[2][i] * (math.log(cmax) - math.log(cmin)))
+        n = max(4, int(round(math.sqrt(cells if kind == "quad" else cells / 2))))
+        sp_l, an_l = 0.5 * math.sqrt(u[5][i]), 2 * math.pi * u[6][i]
+        sp_r, an_r = 0.5 * math.sqrt(u[9][i]), 2 * math.pi * u[10][i]
+        wl = np.array([0.1 + 0.9 * u[3][i], sp_l * math.cos(an_l), sp_l * math.sin(an_l), 0.0, 0.1 + 0.9 * u[4][i]])
+        wr = np.array([0.1 + 0.9 * u[7][i], sp_r * math.cos(an_r), sp_r * math.sin(an_r), 0.0, 0.1 + 0.9 * u[8][i]])
+        out.append(Request(i, kind, n, float(t[i]), wl, wr, 0.3 + 0.4 * float(u[11][i]), steps))
+    return out
+
+
+def prepare(reqs: list[Request], workers: int = 16) -> list[Request]:
+    """Client side: generate each mesh and build its connectivity (threads;
+    the C-ABI call releases the interpreter lock)."""
+    def one(r: Request):
+        mesh = mg.quad_mesh(r.n, seed=SEED + r.rid) if r.kind == "quad" else mg.tri_mesh(r.n, seed=SEED + r.rid)
+        r.host = HostMesh(mesh)
+        r.boundary_2d = mesh.boundary_2d
+        r.cells = mesh.n_elems
+        return r
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, reqs))
+
+
+def config_for(r: Request):
+    return make_config("multi_point", bc_style=2, boundary_2d=r.boundary_2d, local_time_step=False,
+                       second_order=False, cfl=0.4, tmax=1e30)
+
+
+def admit(s: Solver, r: Request):
+    """Upload the mesh, set the configuration and the init_1drp state."""
+    s.upload_mesh(r.host, r.boundary_2d)
+    s.set_config(config_for(r))
+    xc = s.elem_coord
+    w = np.where((xc[:, 0] < r.x_split)[:, None], r.wl[None, :], r.wr[None, :])
+    s.put_sol(mg.primit_to_conserv(w))
+    s.set_time(0.0)
+
+
+def device_bytes(r: Request) -> int:
+    """HBM an admitted instance holds (mesh, partials, states, work arrays)."""
+    return 3200 * r.cells + (4 << 20)
+
+
+class Batch:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = _lib.lib().subfv_batch_create(C.byref(h), device)
+        if st:
+            raise SubfvError(st, "subfv_batch_create failed")
+        self.h = h
+
+    def run(self, solvers, n_steps: int, use_graph: bool = True):
+        arr = (C.c_void_p * len(solvers))(*[s.h.value for s in solvers])
+        st = _lib.lib().subfv_batch_run(self.h, arr, len(solvers), n_steps, int(use_graph))
+        if st:
+            raise SubfvError(st, _lib.lib().subfv_batch_error(self.h).decode())
+
+    def wait(self):
+        st = _lib.lib().subfv_batch_wait(self.h)
+        if st:
+            raise SubfvError(st, _lib.lib().subfv_batch_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().subfv_batch_destroy(self.h)
+            self.h = None
+
+
+@dataclass
+class Result:
+    rid: int
+    latency: float
+    finish: float
+    sol: np.ndarray | None = None
+
+
+@dataclass
+class Params:
+    chunk: int = 30          # steps per batch call (a multiple of 3 keeps the graph cycle aligned)
+    max_batch: int = 256     # instances resident at once
+    use_graph: bool = True
+    budget: int = 96 << 30   # HBM bytes for admitted instances
+
+
+def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep: bool = False,
+          realtime: bool = True) -> tuple[list[Result], float]:
+    """Continuous batching.  Returns the results (latency = finish - arrival)
+    and the makespan in seconds."""
+    batch = Batch(device)
+    pool: list[Solver] = []
+    pending = sorted(reqs, key=lambda r: r.arrival)
+    pi = 0
+    active = []  # [request, solver, remaining]
+    used = 0
+    results = []
+    t0 = time.perf_counter()
+
+    def now():
+        return time.perf_counter() - t0 if realtime else float("inf")
+
+    def admit_due(limit):
+        nonlocal pi, used
+        out = []
+        t = now()
+        while (pi < len(pending) and pending[pi].arrival <= t and limit - len(out) > 0
+               and used + device_bytes(pending[pi]) <= params.budget):
+            r = pending[pi]
+            pi += 1
+            s = pool.pop() if pool else Solver(device)
+            admit(s, r)
+            out.append([r, s, r.steps])
+            used += device_bytes(r)
+        return out
+
+    staged = []
+    try:
+        while pi < len(pending) or active or staged:
+            active += staged
+            staged = []
+            if not active:
+                active = admit_due(params.max_batch)
+                if not active:
+                    time.sleep(max(0.0, pending[pi].arrival - (time.perf_counter() - t0)))
+                    continue
+            steps = min(params.chunk, min(a[2] for a in active))
+            batch.run([a[1] for a in active], steps, params.use_graph)
+            # arrivals are uploaded on their own streams while the batch runs
+            staged = admit_due(params.max_batch - len(active))
+            batch.wait()
+            done_t = time.perf_counter() - t0
+            still = []
+            for a in active:
+                a[2] -= steps
+                if a[2] == 0:
+                    r, s = a[0], a[1]
+                    s.run(0, 0)  # raises the instance's error, if any
+                    sol = s.get_sol() if keep else None
+                    results.append(Result(r.rid, done_t - r.arrival, done_t, sol))
+                    pool.append(s)
+                    used -= device_bytes(r)
+                else:
+                    still.append(a)
+            active = still
+    finally:
+        batch.close()
+        for s in pool:
+            s.close()
+        for a in active + staged:
+            a[1].close()
+    return results, time.perf_counter() - t0
+
+
+def serve_plain(reqs: list[Request], device: int = 0, keep: bool = False,
+                realtime: bool = True) -> tuple[list[Result], float]:
+    """One solve at a time in arrival order, each with its own launches."""
+    s = Solver(device)
+    results = []
+    t0 = time.perf_counter()
+    try:
+        for r in sorted(reqs, key=lambda q: q.arrival):
+            if realtime:
+                wait = r.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            admit(s, r)
+            s.run(r.steps, 0)
+            sol = s.get_sol() if keep else None
+            t = time.perf_counter() - t0
+            results.append(Result(r.rid, t - (r.arrival if realtime else 0.0), t, sol))
+    finally:
+        s.close()
+    return results, time.perf_counter() - t0
+
+
+def summarize(results: list[Result], makespan: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {"solves": len(results), "makespan_s": makespan, "solves_per_s": len(results) / makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+
+
+def autotune(sample: list[Request], device: int = 0, candidates=None) -> tuple[Params, list]:
+    """Times each candidate on the sample (all requests available at once)
+    and returns the fastest with the table of trials."""
+    if candidates is None:
+        candidates = [Params(chunk=c, max_batch=b, use_graph=g)
+                      for c in (15, 30, 60) for b in (64, 256) for g in (False, True)]
+    trials = []
+    for p in candidates:
+        _, ms = serve(sample, p, device, realtime=False)
+        trials.append({"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
+                       "solves_per_s": len(sample) / ms})
+    best = max(range(len(trials)), key=lambda i: trials[i]["solves_per_s"])
+    return candidates[best], trials

@@ -1,0 +1,39 @@
+"""Served batches: every request's final state, advanced in a batch of
+mixed-size instances that join and leave between chunks of steps, equals
+the state of the same request solved alone (tolerance zero: each
+instance's arithmetic is that of subfv_run)."""
+import numpy as np
+import pytest
+
+from subfv_amd import serve as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def requests():
+    reqs = S.make_requests(n_req=7, rate=1e6, steps=21, cmin=6 ** 2, cmax=22 ** 2)
+    reqs[2].steps = 12   # leaves early
+    reqs[5].steps = 33   # stays longer
+    return S.prepare(reqs, workers=4)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_results_equal_solves_alone(requests, graph):
+    print(f"[gpu] serve equality graph={graph}", flush=True)
+    alone, _ = S.serve_plain(requests, keep=True, realtime=False)
+    # capacity 3 and 6-step chunks: instances are admitted while others run
+    mixed, _ = S.serve(requests, S.Params(chunk=6, max_batch=3, use_graph=graph), keep=True, realtime=False)
+    a = {r.rid: r.sol for r in alone}
+    m = {r.rid: r.sol for r in mixed}
+    assert set(a) == set(m) == {r.rid for r in requests}
+    for rid in a:
+        assert np.array_equal(a[rid], m[rid]), rid
+
+
+def test_autotuner_picks_a_candidate(requests):
+    print("[gpu] serve autotune", flush=True)
+    best, trials = S.autotune(requests[:4], candidates=[S.Params(chunk=3, max_batch=2),
+                                                         S.Params(chunk=9, max_batch=4, use_graph=True)])
+    assert len(trials) == 2
+    assert best.chunk in (3, 9)

@@ -1,0 +1,36 @@
+"""The served-request stream (config-5 shape) on the CPU: seeded and
+deterministic, quad/tri about 50/50, cell counts log-uniform in
+[32^2, 512^2], states inside the stated ranges, Poisson arrivals at the
+requested rate; client-side preparation builds each connectivity."""
+import numpy as np
+
+from subfv_amd import serve as S
+
+
+def test_stream_is_seeded_and_in_range():
+    a = S.make_requests(2000, rate=50.0)
+    b = S.make_requests(2000, rate=50.0)
+    assert [(r.kind, r.n, r.arrival, r.x_split) for r in a] == [(r.kind, r.n, r.arrival, r.x_split) for r in b]
+    kinds = np.array([r.kind == "quad" for r in a])
+    assert 0.45 < kinds.mean() < 0.55
+    cells = np.array([r.n * r.n * (1 if r.kind == "quad" else 2) for r in a], float)
+    assert cells.min() >= 0.9 * 32 ** 2 and cells.max() <= 1.1 * 512 ** 2
+    lc = np.log(cells)
+    # log-uniform: the quartiles of log(cells) sit near the quartiles of the interval
+    lo, hi = np.log(32 ** 2), np.log(512 ** 2)
+    q = np.percentile(lc, [25, 50, 75])
+    assert np.allclose(q, lo + (hi - lo) * np.array([0.25, 0.5, 0.75]), atol=0.35)
+    for r in a:
+        for w in (r.wl, r.wr):
+            assert 0.1 <= w[0] <= 1.0 and 0.1 <= w[4] <= 1.0
+            assert np.hypot(w[1], w[2]) <= 0.5 + 1e-12
+        assert 0.3 <= r.x_split <= 0.7
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 50.0) < 0.1 / 50.0
+
+
+def test_prepare_builds_connectivity():
+    reqs = S.prepare(S.make_requests(6, rate=10.0, cmin=8 ** 2, cmax=20 ** 2), workers=3)
+    for r in reqs:
+        assert r.cells == r.n * r.n * (1 if r.kind == "quad" else 2)
+        assert r.host.geti("elem_vert").size > 0
