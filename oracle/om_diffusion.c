@@ -456,6 +456,14 @@ int om_add_diffusion(const om_mesh *m, const om_config *cfg, const double *sol, 
 
 int om_diffusion_available(void) { return 1; }
 
+/* test hook: temp_u with Cv_p */
+double om_test_temp_u(const double *u, double cv) {
+  om_config c;
+  memset(&c, 0, sizeof(c));
+  c.Cv_p = cv;
+  return temp_u(&c, u);
+}
+
 /* test hook: inverse of a column-major n x n matrix */
 void om_test_inv(int n, const double *A, double *out) {
   memcpy(out, A, sizeof(double) * n * n);

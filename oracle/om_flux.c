@@ -1032,3 +1032,8 @@ void om_test_lu_solve(int n, const double *A, const double *b, double *x) {
 void om_test_pinv3(double *A) { om_pinv3(A); }
 void om_test_lu_solve_noswp(int n, const double *A, const double *b, double *x) { om_lu_solve_noswp(n, A, x, b); }
 void om_test_pinv3_sym(double *A) { om_pinv3_sym(A, A); }
+void om_test_primitives(const double *u, double *w, double *u_back, double *a) {
+  conserv_to_primit(u, w);
+  primit_to_conserv(w, u_back);
+  *a = sound_speed_w(w);
+}
