@@ -11,6 +11,9 @@
  *    n*1e-12*max(S) dropped (linear_solver_module.F90:683-715), where the SVD
  *    is computed by one-sided (Hestenes) Jacobi rotations instead of dgesvd.
  *  - om_lu_solve_noswp: linear_solver_module.F90:413-445 verbatim order.
+ *  - both pseudo-inverses first try the cofactor inverse (om_inv3_regular):
+ *    when the determinant proves every singular value above the cut-off,
+ *    the pseudo-inverse is the inverse and no SVD is needed.
  */
 #ifndef OM_DENSE_H
 #define OM_DENSE_H
@@ -135,8 +138,42 @@ static inline void om_svd3(const double *a_in, double *s, double *u, double *v) 
   }
 }
 
+/* Cofactor inverse of the 3x3 matrix with rows (a b c), (d e f), (g h i),
+ * written column-major into P, when it is provably far from the
+ * pseudo-inverse cut-off; returns 0 (P untouched) otherwise.  With F the
+ * Frobenius norm, s_max <= F and s_min = |det|/(s_max s_mid) >= 2|det|/F^2,
+ * so |det| > 2e-12 F^3 gives s_min > 3.9e-12 s_max: all three singular values
+ * pass the 3e-12 * max cut-off of linear_solver_module.F90:703-711 with a
+ * margin far above the rounding of det (< 3e-15 F^3), and the pseudo-inverse
+ * equals the inverse. */
+static inline int om_inv3_regular(double a, double b, double c, double d, double e, double f, double g, double h,
+                                  double i, double F2, double *P) {
+  const double c00 = e * i - f * h, c01 = f * g - d * i, c02 = d * h - e * g;
+  const double det = a * c00 + b * c01 + c * c02;
+  if (!(fabs(det) > 2e-12 * (F2 * sqrt(F2)))) return 0;
+  const double r = 1.0 / det;
+  P[0] = c00 * r;
+  P[1] = c01 * r;
+  P[2] = c02 * r;
+  P[3] = (c * h - b * i) * r;
+  P[4] = (a * i - c * g) * r;
+  P[5] = (b * g - a * h) * r;
+  P[6] = (b * f - c * e) * r;
+  P[7] = (c * d - a * f) * r;
+  P[8] = (a * e - b * d) * r;
+  return 1;
+}
+
 /* In-place pseudo-inverse of a column-major 3x3 matrix. */
 static inline void om_pinv3(double *A) {
+  {
+    double F2 = 0.0, P[9];
+    for (int k = 0; k < 9; k++) F2 = F2 + A[k] * A[k];
+    if (om_inv3_regular(A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8], F2, P)) {
+      memcpy(A, P, sizeof(P));
+      return;
+    }
+  }
   double s[3], u[9], v[9];
   om_svd3(A, s, u, v);
   double tol = 3.0 * 1e-12 * s[0];
@@ -151,10 +188,16 @@ static inline void om_pinv3(double *A) {
 }
 
 /* Pseudo-inverse of a symmetric 3x3 matrix (the nodal matrices of the
- * velocity solvers): cyclic Jacobi rotations M = Q L Q^T, singular values
+ * velocity solvers): the cofactor inverse when om_inv3_regular accepts it,
+ * otherwise cyclic Jacobi rotations M = Q L Q^T, singular values
  * |L| sorted decreasingly, eigenvalues with |l| <= 3e-12 max|l| dropped;
  * P = sum_k q_k (1/l_k) q_k^T.  A column-major in, P column-major out. */
 static inline void om_pinv3_sym(const double *A, double *P) {
+  {
+    const double a00 = A[0], a01 = A[3], a02 = A[6], a11 = A[4], a12 = A[7], a22 = A[8];
+    const double F2 = ((a00 * a00 + a11 * a11) + a22 * a22) + 2.0 * ((a01 * a01 + a02 * a02) + a12 * a12);
+    if (om_inv3_regular(a00, a01, a02, a01, a11, a12, a02, a12, a22, F2, P)) return;
+  }
   double a[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) a[i][j] = A[i + 3 * j];

@@ -456,7 +456,7 @@ int ensure_diffusion(subfv_ctx *c) {
   size_t free_b = 0, total_b = 0;
   HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
   const size_t need = sizeof(double) * ((size_t)off[h.nv] + (size_t)kDiffSlot * h.nse + 29ull * h.ne);
-  if (need + (size_t)1 << 30 > free_b)
+  if (need + ((size_t)1 << 30) > free_b)
     return fail(c, SUBFV_E_DEVICE, "viscous terms: " + std::to_string(need >> 20) + " MiB needed, " +
                                        std::to_string(free_b >> 20) + " MiB free");
   int r;

@@ -34,7 +34,7 @@ struct DiffParams {
 };
 
 // packed per-vertex block: sizes for nf sub-faces and ns sub-elements
-SFV_DEV __host__ inline long diff_block_size(int nf, int ns) {
+SFV_DEV __host__ long diff_block_size(int nf, int ns) {
   const long F = 3 * nf, E = 3 * ns;
   return E * E + E + 9L * ns + F * E + F + (long)ns * ns + 2L * ns;
 }

@@ -382,10 +382,47 @@ SFV_DEV void lu_solve(double *A, const double *b, double *x) {
   for (int i = 0; i < N; i++) x[i] = w[i];
 }
 
+// Cofactor inverse of the 3x3 matrix with rows (a b c), (d e f), (g h i),
+// written column-major into P, when it is provably far from the
+// pseudo-inverse cut-off; false (P untouched) otherwise.  With F the
+// Frobenius norm (F2 = F^2), s_max <= F and s_min = |det| / (s_max s_mid) >=
+// 2 |det| / F^2, so |det| > 2e-12 F^3 gives s_min > 3.9e-12 s_max: every
+// singular value passes the 3e-12 * max cut-off of
+// linear_solver_module.F90:703-711 with a margin far above the rounding of
+// det (< 3e-15 F^3), and the pseudo-inverse is the inverse.
+SFV_DEV bool inv3_regular(double a, double b, double c, double d, double e, double f, double g, double h, double i,
+                          double F2, double *P) {
+  const double c00 = e * i - f * h, c01 = f * g - d * i, c02 = d * h - e * g;
+  const double det = a * c00 + b * c01 + c * c02;
+  if (!(fabs(det) > 2e-12 * (F2 * sqrt(F2)))) return false;
+  const double r = 1.0 / det;
+  P[0] = c00 * r;
+  P[1] = c01 * r;
+  P[2] = c02 * r;
+  P[3] = (c * h - b * i) * r;
+  P[4] = (a * i - c * g) * r;
+  P[5] = (b * g - a * h) * r;
+  P[6] = (b * f - c * e) * r;
+  P[7] = (c * d - a * f) * r;
+  P[8] = (a * e - b * d) * r;
+  return true;
+}
+
 // Pseudo-inverse V S^+ U^T of a column-major 3x3 matrix, singular values
-// below 3e-12 * max(S) dropped (linear_solver_module.F90:683-715); the SVD
-// is computed with one-sided Jacobi rotations, sorted by decreasing value.
+// below 3e-12 * max(S) dropped (linear_solver_module.F90:683-715): the
+// cofactor inverse when the determinant shows no value near the cut-off,
+// otherwise an SVD by one-sided Jacobi rotations, sorted by decreasing value.
 SFV_DEV void pinv3(double *A) {
+  {
+    double F2 = 0.0, P[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) F2 = F2 + A[k] * A[k];
+    if (inv3_regular(A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8], F2, P)) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) A[k] = P[k];
+      return;
+    }
+  }
   double a[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 #pragma unroll
   for (int i = 0; i < 9; i++) a[i] = A[i];
@@ -465,11 +502,17 @@ SFV_DEV void pinv3(double *A) {
 }
 
 // Pseudo-inverse of a symmetric 3x3 matrix (a00, a01, a02, a11, a12, a22),
-// written column-major into P: cyclic Jacobi rotations give M = Q L Q^T,
-// the singular values are |L|, and eigenvalues with |l| <= 3e-12 * max |l|
-// are dropped, the cut-off of linear_solver_module.F90:703-711.
+// written column-major into P: the cofactor inverse when the determinant
+// shows no eigenvalue near the cut-off; otherwise cyclic Jacobi rotations give
+// M = Q L Q^T, the singular values are |L|, and eigenvalues with
+// |l| <= 3e-12 * max |l| are dropped, the cut-off of
+// linear_solver_module.F90:703-711.
 SFV_DEV void pinv3_sym(const double *S6, double *P) {
   double a00 = S6[0], a01 = S6[1], a02 = S6[2], a11 = S6[3], a12 = S6[4], a22 = S6[5];
+  {
+    const double F2 = ((a00 * a00 + a11 * a11) + a22 * a22) + 2.0 * ((a01 * a01 + a02 * a02) + a12 * a12);
+    if (inv3_regular(a00, a01, a02, a01, a11, a12, a02, a12, a22, F2, P)) return;
+  }
   double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   for (int sweep = 0; sweep < 16; sweep++) {
     bool rotated = false;

@@ -89,6 +89,34 @@ def test_pinv3_sym_matches_numpy_including_cutoff():
         assert np.allclose(got, ref, rtol=rtol, atol=rtol * np.abs(ref).max()), (k, lam)
 
 
+@pytest.mark.parametrize("sym", [False, True])
+def test_pinv3_across_the_cofactor_threshold(sym):
+    """Smallest singular value swept through the region where the pseudo-
+    inverses switch between the cofactor inverse and the Jacobi SVD
+    (|det| ~ 2e-12 F^3) and through the 3e-12 cut-off itself: the result
+    keeps or drops the value exactly as the SVD definition does."""
+    fn = O.lib().om_test_pinv3_sym if sym else O.lib().om_test_pinv3
+    fn.argtypes = [np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+    rng = np.random.default_rng(7 + sym)
+    for r in np.geomspace(1e-13, 1e-9, 41):
+        if abs(r / 3e-12 - 1.0) < 0.05:
+            continue  # too close to the cut-off for the rounding of either SVD
+        Q1, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        Q2 = Q1 if sym else np.linalg.qr(rng.standard_normal((3, 3)))[0]
+        s = np.array([1.0, 0.3 + 0.7 * rng.random(), r]) * 10.0 ** rng.uniform(-8, 2)
+        A = Q1 @ np.diag(s) @ Q2.T
+        if sym:
+            A = 0.5 * (A + A.T)
+        ref = np.linalg.pinv(A, rcond=3e-12, hermitian=sym)
+        cm = np.ascontiguousarray(A.T).copy()
+        fn(cm)
+        got = cm.reshape(3, 3).T
+        kept = s[s > 3e-12 * s.max()]
+        rtol = 50 * 2.2e-16 * kept.max() / kept.min()
+        assert np.allclose(got, ref, rtol=rtol, atol=rtol * np.abs(ref).max()), (r, s)
+        assert (np.abs(got).max() * s.max() > 1e8) == (r > 3e-12)
+
+
 def test_lu_solve_noswp_matches_numpy():
     fn = O.lib().om_test_lu_solve_noswp
     dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
