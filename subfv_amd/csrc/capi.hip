@@ -467,7 +467,7 @@ int ensure_diffusion(subfv_ctx *c) {
   if (!c->diag && (r = dalloc(c, &c->diag, 25ull * h.ne))) return r;
   HIPCHK(c, hipMemsetAsync(c->diag, 0, sizeof(double) * 25 * h.ne, c->stream));
   HIPCHK(c, hipMemsetAsync(c->rdiff, 0, sizeof(double) * 4 * h.ne, c->stream));
-  DiffArgs a;
+  DiffArgs a{};
   a.m = c->dm;
   a.p = c->dparams;
   a.doff = c->doff;
@@ -487,7 +487,7 @@ int ensure_diffusion(subfv_ctx *c) {
 int launch_diffusion(subfv_ctx *c, const double *U) {
   int r = ensure_diffusion(c);
   if (r) return r;
-  DiffArgs a;
+  DiffArgs a{};
   a.m = c->dm;
   a.p = c->dparams;
   a.doff = c->doff;

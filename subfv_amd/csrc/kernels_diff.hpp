@@ -380,7 +380,11 @@ constexpr int kDiffWaves = 4;
 
 struct DiffApplyLds {
   double Ve[kDiffMaxF], Vf[kDiffMaxF], SVe[kDiffMaxF];
-  double th[kDiffMaxSub], P[kDiffMaxSub][5], rho[kDiffMaxSub];
+  double th[kDiffMaxSub], rho[kDiffMaxSub], P[kDiffMaxSub][5], vol[kDiffMaxSub], pu[kDiffMaxSub];
+  double nrm[kDiffMaxSub][4];  // unit normal and area of each sub-face of the vertex
+  int le[kDiffMaxSub];         // left cell of each sub-face
+  int cell[kDiffMaxSub];       // cell of each slot
+  int ssf[kDiffMaxSub][kSlotSrc + 1];  // local sub-faces of each slot's sub-element, count first
 };
 
 // One wavefront per vertex: the scatter terms of
@@ -388,7 +392,15 @@ struct DiffApplyLds {
 // build_diff_mat_into_global_mat_diag (:139-182) per sub-element slot.
 // Row of PD: [0..2] rhs(2:4), [3] rhs(5), [4..12] diag(2:4, 2:4) column-major,
 // [13..17] diag(5, 1:5).
-__global__ __launch_bounds__(64 * kDiffWaves) void k_diff_apply(DiffArgs a) {
+//
+// (a) lanes < ns load their slot's cell state and sub-element sub-face list,
+//     lanes < nf their sub-face's geometry, all into LDS, in parallel;
+// (b) lane r < F + E owns one row of the two column-major streams N^-1 S
+//     (F x E) and S~^T N^-1 S (E x E): Vf = N^-1 S Ve + N^-1 B and
+//     SVe = S~^T N^-1 S Ve, coalesced;
+// (c) the slot terms, spread over the lanes by (slot, entry), read only LDS
+//     and the vertex's block the wavefront has just streamed.
+__global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
   __shared__ DiffApplyLds lds_all[kDiffWaves];
   const DevMesh &m = a.m;
   const int wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -402,79 +414,102 @@ __global__ __launch_bounds__(64 * kDiffWaves) void k_diff_apply(DiffArgs a) {
   const double *pre = a.pre + a.doff[v];
   const double *sns = pre, *sb = sns + (long)E * E, *blcs = sb + E, *ninvs = blcs + 9L * ns;
   const double *ninvb = ninvs + (long)F * E, *snsh = ninvb + F, *sbh = snsh + (long)ns * ns, *b1 = sbh + ns;
-  if (lane < ns) {
-    const int c = m.se_cell[se0 + lane];
-    double u[5];
-    for (int k = 0; k < 5; k++) u[k] = a.U[5L * c + k];
-    for (int k = 0; k < 3; k++) L.Ve[3 * lane + k] = u[k + 1] / u[0];
-    L.rho[lane] = u[0];
-    L.th[lane] = temp_u(u, cv);
-    dtheta_du(u, cv, L.P[lane]);
-  }
-  wsync_d();
-  if (lane < F) {
-    double acc = 0.0;
-    for (int j = 0; j < E; j++) acc = acc + L.Ve[j] * ninvs[lane + (long)F * j];
-    L.Vf[lane] = acc + ninvb[lane];
-  }
-  if (lane < E) {
-    double acc = 0.0;
-    for (int j = 0; j < E; j++) acc = acc + L.Ve[j] * sns[lane + (long)E * j];
-    L.SVe[lane] = acc;
-  }
-  wsync_d();
+  // ---- (a)
   if (lane < ns) {
     const int l = lane;
-    const int cell = m.se_cell[se0 + l];
+    const int c = m.se_cell[se0 + l];
     const long se = m.v_seid[se0 + l];
+    double u[5];
+    for (int k = 0; k < 5; k++) u[k] = a.U[5L * c + k];
+    const int pb = m.se_sfoff[se], pe = m.se_sfoff[se + 1];
+    L.ssf[l][0] = pe - pb;  // at most kSlotSrc (checked when the groups are built)
+    for (int pp = pb; pp < pe; pp++) L.ssf[l][1 + pp - pb] = m.se_sf[pp] - sf0;
+    L.vol[l] = m.se_vol[se];
+    for (int k = 0; k < 3; k++) L.Ve[3 * l + k] = u[k + 1] / u[0];
+    L.rho[l] = u[0];
+    L.cell[l] = c;
+    L.th[l] = temp_u(u, cv);
+    double P[5];
+    dtheta_du(u, cv, P);
+    double pu = 0.0;
+    for (int k = 0; k < 5; k++) pu = pu + P[k] * u[k];
+    for (int k = 0; k < 5; k++) L.P[l][k] = P[k];
+    L.pu[l] = pu;
+  }
+  if (lane < nf) {
+    const long s = sf0 + lane;
+    for (int k = 0; k < 4; k++) L.nrm[lane][k] = m.sfn[k * m.SF + s];
+    L.le[lane] = m.sf_le[s];
+  }
+  wsync_d();
+  // ---- (b)
+  for (int r = lane; r < F + E; r += 64) {
+    const bool vf = r < F;
+    const double *col = vf ? ninvs + r : sns + (r - F);
+    const long ld = vf ? F : E;
+    double acc = 0.0;
+#pragma unroll 6
+    for (int j = 0; j < E; j++) acc = acc + L.Ve[j] * col[ld * j];
+    if (vf)
+      L.Vf[r] = acc + ninvb[r];
+    else
+      L.SVe[r - F] = acc;
+  }
+  wsync_d();
+  // ---- (c) diagonal block of the momentum rows, 9 entries per slot
+  for (int t = lane; t < 9 * ns; t += 64) {
+    const int l = t / 9, ij = t - 9 * l, i = ij % 3, j = ij / 3;
     const double rho = L.rho[l];
-    double out[kDiffSlot];
-    // momentum rhs and diagonal block
-    for (int i = 0; i < 3; i++) {
-      double mv = 0.0;
-      for (int j = 0; j < 3; j++) mv = mv + sns[(3 * l + i) + (long)E * (3 * l + j)] * L.Ve[3 * l + j];
-      out[i] = (L.SVe[3 * l + i] - mv) + sb[3 * l + i];
-    }
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++)
-        out[4 + i + 3 * j] = -sns[(3 * l + i) + (long)E * (3 * l + j)] / rho + blcs[9 * l + j + 3 * i] / rho;
-    // viscous work of the sub-element
+    a.PD[(long)kDiffSlot * (se0 + l) + 4 + ij] =
+        -sns[(3 * l + i) + (long)E * (3 * l + j)] / rho + blcs[9 * l + j + 3 * i] / rho;
+  }
+  // momentum rhs, 3 per slot
+  for (int t = lane; t < 3 * ns; t += 64) {
+    const int l = t / 3, i = t - 3 * l;
+    double mv = 0.0;
+    for (int j = 0; j < 3; j++) mv = mv + sns[(3 * l + i) + (long)E * (3 * l + j)] * L.Ve[3 * l + j];
+    a.PD[(long)kDiffSlot * (se0 + l) + i] = (L.SVe[3 * l + i] - mv) + sb[3 * l + i];
+  }
+  // heat row of the diagonal, 5 per slot
+  for (int t = lane; t < 5 * ns; t += 64) {
+    const int l = t / 5, k = t - 5 * l;
+    const double sll = snsh[l + (long)ns * l];
+    a.PD[(long)kDiffSlot * (se0 + l) + 13 + k] = (b1[l] - sll) * L.P[l][k];
+  }
+  // viscous work and heat rhs, one lane per slot
+  if (lane < ns) {
+    const int l = lane, cell = L.cell[l], cnt = L.ssf[l][0];
     double Spc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
-      const long s = m.se_sf[pp];
-      const int lo = (int)(s - sf0);
+    for (int e = 0; e < cnt; e++) {
+      const int lo = L.ssf[l][1 + e];
+      const double sg = L.le[lo] == cell ? 1.0 : -1.0;
       double nl[3], vm[3];
-      sf_normal(m, s, m.sf_le[s] == cell ? 1.0 : -1.0, nl);
+      for (int d = 0; d < 3; d++) nl[d] = sg * L.nrm[lo][d];
       for (int d = 0; d < 3; d++) vm[d] = L.Vf[3 * lo + d] - L.Ve[3 * l + d];
       const double dv = vm[0] * nl[0] + vm[1] * nl[1] + vm[2] * nl[2];
-      const double ar = m.sfn[3 * m.SF + s];
+      const double ar = L.nrm[lo][3];
       for (int j = 0; j < 3; j++)
         for (int i = 0; i < 3; i++)
           Spc[i + 3 * j] = Spc[i + 3 * j] + ar * (vm[i] * nl[j] + vm[j] * nl[i] - (2.0 / 3.0) * dv * (i == j ? 1.0 : 0.0));
     }
-    for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * a.p.mu / m.se_vol[se];
+    for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * a.p.mu / L.vol[l];
     double work = 0.0;
-    for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
-      const long s = m.se_sf[pp];
-      const int lo = (int)(s - sf0);
+    for (int e = 0; e < cnt; e++) {
+      const int lo = L.ssf[l][1 + e];
+      const double sg = L.le[lo] == cell ? 1.0 : -1.0;
       double nl[3];
-      sf_normal(m, s, m.sf_le[s] == cell ? 1.0 : -1.0, nl);
+      for (int d = 0; d < 3; d++) nl[d] = sg * L.nrm[lo][d];
       double sv[3];
       for (int i = 0; i < 3; i++)
         sv[i] = Spc[i] * L.Vf[3 * lo] + Spc[i + 3] * L.Vf[3 * lo + 1] + Spc[i + 6] * L.Vf[3 * lo + 2];
-      work = work + m.sfn[3 * m.SF + s] * (sv[0] * nl[0] + sv[1] * nl[1] + sv[2] * nl[2]);
+      work = work + L.nrm[lo][3] * (sv[0] * nl[0] + sv[1] * nl[1] + sv[2] * nl[2]);
     }
-    // heat
     double h = 0.0;
     for (int p = 0; p < ns; p++) h = h + snsh[l + (long)ns * p] * L.th[p];
     const double sll = snsh[l + (long)ns * l];
-    double pu = 0.0;
-    for (int k = 0; k < 5; k++) pu = pu + L.P[l][k] * a.U[5L * cell + k];
+    const double pu = L.pu[l];
     h = h - sll * pu + (b1[l] - sll) * pu - b1[l] * L.th[l] + sbh[l];
-    out[3] = work + h;
-    for (int k = 0; k < 5; k++) out[13 + k] = (b1[l] - sll) * L.P[l][k];
-    double *o = a.PD + (long)kDiffSlot * (se0 + l);
-    for (int k = 0; k < kDiffSlot; k++) o[k] = out[k];
+    a.PD[(long)kDiffSlot * (se0 + l) + 3] = work + h;
   }
 }
 

@@ -325,15 +325,18 @@ SFV_DEV void lu_solve(double *A, const double *b, double *x) {
         p = i;
       }
     piv[k] = p;
-    if (A[p + N * k] != 0.0) {
-      if (p != k) {
+    if (amax != 0.0) {  // amax = |A(p, k)|
+      // row interchange with static indices only (no scratch for A)
 #pragma unroll
-        for (int j = 0; j < N; j++) {
-          double t = A[k + N * j];
-          A[k + N * j] = A[p + N * j];
-          A[p + N * j] = t;
+      for (int i = k + 1; i < N; i++)
+        if (i == p) {
+#pragma unroll
+          for (int j = 0; j < N; j++) {
+            double t = A[k + N * j];
+            A[k + N * j] = A[i + N * j];
+            A[i + N * j] = t;
+          }
         }
-      }
       double pv = A[k + N * k];
       if (fabs(pv) >= 2.2250738585072014e-308) {
         double r = 1.0 / pv;
