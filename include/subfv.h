@@ -26,8 +26,11 @@
  *   subfv_set_diffusion         activate_diffusion and the viscous namelist entries
  *                               (src/ns/ns_global_data_module.F90:8-12,177-213)
  *   subfv_run                   the time loop of src/ns/ns_main.F90:83-107
- *   subfv_bcsr_*                bcsr_matrix_type, bcsr_matmul, bcsr_diag, bcsr_gmres_diag
- *                               (src/linear_algebra/sparse_bcsr_linear_module.F90:6-29,1414-1436,1592-1691)
+ *   subfv_bcsr_*                bcsr_matrix_type, bcsr_matmul, bcsr_diag, bcsr_gmres_diag, bcsr_gmres,
+ *                               bcsr_gmres_lusgs, bcsr_gmres_parallel_lusgs, bcsr_lusgs,
+ *                               bcsr_parallel_lusgs, bcsr_pcgm, add_block_to_bcsr
+ *                               (src/linear_algebra/sparse_bcsr_linear_module.F90:6-29,56-71,194-290,
+ *                               447-548,885-1037,1110-1169,1268-1335,1414-1502,1592-1691)
  */
 #ifndef SUBFV_H
 #define SUBFV_H
@@ -200,6 +203,37 @@ int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs);
  * stats[2] = SpMV count. Returns SUBFV_E_NOT_CONVERGED (soft) if not conv. */
 int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                           int32_t precision, int32_t *conv, double *stats);
+/* The other GMRES loops on the same interface: precond 0 = bcsr_gmres
+ * (:194-290, no preconditioner, 20 restarts), 1 = bcsr_gmres_diag (as above),
+ * 2 = bcsr_gmres_lusgs (:447-548, bcsr_lusgs inside every Arnoldi step and on
+ * the combination, 20 restarts), 3 = bcsr_gmres_parallel_lusgs (:885-1037,
+ * with the colour sets of subfv_bcsr_set_colors).  precision 1 (FP32 inner)
+ * is accepted for precond 0 and 1 only. */
+int subfv_bcsr_gmres(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart, int32_t precond,
+                     int32_t precision, int32_t *conv, double *stats);
+/* bcsr_lusgs (:1268-1335) as the reference writes it: (L + I) y = rhs, then
+ * (L + D) x = y with D^-1 by lu_solve_noswp, L the blocks j = row(i), ...
+ * while col(j) < i.  The device runs the rows of each dependency level of L
+ * together (the level schedule is built by subfv_bcsr_create). */
+int subfv_bcsr_lusgs(subfv_bcsr *A, double *x, const double *rhs);
+/* colour sets of bcsr_parallel_lusgs (color_set_type): color[i] in 1..n_sets
+ * per block row, or NULL for a greedy colouring of the symmetric closure of the
+ * block pattern (rows in order, smallest free colour).  *n_sets receives the
+ * number of sets. */
+int subfv_bcsr_set_colors(subfv_bcsr *A, const int32_t *color, int32_t *n_sets);
+/* bcsr_parallel_lusgs (:1438-1502): forward over the colour sets with the
+ * blocks of lower colour, backward with the blocks of higher colour and
+ * D^-1 by lu_solve_noswp. */
+int subfv_bcsr_parallel_lusgs(subfv_bcsr *A, double *x, const double *rhs);
+/* bcsr_pcgm (:1110-1169): preconditioned CG from the given x (in/out) with
+ * lu_solve(3, ...) on the first nine values of each diagonal block, as the
+ * reference hard-codes.  stats[0] = iterations, stats[1] = sqrt(M^-1 r . r).
+ * SUBFV_E_NOT_CONVERGED where the reference stops with "did not converge",
+ * SUBFV_E_INVALID where lu_solve stops on a singular block. */
+int subfv_bcsr_pcgm(subfv_bcsr *A, double *x, const double *rhs, double tol, double *stats);
+/* add_block_to_bcsr (:56-71): v(:,:,k) += M (b x b column-major) for the first
+ * stored block (i, j), 1-based; nothing if the block is not stored. */
+int subfv_bcsr_add_block(subfv_bcsr *A, int32_t i, int32_t j, const double *M);
 
 /* ------------------------------------------------------- served batches */
 /* No counterpart in the reference (it runs one solve per process): a batch

@@ -252,6 +252,46 @@ static inline void om_pinv3_sym(const double *A, double *P) {
     }
 }
 
+/* lu_solve, linear_solver_module.F90:447-484: partial pivoting by the first
+ * largest |C(i,k)| (rows always swapped), multipliers C(i,k) * (1/C(k,k)),
+ * right-hand side eliminated with the factorisation, back substitution over
+ * j increasing.  Returns 1 where the reference stops (|C(n,n)| < 1e-12). */
+static inline int om_lu_solve_plain(int n, const double *A, double *x, const double *b) {
+  double C[64];
+  for (int q = 0; q < n * n; q++) C[q] = A[q];
+  for (int i = 0; i < n; i++) x[i] = b[i];
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double amax = fabs(C[k + n * k]);
+    for (int i = k + 1; i < n; i++)
+      if (fabs(C[i + n * k]) > amax) {
+        amax = fabs(C[i + n * k]);
+        p = i;
+      }
+    for (int j = 0; j < n; j++) {
+      double t = C[k + n * j];
+      C[k + n * j] = C[p + n * j];
+      C[p + n * j] = t;
+    }
+    double t = x[k];
+    x[k] = x[p];
+    x[p] = t;
+    double invdiag = 1.0 / C[k + n * k];
+    for (int i = k + 1; i < n; i++) {
+      double lik = C[i + n * k] * invdiag;
+      for (int j = k; j < n; j++) C[i + n * j] = C[i + n * j] - lik * C[k + n * j];
+      x[i] = x[i] - lik * x[k];
+    }
+  }
+  if (fabs(C[(n - 1) + n * (n - 1)]) < 1e-12) return 1;
+  x[n - 1] = x[n - 1] / C[(n - 1) + n * (n - 1)];
+  for (int k = n - 2; k >= 0; k--) {
+    for (int j = k + 1; j < n; j++) x[k] = x[k] - C[k + n * j] * x[j];
+    x[k] = x[k] / C[k + n * k];
+  }
+  return 0;
+}
+
 /* lu_solve_noswp, linear_solver_module.F90:413-445 (no pivoting) */
 static inline void om_lu_solve_noswp(int n, const double *A, double *x, const double *b) {
   double C[64];

@@ -139,6 +139,12 @@ def lib():
         L.om_bcsr_matmul.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp]
         L.om_bcsr_diag.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp]
         L.om_bcsr_gmres_diag.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.om_bcsr_lusgs.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp]
+        L.om_bcsr_parallel_lusgs.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_int, ip, ip, ip]
+        L.om_bcsr_add_block.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.c_int, dp]
+        L.om_bcsr_gmres_kind.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int,
+                                         C.c_int, ip, ip, ip, dp]
+        L.om_bcsr_pcgm.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, dp]
         L.om_add_diffusion.argtypes = [P, cp, dp, dp, dp]
         L.om_test_inv.argtypes = [C.c_int, dp, dp]
         _lib = L
@@ -317,6 +323,67 @@ def bcsr_gmres_diag(m, b, row, col, v, rhs, tol, restart=30):
                                     np.ascontiguousarray(v, np.float64).ravel(), x,
                                     np.ascontiguousarray(rhs, np.float64), tol, restart, info)
     return x, bool(conv), dict(restarts=int(info[0]), rho=info[1])
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64).ravel()
+
+
+def color_sets(color):
+    """(nsets, set_off, set_rows) from a 1-based colour per row: the rows of
+    set cs in increasing order."""
+    color = np.asarray(color, np.int32)
+    nsets = int(color.max())
+    rows = np.argsort(color, kind="stable").astype(np.int32) + 1
+    off = np.concatenate([[0], np.cumsum(np.bincount(color, minlength=nsets + 1)[1:])]).astype(np.int32)
+    return nsets, off, rows
+
+
+def bcsr_lusgs(m, b, row, col, v, rhs):
+    x = np.zeros(b * m)
+    lib().om_bcsr_lusgs(m, b, _i32(row), _i32(col), _f64(v), x, _f64(rhs))
+    return x
+
+
+def bcsr_parallel_lusgs(m, b, row, col, v, rhs, color):
+    x = np.zeros(b * m)
+    nsets, off, rows = color_sets(color)
+    lib().om_bcsr_parallel_lusgs(m, b, _i32(row), _i32(col), _f64(v), x, _f64(rhs), nsets, off, rows, _i32(color))
+    return x
+
+
+def bcsr_add_block(m, b, row, col, v, i, j, M):
+    """add_block_to_bcsr on a copy of v; returns the new values."""
+    out = _f64(v).copy()
+    lib().om_bcsr_add_block(m, b, _i32(row), _i32(col), out, i, j, _f64(np.asarray(M).T))
+    return out
+
+
+def bcsr_gmres(m, b, row, col, v, rhs, tol, restart=30, precond="none", color=None):
+    """bcsr_gmres (precond "none"), bcsr_gmres_lusgs ("lusgs") and
+    bcsr_gmres_parallel_lusgs ("parallel_lusgs", with a 1-based colour per row)."""
+    kind = {"none": 0, "lusgs": 2, "parallel_lusgs": 3}[precond]
+    x = np.zeros(b * m)
+    info = np.zeros(2)
+    if kind == 3:
+        nsets, off, rows = color_sets(color)
+        col_arr = _i32(color)
+    else:
+        nsets, off, rows, col_arr = 0, np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1, np.int32)
+    conv = lib().om_bcsr_gmres_kind(kind, m, b, _i32(row), _i32(col), _f64(v), x, _f64(rhs), tol, restart,
+                                    nsets, off, rows, col_arr, info)
+    return x, bool(conv), dict(restarts=int(info[0]), rho=info[1])
+
+
+def bcsr_pcgm(m, b, row, col, v, rhs, tol, x0=None):
+    x = np.zeros(b * m) if x0 is None else _f64(x0).copy()
+    info = np.zeros(2)
+    conv = lib().om_bcsr_pcgm(m, b, _i32(row), _i32(col), _f64(v), x, _f64(rhs), tol, info)
+    return x, int(conv), dict(iterations=int(info[0]), rho=info[1])
 
 
 def inv(A: np.ndarray) -> np.ndarray:

@@ -128,6 +128,12 @@ SIGNATURES = {
     "subfv_bcsr_matmul": (C.c_int, [_P, _dp, _dp]),
     "subfv_bcsr_diag": (C.c_int, [_P, _dp, _dp]),
     "subfv_bcsr_gmres_diag": (C.c_int, [_P, _dp, _dp, C.c_double, _i32, _i32, _ip, _dp]),
+    "subfv_bcsr_gmres": (C.c_int, [_P, _dp, _dp, C.c_double, _i32, _i32, _i32, _ip, _dp]),
+    "subfv_bcsr_lusgs": (C.c_int, [_P, _dp, _dp]),
+    "subfv_bcsr_set_colors": (C.c_int, [_P, _ip, _ip]),
+    "subfv_bcsr_parallel_lusgs": (C.c_int, [_P, _dp, _dp]),
+    "subfv_bcsr_pcgm": (C.c_int, [_P, _dp, _dp, C.c_double, _dp]),
+    "subfv_bcsr_add_block": (C.c_int, [_P, _i32, _i32, _dp]),
 }
 
 _lib = None

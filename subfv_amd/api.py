@@ -303,6 +303,68 @@ class Bcsr:
             self.solver._check(st)
         return x, bool(conv.value), {"restarts": int(stats[0]), "rho": float(stats[1]), "spmv": int(stats[2])}
 
+    PRECONDITIONERS = {"none": 0, "diag": 1, "lusgs": 2, "parallel_lusgs": 3}
+
+    def gmres(self, rhs, tol: float, restart: int = 30, precond: str = "none", precision: int = 0):
+        """bcsr_gmres ("none"), bcsr_gmres_diag ("diag"), bcsr_gmres_lusgs
+        ("lusgs") or bcsr_gmres_parallel_lusgs ("parallel_lusgs", colour sets
+        from set_colors) from x = 0.  Returns (x, conv, stats) as gmres_diag."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n)
+        conv = C.c_int32(0)
+        stats = np.zeros(3)
+        st = _lib.lib().subfv_bcsr_gmres(self.h, _lib.dptr(x), _lib.dptr(rhs), tol, restart,
+                                         self.PRECONDITIONERS[precond], precision, C.byref(conv), _lib.dptr(stats))
+        if st not in (0, _lib.E_NOT_CONVERGED):
+            self.solver._check(st)
+        return x, bool(conv.value), {"restarts": int(stats[0]), "rho": float(stats[1]), "spmv": int(stats[2])}
+
+    def lusgs(self, rhs):
+        """bcsr_lusgs: (L + I) y = rhs, then (L + D) x = y."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n)
+        self.solver._check(_lib.lib().subfv_bcsr_lusgs(self.h, _lib.dptr(x), _lib.dptr(rhs)))
+        return x
+
+    def set_colors(self, color=None) -> int:
+        """Colour sets of bcsr_parallel_lusgs: a 1-based colour per block row,
+        or None for the library's greedy colouring.  Returns the number of sets."""
+        n = C.c_int32(0)
+        if color is None:
+            st = _lib.lib().subfv_bcsr_set_colors(self.h, None, C.byref(n))
+        else:
+            self._color = _lib.i32(color)
+            if self._color.size != self.m:
+                raise ValueError("one colour per block row")
+            st = _lib.lib().subfv_bcsr_set_colors(self.h, _lib.iptr(self._color), C.byref(n))
+        self.solver._check(st)
+        return int(n.value)
+
+    def parallel_lusgs(self, rhs):
+        """bcsr_parallel_lusgs with the colour sets of set_colors."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n)
+        self.solver._check(_lib.lib().subfv_bcsr_parallel_lusgs(self.h, _lib.dptr(x), _lib.dptr(rhs)))
+        return x
+
+    def pcgm(self, rhs, tol: float, x0=None):
+        """bcsr_pcgm from x0 (zero by default).  Returns (x, conv, stats);
+        a singular diagonal block raises SubfvError."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n) if x0 is None else self._vec(x0).copy()
+        stats = np.zeros(2)
+        st = _lib.lib().subfv_bcsr_pcgm(self.h, _lib.dptr(x), _lib.dptr(rhs), tol, _lib.dptr(stats))
+        if st not in (0, _lib.E_NOT_CONVERGED):
+            self.solver._check(st)
+        return x, st == 0, {"iterations": int(stats[0]), "rho": float(stats[1])}
+
+    def add_block(self, i: int, j: int, M):
+        """add_block_to_bcsr: block (i, j) (1-based) += M (b x b)."""
+        M = np.ascontiguousarray(np.asarray(M, np.float64).T).ravel()
+        if M.size != self.b * self.b:
+            raise ValueError("M must be b x b")
+        self.solver._check(_lib.lib().subfv_bcsr_add_block(self.h, int(i), int(j), _lib.dptr(M)))
+
     def close(self):
         if getattr(self, "h", None):
             _lib.lib().subfv_bcsr_destroy(self.h)

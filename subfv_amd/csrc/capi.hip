@@ -1059,6 +1059,18 @@ struct subfv_bcsr {
   double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
   double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr;
   unsigned int *ticket = nullptr;
+  // host copies (0-based) for add_block and the sweep schedules
+  std::vector<int> h_row, h_col, h_diag;
+  // bcsr_lusgs: rows grouped by dependency level of the lower triangle
+  std::vector<int> lvl_off;
+  int *lvl_rows = nullptr;
+  // bcsr_parallel_lusgs: colour sets
+  int nsets = 0;
+  std::vector<int> set_off;
+  int *set_rows = nullptr, *color = nullptr;
+  // bcsr_pcgm
+  double *p = nullptr, *pmr = nullptr;
+  int *perr = nullptr;
   std::vector<void *> allocs;
 };
 
@@ -1149,6 +1161,71 @@ int ensure_workspace(subfv_bcsr *A, int mr) {
   return 0;
 }
 
+
+// bcsr_lusgs: out = (L + D)^-1 (L + I)^-1 in, level by level (y in A->w)
+template <int B>
+void lusgs_b(subfv_bcsr *A, const double *in, double *out) {
+  hipStream_t s = A->ctx->stream;
+  const int nlev = (int)A->lvl_off.size() - 1;
+  for (int l = 0; l < nlev; l++) {
+    const int o = A->lvl_off[l], cnt = A->lvl_off[l + 1] - o;
+    hipLaunchKernelGGL((k_lusgs_level<B, false>), dim3(nblk(cnt, 256)), dim3(256), 0, s, cnt, A->lvl_rows + o, A->row,
+                       A->col, A->v, (const double *)A->F64, in, A->w);
+  }
+  for (int l = 0; l < nlev; l++) {
+    const int o = A->lvl_off[l], cnt = A->lvl_off[l + 1] - o;
+    hipLaunchKernelGGL((k_lusgs_level<B, true>), dim3(nblk(cnt, 256)), dim3(256), 0, s, cnt, A->lvl_rows + o, A->row,
+                       A->col, A->v, (const double *)A->F64, (const double *)A->w, out);
+  }
+}
+
+// bcsr_parallel_lusgs: forward over the colour sets, then backward
+template <int B>
+void plusgs_b(subfv_bcsr *A, const double *in, double *out) {
+  hipStream_t s = A->ctx->stream;
+  for (int cs = 1; cs <= A->nsets; cs++) {
+    const int o = A->set_off[cs - 1], cnt = A->set_off[cs] - o;
+    if (cnt)
+      hipLaunchKernelGGL((k_plusgs_set<B, false>), dim3(nblk(cnt, 256)), dim3(256), 0, s, cnt, A->set_rows + o, cs,
+                         A->color, A->row, A->col, A->v, (const double *)A->F64, in, A->w);
+  }
+  for (int cs = A->nsets; cs >= 1; cs--) {
+    const int o = A->set_off[cs - 1], cnt = A->set_off[cs] - o;
+    if (cnt)
+      hipLaunchKernelGGL((k_plusgs_set<B, true>), dim3(nblk(cnt, 256)), dim3(256), 0, s, cnt, A->set_rows + o, cs,
+                         A->color, A->row, A->col, A->v, (const double *)A->F64, (const double *)A->w, out);
+  }
+}
+
+int lusgs(subfv_bcsr *A, const double *in, double *out) {
+  switch (A->b) {
+    case 3: lusgs_b<3>(A, in, out); break;
+    case 4: lusgs_b<4>(A, in, out); break;
+    default: lusgs_b<5>(A, in, out);
+  }
+  HIPCHK(A->ctx, hipGetLastError());
+  return 0;
+}
+
+int plusgs(subfv_bcsr *A, const double *in, double *out) {
+  if (A->nsets <= 0) return fail(A->ctx, SUBFV_E_INVALID, "no colour sets: call subfv_bcsr_set_colors first");
+  switch (A->b) {
+    case 3: plusgs_b<3>(A, in, out); break;
+    case 4: plusgs_b<4>(A, in, out); break;
+    default: plusgs_b<5>(A, in, out);
+  }
+  HIPCHK(A->ctx, hipGetLastError());
+  return 0;
+}
+
+// preconditioners of the GMRES variants: 0 none, 1 block-Jacobi, 2 LU-SGS,
+// 3 coloured LU-SGS
+int precond(subfv_bcsr *A, int kind, bool fp32, const double *in, double *out) {
+  if (kind == 1) return bjacobi(A, fp32, in, out);
+  if (kind == 2) return lusgs(A, in, out);
+  return plusgs(A, in, out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1169,10 +1246,28 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
     }
     if (di[i] < 0) return fail(c, SUBFV_E_INVALID, "block row without a diagonal block");
   }
+  // dependency levels of bcsr_lusgs's sweeps: row i waits for the rows
+  // col(j) < i met by j = row(i), ... while col(j) < i
+  std::vector<int> lvl(m, 0);
+  int nlev = 0;
+  for (int i = 0; i < m; i++) {
+    int l = 0;
+    for (int j = r0[i]; j < r0[i + 1] && c0[j] < i; j++) l = std::max(l, lvl[c0[j]] + 1);
+    lvl[i] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  std::vector<int> loff(nlev + 1, 0), lrows(m);
+  for (int i = 0; i < m; i++) loff[lvl[i] + 1]++;
+  for (int l = 0; l < nlev; l++) loff[l + 1] += loff[l];
+  {
+    std::vector<int> fill(loff.begin(), loff.end() - 1);
+    for (int i = 0; i < m; i++) lrows[fill[lvl[i]]++] = i;
+  }
   HIPCHK(c, hipSetDevice(c->device));
   auto *A = new subfv_bcsr();
   A->ctx = c;
   A->m = m;
+  A->lvl_off = loff;
   A->b = b;
   A->nnzb = nnzb;
   A->n = (long)m * b;
@@ -1189,6 +1284,7 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   up(&A->row, r0.data(), r0.size());
   up(&A->col, c0.data(), c0.size());
   up(&A->diag_idx, di.data(), di.size());
+  up(&A->lvl_rows, lrows.data(), lrows.size());
   up(&A->v, v, nv);
   up(&A->v32, v32.data(), nv);
   if (!r) r = balloc(A, &A->F64, (size_t)m * b * b);
@@ -1199,6 +1295,7 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   if (!r) r = balloc(A, &A->z, 256);
   if (!r) r = balloc(A, &A->part, kRedBlocks);
   if (!r) r = balloc(A, &A->ticket, 1);
+  if (!r) r = balloc(A, &A->perr, 1);
   if (r) {
     subfv_bcsr_destroy(A);
     return r;
@@ -1221,6 +1318,9 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
+  A->h_row = std::move(r0);
+  A->h_col = std::move(c0);
+  A->h_diag = std::move(di);
   *out = A;
   return 0;
 }
@@ -1254,16 +1354,25 @@ int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs) {
   return 0;
 }
 
-// bcsr_gmres_diag (sparse_bcsr_linear_module.F90:1592-1691) with the Arnoldi
-// process of modifiedArnoldi_diag (:1568-1590) on the device and the
-// Givens rotations / back substitution on the host in the reference order.
-int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x_out, const double *rhs, double tol, int32_t mr,
-                          int32_t precision, int32_t *conv_out, double *stats) {
-  if (!A || !x_out || !rhs || mr < 1 || mr > 200) return SUBFV_E_INVALID;
+}  // extern "C"
+
+namespace {
+
+// The restarted GMRES loops of bcsr_gmres (kind 0, :194-290, modifiedArnoldi
+// :319-340), bcsr_gmres_diag (1, :1592-1691, modifiedArnoldi_diag
+// :1568-1590), bcsr_gmres_lusgs (2, :447-548, modifiedArnoldi_lusgs :642-665)
+// and bcsr_gmres_parallel_lusgs (3, :885-1037, :583-611): the Arnoldi process
+// on the device, the Givens rotations and back substitution on the host in
+// the reference order.
+int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double tol, int32_t mr,
+               int32_t precision, int32_t *conv_out, double *stats) {
+  if (!A || !x_out || !rhs || mr < 1 || mr > 200 || kind < 0 || kind > 3) return SUBFV_E_INVALID;
   subfv_ctx *c = A->ctx;
   hipStream_t s = c->stream;
   const bool fp32 = (precision & 1) != 0;
-  const int maxit = 40;
+  if (fp32 && kind >= 2) return fail(c, SUBFV_E_INVALID, "the LU-SGS variants run in FP64 only");
+  if (kind == 3 && A->nsets <= 0) return fail(c, SUBFV_E_INVALID, "no colour sets: call subfv_bcsr_set_colors first");
+  const int maxit = kind == 1 ? 40 : 20;
   int r = ensure_workspace(A, mr);
   if (r) return r;
   const long n = A->n;
@@ -1290,8 +1399,12 @@ int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x_out, const double *rhs, doubl
       HIPCHK(c, hipMemcpyAsync(A->scal, &rho, sizeof(double), hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
       for (int j = 2; j <= mr + 1; j++) {
-        if ((r = bjacobi(A, fp32, Vc(j - 1), A->tmp))) return r;
-        if ((r = spmv(A, fp32, A->tmp, Vc(j)))) return r;
+        if (kind == 0) {
+          if ((r = spmv(A, fp32, Vc(j - 1), Vc(j)))) return r;
+        } else {
+          if ((r = precond(A, kind, fp32, Vc(j - 1), A->tmp))) return r;
+          if ((r = spmv(A, fp32, A->tmp, Vc(j)))) return r;
+        }
         nspmv++;
         double *w = Vc(j);
         mgs_step(A, w, nullptr, nullptr, Vc(1), Hd(1, j - 1));
@@ -1336,9 +1449,13 @@ int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x_out, const double *rhs, doubl
         z[k - 1] = z[k - 1] / h(k, k);
       }
       HIPCHK(c, hipMemcpyAsync(A->z, z.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_combine, vg, vb, 0, s, n, nr, A->V, n, A->z, A->r);
-      if ((r = bjacobi(A, fp32, A->r, A->zp))) return r;
-      hipLaunchKernelGGL(k_add, vg, vb, 0, s, n, A->x, A->zp);
+      if (kind == 0) {
+        hipLaunchKernelGGL(k_add_combination, vg, vb, 0, s, n, nr, (const double *)A->V, n, (const double *)A->z, A->x);
+      } else {
+        hipLaunchKernelGGL(k_combine, vg, vb, 0, s, n, nr, A->V, n, A->z, A->r);
+        if ((r = precond(A, kind, fp32, A->r, A->zp))) return r;
+        hipLaunchKernelGGL(k_add, vg, vb, 0, s, n, A->x, A->zp);
+      }
       if ((r = spmv(A, false, A->x, A->tmp))) return r;
       nspmv++;
       hipLaunchKernelGGL(k_sub, vg, vb, 0, s, n, A->rhs, A->tmp, A->r);
@@ -1358,6 +1475,208 @@ int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x_out, const double *rhs, doubl
     stats[2] = (double)nspmv;
   }
   return conv ? 0 : fail(c, SUBFV_E_NOT_CONVERGED, "GMRES did not converge");
+}
+
+}  // namespace
+
+extern "C" {
+
+int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
+                          int32_t precision, int32_t *conv, double *stats) {
+  return gmres_impl(A, 1, x, rhs, tol, restart, precision, conv, stats);
+}
+
+int subfv_bcsr_gmres(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart, int32_t precond_kind,
+                     int32_t precision, int32_t *conv, double *stats) {
+  return gmres_impl(A, precond_kind, x, rhs, tol, restart, precision, conv, stats);
+}
+
+int subfv_bcsr_lusgs(subfv_bcsr *A, double *x, const double *rhs) {
+  if (!A || !x || !rhs) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  HIPCHK(c, hipMemcpyAsync(A->rhs, rhs, A->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int r = lusgs(A, A->rhs, A->tmp);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(x, A->tmp, A->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int subfv_bcsr_set_colors(subfv_bcsr *A, const int32_t *color, int32_t *n_sets) {
+  if (!A) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  const int m = A->m;
+  std::vector<int> col1(m);
+  if (color) {
+    for (int i = 0; i < m; i++) {
+      if (color[i] < 1) return fail(c, SUBFV_E_INVALID, "colours are 1-based");
+      col1[i] = color[i];
+    }
+  } else {
+    // greedy colouring of the symmetric closure of the block pattern, rows in order
+    std::vector<std::vector<int>> adj(m);
+    for (int i = 0; i < m; i++)
+      for (int j = A->h_row[i]; j < A->h_row[i + 1]; j++) {
+        const int k = A->h_col[j];
+        if (k != i) {
+          adj[i].push_back(k);
+          adj[k].push_back(i);
+        }
+      }
+    std::vector<int> mark(m + 2, -1);
+    for (int i = 0; i < m; i++) {
+      col1[i] = 0;
+    }
+    for (int i = 0; i < m; i++) {
+      for (int k : adj[i])
+        if (col1[k] > 0) mark[col1[k]] = i;
+      int cc = 1;
+      while (mark[cc] == i) cc++;
+      col1[i] = cc;
+    }
+  }
+  int ns = 0;
+  for (int i = 0; i < m; i++) ns = std::max(ns, col1[i]);
+  std::vector<int> off(ns + 1, 0), rows(m);
+  for (int i = 0; i < m; i++) off[col1[i]]++;
+  for (int k = 0; k < ns; k++) off[k + 1] += off[k];
+  {
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int i = 0; i < m; i++) rows[fill[col1[i] - 1]++] = i;
+  }
+  for (int **p : {&A->set_rows, &A->color})
+    if (*p) {
+      (void)hipFree(*p);
+      A->allocs.erase(std::find(A->allocs.begin(), A->allocs.end(), (void *)*p));
+      *p = nullptr;
+    }
+  int r;
+  if ((r = balloc(A, &A->set_rows, (size_t)m))) return r;
+  if ((r = balloc(A, &A->color, (size_t)m))) return r;
+  HIPCHK(c, hipMemcpy(A->set_rows, rows.data(), m * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(A->color, col1.data(), m * sizeof(int), hipMemcpyHostToDevice));
+  A->set_off = off;
+  A->nsets = ns;
+  if (n_sets) *n_sets = ns;
+  return 0;
+}
+
+int subfv_bcsr_parallel_lusgs(subfv_bcsr *A, double *x, const double *rhs) {
+  if (!A || !x || !rhs) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  HIPCHK(c, hipMemcpyAsync(A->rhs, rhs, A->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int r = plusgs(A, A->rhs, A->tmp);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(x, A->tmp, A->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// bcsr_pcgm (sparse_bcsr_linear_module.F90:1110-1169): CG from the given x
+// with lu_solve(3, ...) on each diagonal block as preconditioner; dot
+// products by the device reductions, scalars on the host.
+int subfv_bcsr_pcgm(subfv_bcsr *A, double *x_io, const double *rhs, double tol, double *stats) {
+  if (!A || !x_io || !rhs) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  hipStream_t s = c->stream;
+  const long n = A->n;
+  int r;
+  if (!A->p && (r = balloc(A, &A->p, (size_t)n))) return r;
+  if (!A->pmr && (r = balloc(A, &A->pmr, (size_t)n))) return r;
+  dim3 vg(kRedBlocks), vb(kVecThreads), mg(nblk(A->m, 256)), mb(256);
+  auto dot = [&](double *a, const double *b, double *out) -> int {
+    mgs_step(A, a, nullptr, nullptr, b, A->scal);
+    HIPCHK(c, hipMemcpyAsync(out, A->scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+  };
+  auto prec = [&]() {
+    hipLaunchKernelGGL(k_pcg_prec, mg, mb, 0, s, A->m, A->b, (const int *)A->diag_idx, (const double *)A->v,
+                       (const double *)A->r, A->pmr, A->perr);
+  };
+  HIPCHK(c, hipMemcpyAsync(A->x, x_io, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(A->rhs, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(A->perr, 0, sizeof(int), s));
+  HIPCHK(c, hipMemsetAsync(A->pmr, 0, n * sizeof(double), s));
+  if ((r = spmv(A, false, A->x, A->tmp))) return r;
+  hipLaunchKernelGGL(k_sub, vg, vb, 0, s, n, A->rhs, A->tmp, A->r);
+  prec();
+  double rdr = 0.0, rpdrp = 0.0;
+  if ((r = dot(A->pmr, A->r, &rdr))) return r;
+  long k = 0;
+  bool conv;
+  if (sqrt(rdr) < tol) {
+    conv = true;
+    rpdrp = rdr;
+  } else {
+    HIPCHK(c, hipMemcpyAsync(A->p, A->pmr, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (k = 1; k <= n; k++) {
+      if ((r = spmv(A, false, A->p, A->tmp))) return r;
+      double pap = 0.0;
+      if ((r = dot(A->p, A->tmp, &pap))) return r;
+      const double alpha = rdr / pap;
+      hipLaunchKernelGGL(k_axpy, vg, vb, 0, s, n, alpha, (const double *)A->p, A->x);
+      hipLaunchKernelGGL(k_axpy, vg, vb, 0, s, n, -alpha, (const double *)A->tmp, A->r);
+      prec();
+      if ((r = dot(A->pmr, A->r, &rpdrp))) return r;
+      if (sqrt(rpdrp) <= tol) {
+        double rr = 0.0;
+        if ((r = dot(A->r, A->r, &rr))) return r;
+        if (sqrt(rr) <= tol) break;
+      }
+      hipLaunchKernelGGL(k_xpby, vg, vb, 0, s, n, (const double *)A->pmr, rpdrp / rdr, A->p);
+      rdr = rpdrp;
+    }
+    conv = !(sqrt(rpdrp) > tol);
+  }
+  int herr = 0;
+  HIPCHK(c, hipMemcpyAsync(&herr, A->perr, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(x_io, A->x, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (stats) {
+    stats[0] = (double)std::min(k, n);
+    stats[1] = sqrt(rpdrp);
+  }
+  if (herr) return fail(c, SUBFV_E_INVALID, "PCG: a diagonal block is singular (lu_solve stop)");
+  return conv ? 0 : fail(c, SUBFV_E_NOT_CONVERGED, "PCG did not converge");
+}
+
+// add_block_to_bcsr (:56-71): v(:,:,k) += M for the first k of row i with
+// col(k) == j (1-based i, j); nothing when the block is not stored.  The
+// block-Jacobi factors are refreshed when a diagonal block changes.
+int subfv_bcsr_add_block(subfv_bcsr *A, int32_t i, int32_t j, const double *M) {
+  if (!A || !M || i < 1 || i > A->m || j < 1 || j > A->m) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  hipStream_t s = c->stream;
+  long k = -1;
+  for (int q = A->h_row[i - 1]; q < A->h_row[i]; q++)
+    if (A->h_col[q] == j - 1) {
+      k = q;
+      break;
+    }
+  if (k < 0) return 0;
+  const int bb = A->b * A->b;
+  HIPCHK(c, hipMemcpyAsync(A->z, M, bb * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_add_block, dim3(1), dim3(64), 0, s, bb, k, (const double *)A->z, A->v, A->v32);
+  if (k == A->h_diag[i - 1]) {
+    dim3 g(nblk(A->m, 256)), bl(256);
+    switch (A->b) {
+      case 3:
+        hipLaunchKernelGGL((k_bcsr_factor<double, 3>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F64);
+        hipLaunchKernelGGL((k_bcsr_factor<float, 3>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F32);
+        break;
+      case 4:
+        hipLaunchKernelGGL((k_bcsr_factor<double, 4>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F64);
+        hipLaunchKernelGGL((k_bcsr_factor<float, 4>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F32);
+        break;
+      default:
+        hipLaunchKernelGGL((k_bcsr_factor<double, 5>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F64);
+        hipLaunchKernelGGL((k_bcsr_factor<float, 5>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F32);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(s));
+  return 0;
 }
 
 }  // extern "C"

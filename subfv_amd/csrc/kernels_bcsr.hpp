@@ -64,15 +64,9 @@ __global__ void k_bcsr_factor(int m, const int *diag_idx, const double *v, T *F)
     for (int j = 0; j < B; j++) o[ii + B * j] = (T)((j >= ii) ? Cm[ii + B * j] : L[ii + B * j]);
 }
 
-// x = D^-1 b with the cached factors, in lu_solve_noswp's operation order.
+// w = D^-1 w with the cached factors f, in lu_solve_noswp's operation order.
 template <typename T, int B>
-__global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const T *f = F + i * B * B;
-  T w[B];
-#pragma unroll
-  for (int q = 0; q < B; q++) w[q] = (T)b[(long)B * i + q];
+SFV_DEV void noswp_apply(const T *f, T *w) {
 #pragma unroll
   for (int k = 0; k < B; k++)
 #pragma unroll
@@ -84,9 +78,173 @@ __global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
     for (int j = k + 1; j < B; j++) w[k] = w[k] - f[k + B * j] * w[j];
     w[k] = w[k] / f[k + B * k];
   }
+}
+
+// x = D^-1 b with the cached factors, in lu_solve_noswp's operation order.
+template <typename T, int B>
+__global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T *f = F + i * B * B;
+  T w[B];
+#pragma unroll
+  for (int q = 0; q < B; q++) w[q] = (T)b[(long)B * i + q];
+  noswp_apply<T, B>(f, w);
 #pragma unroll
   for (int q = 0; q < B; q++) x[(long)B * i + q] = (double)w[q];
 }
+
+// w = w - blk x (one block of a sweep: the block's product summed over its
+// columns from zero, then subtracted)
+template <int B>
+SFV_DEV void sub_block(const double *blk, const double *xb, double *w) {
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < B; c++) acc = acc + blk[r + B * c] * xb[c];
+    w[r] = w[r] - acc;
+  }
+}
+
+// One dependency level of bcsr_lusgs (sparse_bcsr_linear_module.F90:1268-1335):
+// out_i = in_i - sum over j = row(i), ... while col(j) < i of A_ij out_col(j),
+// then (second sweep, SOLVE) out_i = D_i^-1 out_i.  Rows of one level depend
+// only on rows of earlier levels, so the level's rows run in parallel.
+template <int B, bool SOLVE>
+__global__ __launch_bounds__(256) void k_lusgs_level(int cnt, const int *rows, const int *row, const int *col,
+                                                     const double *v, const double *F, const double *in,
+                                                     double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  double w[B];
+#pragma unroll
+  for (int r = 0; r < B; r++) w[r] = in[(long)B * i + r];
+  for (int j = row[i]; j < row[i + 1] && col[j] < i; j++) sub_block<B>(v + (long)j * B * B, out + (long)B * col[j], w);
+  if (SOLVE) noswp_apply<double, B>(F + (long)i * B * B, w);
+#pragma unroll
+  for (int r = 0; r < B; r++) out[(long)B * i + r] = w[r];
+}
+
+// One colour set of bcsr_parallel_lusgs (:1438-1502).  Forward (!BACK):
+// out_i = in_i - sum over the row's blocks with color(col) < cs of A_ij out_col;
+// backward: the blocks with color(col) > cs, then out_i = D_i^-1 out_i.
+template <int B, bool BACK>
+__global__ __launch_bounds__(256) void k_plusgs_set(int cnt, const int *rows, int cs, const int *color,
+                                                    const int *row, const int *col, const double *v,
+                                                    const double *F, const double *in, double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  double w[B];
+#pragma unroll
+  for (int r = 0; r < B; r++) w[r] = in[(long)B * i + r];
+  for (int j = row[i]; j < row[i + 1]; j++) {
+    const int cc = color[col[j]];
+    if (BACK ? (cc > cs) : (cc < cs)) sub_block<B>(v + (long)j * B * B, out + (long)B * col[j], w);
+  }
+  if (BACK) noswp_apply<double, B>(F + (long)i * B * B, w);
+#pragma unroll
+  for (int r = 0; r < B; r++) out[(long)B * i + r] = w[r];
+}
+
+// lu_solve (linear_solver_module.F90:447-484) of a 3 x 3 system: first
+// largest pivot, rows always swapped, multipliers C(i,k) * (1/C(k,k)),
+// back substitution over j increasing.  False where the reference stops
+// (|C(3,3)| < 1e-12).
+SFV_DEV bool lu_solve_plain3(const double *A, const double *b, double *x) {
+  double C[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) C[q] = A[q];
+#pragma unroll
+  for (int i = 0; i < 3; i++) x[i] = b[i];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    int p = k;
+    double amax = fabs(C[k + 3 * k]);
+#pragma unroll
+    for (int i = k + 1; i < 3; i++)
+      if (fabs(C[i + 3 * k]) > amax) {
+        amax = fabs(C[i + 3 * k]);
+        p = i;
+      }
+#pragma unroll
+    for (int i = k + 1; i < 3; i++)
+      if (i == p) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          double t = C[k + 3 * j];
+          C[k + 3 * j] = C[i + 3 * j];
+          C[i + 3 * j] = t;
+        }
+        double t = x[k];
+        x[k] = x[i];
+        x[i] = t;
+      }
+    double invdiag = 1.0 / C[k + 3 * k];
+#pragma unroll
+    for (int i = k + 1; i < 3; i++) {
+      double lik = C[i + 3 * k] * invdiag;
+#pragma unroll
+      for (int j = k; j < 3; j++) C[i + 3 * j] = C[i + 3 * j] - lik * C[k + 3 * j];
+      x[i] = x[i] - lik * x[k];
+    }
+  }
+  if (fabs(C[8]) < 1e-12) return false;
+  x[2] = x[2] / C[8];
+#pragma unroll
+  for (int k = 1; k >= 0; k--) {
+#pragma unroll
+    for (int j = k + 1; j < 3; j++) x[k] = x[k] - C[k + 3 * j] * x[j];
+    x[k] = x[k] / C[k + 3 * k];
+  }
+  return true;
+}
+
+// preconditioner of bcsr_pcgm (:1110-1169): lu_solve(3, v(:,:,d(i)), mr_i, r_i)
+// on the first nine values of each diagonal block (the 3 x 3 dummy argument
+// of the reference); the first three entries of mr_i are written.
+__global__ __launch_bounds__(256) void k_pcg_prec(int m, int b, const int *diag_idx, const double *v,
+                                                  const double *r, double *mr, int *err) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double x[3];
+  if (!lu_solve_plain3(v + (long)diag_idx[i] * b * b, r + (long)b * i, x)) atomicOr(err, 1);
+#pragma unroll
+  for (int q = 0; q < 3; q++) mr[(long)b * i + q] = x[q];
+}
+
+// y = y + a * x   (a passed by value)
+__global__ void k_axpy(long n, double a, const double *x, double *y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = y[i] + a * x[i];
+}
+
+// p = mr + beta * p
+__global__ void k_xpby(long n, const double *mr, double beta, double *p) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = mr[i] + beta * p[i];
+}
+
+// x = x + z[0] V[:,0] + z[1] V[:,1] + ... added one column at a time (bcsr_gmres)
+__global__ void k_add_combination(long n, int k, const double *V, long ldv, const double *z, double *x) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    double acc = x[e];
+    for (int i = 0; i < k; i++) acc = acc + z[i] * V[(long)i * ldv + e];
+    x[e] = acc;
+  }
+}
+
+// v(:,:,k) += M (add_block_to_bcsr), FP64 values and their FP32 copy
+__global__ void k_add_block(int bb, long k, const double *M, double *v, float *v32) {
+  const int q = threadIdx.x;
+  if (q >= bb) return;
+  const double nv = v[k * bb + q] + M[q];
+  v[k * bb + q] = nv;
+  v32[k * bb + q] = (float)nv;
+}
+
 
 // -------------------------------------------------------- vector reductions
 // Sums in two launches: per-block partials (fixed grid, fixed order), then
