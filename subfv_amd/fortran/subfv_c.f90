@@ -212,6 +212,49 @@ module subfv_c
       integer(c_int32_t), intent(out) :: conv
       real(c_double), intent(out) :: stats(3)
     end function
+    integer(c_int) function subfv_bcsr_gmres(A, x, rhs, tol, restart, precond, precision, conv, stats) &
+        bind(C, name="subfv_bcsr_gmres")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: A
+      real(c_double), intent(out) :: x(*)
+      real(c_double), intent(in) :: rhs(*)
+      real(c_double), value :: tol
+      integer(c_int32_t), value :: restart, precond, precision
+      integer(c_int32_t), intent(out) :: conv
+      real(c_double), intent(out) :: stats(3)
+    end function
+    integer(c_int) function subfv_bcsr_lusgs(A, x, rhs) bind(C, name="subfv_bcsr_lusgs")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: A
+      real(c_double), intent(out) :: x(*)
+      real(c_double), intent(in) :: rhs(*)
+    end function
+    integer(c_int) function subfv_bcsr_set_colors(A, color, n_sets) bind(C, name="subfv_bcsr_set_colors")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: A
+      integer(c_int32_t), intent(in) :: color(*)
+      integer(c_int32_t), intent(out) :: n_sets
+    end function
+    integer(c_int) function subfv_bcsr_parallel_lusgs(A, x, rhs) bind(C, name="subfv_bcsr_parallel_lusgs")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: A
+      real(c_double), intent(out) :: x(*)
+      real(c_double), intent(in) :: rhs(*)
+    end function
+    integer(c_int) function subfv_bcsr_pcgm(A, x, rhs, tol, stats) bind(C, name="subfv_bcsr_pcgm")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: A
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: rhs(*)
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: stats(2)
+    end function
+    integer(c_int) function subfv_bcsr_add_block(A, i, j, M) bind(C, name="subfv_bcsr_add_block")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: A
+      integer(c_int32_t), value :: i, j
+      real(c_double), intent(in) :: M(*)
+    end function
   end interface
 
 contains

@@ -327,4 +327,94 @@ contains
     if (st /= SUBFV_OK .and. st /= SUBFV_E_NOT_CONVERGED) call check(ctx, st)
     conv = c /= 0
   end subroutine
+
+  ! the GMRES variants on the same interface: precond 0 bcsr_gmres, 2
+  ! bcsr_gmres_lusgs, 3 bcsr_gmres_parallel_lusgs (colour sets from
+  ! bcsr_set_color_sets)
+  subroutine bcsr_gmres_kind(ctx, A, x, b, tol, m, conv, precond)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(out) :: x(:)
+    real(DP), intent(in) :: b(:), tol
+    integer, intent(in) :: m, precond
+    logical, intent(out) :: conv
+    integer(c_int32_t) :: c
+    integer(c_int) :: st
+    real(DP) :: stats(3)
+    st = subfv_bcsr_gmres(A, x, b, tol, int(m, c_int32_t), int(precond, c_int32_t), 0_c_int32_t, c, stats)
+    if (st /= SUBFV_OK .and. st /= SUBFV_E_NOT_CONVERGED) call check(ctx, st)
+    conv = c /= 0
+  end subroutine
+
+  ! bcsr_gmres(A, x, b, tol, m, conv, ...)
+  subroutine bcsr_gmres(ctx, A, x, b, tol, m, conv)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(out) :: x(:)
+    real(DP), intent(in) :: b(:), tol
+    integer, intent(in) :: m
+    logical, intent(out) :: conv
+    call bcsr_gmres_kind(ctx, A, x, b, tol, m, conv, 0)
+  end subroutine
+
+  ! bcsr_gmres_lusgs(A, x, b, tol, m, conv, ...)
+  subroutine bcsr_gmres_lusgs(ctx, A, x, b, tol, m, conv)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(out) :: x(:)
+    real(DP), intent(in) :: b(:), tol
+    integer, intent(in) :: m
+    logical, intent(out) :: conv
+    call bcsr_gmres_kind(ctx, A, x, b, tol, m, conv, 2)
+  end subroutine
+
+  ! the colour sets of bcsr_parallel_lusgs, given as a colour per block row
+  subroutine bcsr_set_color_sets(ctx, A, color, n_color_sets)
+    type(c_ptr), intent(in) :: ctx, A
+    integer(c_int32_t), intent(in) :: color(:)
+    integer, intent(out) :: n_color_sets
+    integer(c_int32_t) :: ns
+    call check(ctx, subfv_bcsr_set_colors(A, color, ns))
+    n_color_sets = ns
+  end subroutine
+
+  ! bcsr_gmres_parallel_lusgs(A, x, b, tol, m, n_color_sets, color_set, color, conv, ...)
+  subroutine bcsr_gmres_parallel_lusgs(ctx, A, x, b, tol, m, conv)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(out) :: x(:)
+    real(DP), intent(in) :: b(:), tol
+    integer, intent(in) :: m
+    logical, intent(out) :: conv
+    call bcsr_gmres_kind(ctx, A, x, b, tol, m, conv, 3)
+  end subroutine
+
+  ! bcsr_lusgs(A, x, b)
+  subroutine bcsr_lusgs(ctx, A, x, b)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(out) :: x(:)
+    real(DP), intent(in) :: b(:)
+    call check(ctx, subfv_bcsr_lusgs(A, x, b))
+  end subroutine
+
+  ! bcsr_parallel_lusgs(A, x, b, n_color_sets, color_set, color)
+  subroutine bcsr_parallel_lusgs(ctx, A, x, b)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(out) :: x(:)
+    real(DP), intent(in) :: b(:)
+    call check(ctx, subfv_bcsr_parallel_lusgs(A, x, b))
+  end subroutine
+
+  ! bcsr_pcgm(A, x, b, tol): x is the initial guess on entry
+  subroutine bcsr_pcgm(ctx, A, x, b, tol)
+    type(c_ptr), intent(in) :: ctx, A
+    real(DP), intent(inout) :: x(:)
+    real(DP), intent(in) :: b(:), tol
+    real(DP) :: stats(2)
+    call check(ctx, subfv_bcsr_pcgm(A, x, b, tol, stats))
+  end subroutine
+
+  ! add_block_to_bcsr(mat, i, j, M)
+  subroutine add_block_to_bcsr(ctx, A, i, j, M)
+    type(c_ptr), intent(in) :: ctx, A
+    integer, intent(in) :: i, j
+    real(DP), intent(in) :: M(:, :)
+    call check(ctx, subfv_bcsr_add_block(A, int(i, c_int32_t), int(j, c_int32_t), M))
+  end subroutine
 end module subfv_ns_gpu
