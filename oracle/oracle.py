@@ -147,6 +147,9 @@ def lib():
         L.om_bcsr_pcgm.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, dp]
         L.om_add_diffusion.argtypes = [P, cp, dp, dp, dp]
         L.om_test_inv.argtypes = [C.c_int, dp, dp]
+        L.om_inverse_3_by_3.argtypes = [dp, dp]
+        L.om_compute_rhs_lagrange.argtypes = [P, dp, dp, C.c_int, dp, dp]
+        L.om_acoustic_step.argtypes = [P, dp, C.c_int, ip, dp, C.c_double, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -241,6 +244,35 @@ class OracleMesh:
         if st:
             raise RuntimeError(f"oracle compute_euler_flux status {st}")
         return (rhs, sl, vp) if want_vp else (rhs, sl)
+
+    def compute_rhs_lagrange(self, sol: np.ndarray, mass: np.ndarray, b2d: bool = True):
+        """compute_rhs_lagrange: (rhs (ne, 5), vp (nv, 3))."""
+        ne, nv = self.n_elems, self.n_vert
+        sol = np.ascontiguousarray(sol, np.float64).reshape(ne, 5)
+        rhs = np.zeros((ne, 5))
+        vp = np.zeros((nv, 3))
+        lib().om_compute_rhs_lagrange(self.h, sol.ravel(), np.ascontiguousarray(mass, np.float64), int(b2d),
+                                      rhs.reshape(-1), vp.reshape(-1))
+        return rhs, vp
+
+    def acoustic_step(self, sol: np.ndarray, cfl: float, bc_kind=(), bc_val=()):
+        """One step of the acoustic loop on sol (ne, 4) = (p, u, v, w).
+        bc_kind[t-1] for tag t: 0 wall, 1 freestream, 2 outflow; bc_val[t-1] the
+        freestream state.  Returns (new sol, dt)."""
+        ne = self.n_elems
+        out = np.ascontiguousarray(sol, np.float64).reshape(ne, 4).copy()
+        nb = len(bc_kind)
+        kinds = np.zeros(nb + 1, np.int32)
+        vals = np.zeros((nb + 1, 4))
+        for t in range(nb):
+            kinds[t + 1] = bc_kind[t]
+            if len(bc_val) > t:
+                vals[t + 1] = bc_val[t]
+        dt = C.c_double(0.0)
+        st = lib().om_acoustic_step(self.h, out.reshape(-1), nb, kinds, vals.reshape(-1), cfl, C.byref(dt))
+        if st:
+            raise RuntimeError("oracle acoustic_step: unsupported boundary tag")
+        return out, dt.value
 
     def add_diffusion(self, cfg: Config, sol: np.ndarray, rhs=None, diag=None):
         """The viscous block of compute_rhs_ns (momentum, then heat) added to
@@ -384,6 +416,12 @@ def bcsr_pcgm(m, b, row, col, v, rhs, tol, x0=None):
     info = np.zeros(2)
     conv = lib().om_bcsr_pcgm(m, b, _i32(row), _i32(col), _f64(v), x, _f64(rhs), tol, info)
     return x, int(conv), dict(iterations=int(info[0]), rho=info[1])
+
+
+def inverse_3_by_3(A: np.ndarray) -> np.ndarray:
+    out = np.zeros(9)
+    lib().om_inverse_3_by_3(np.ascontiguousarray(np.asarray(A, np.float64).T).ravel(), out)
+    return out.reshape(3, 3).T
 
 
 def inv(A: np.ndarray) -> np.ndarray:

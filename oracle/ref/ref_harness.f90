@@ -63,4 +63,10 @@ contains
     real(c_double), intent(in) :: u(5), du(5)
     ref_max_omega = max_omega_for_positivity(u, du)
   end function
+  ! the intrinsic norm2 as the compiler of the reference evaluates it
+  real(c_double) function ref_norm2(n, x) bind(C, name="ref_norm2")
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: x(n)
+    ref_norm2 = norm2(x)
+  end function
 end module ref_harness
