@@ -184,6 +184,22 @@ int subfv_run(subfv_ctx *ctx, int32_t n_steps, int32_t time_scheme, double *t, d
 int subfv_profile_enable(subfv_ctx *ctx, int32_t on);
 int subfv_profile_read(subfv_ctx *ctx, const char *phase, double *ms_total, int64_t *count);
 
+/* ------------------------------------------------ lagrange / acoustic sweeps */
+/* compute_rhs_lagrange (src/lagrange/lagrange_module.F90:14-147) on the
+ * context's mesh: sol[C][5] = (rho, u, v, w, E), mass[C]; writes rhs[C][5]
+ * and the nodal velocities vp[V][3].  b2d as boundary_2d.  Each cell's rhs
+ * receives its terms in the reference's vertex / sub-face order. */
+int subfv_compute_rhs_lagrange(subfv_ctx *ctx, const double *sol, const double *mass, int32_t b2d, double *rhs,
+                               double *vp);
+/* n_steps of the acoustic loop (src/acoustic/acoustic_main.F90:171-290:
+ * nodal pressure, multi_point_pressure fluxes in the face frame of rotate,
+ * dt = min vol / (face area * a), sol += cfl dt / vol * flux, t += dt) on
+ * sol[C][4] = (p, u, v, w) in place.  Boundary tag t (1..n_bc) has
+ * bc_kind[t-1]: 0 wall, 1 freestream (state bc_val[t-1][4]), 2 outflow;
+ * untagged faces are walls.  *t advances; *dt_last gets the last step. */
+int subfv_acoustic_run(subfv_ctx *ctx, double *sol, int32_t n_steps, double cfl, int32_t n_bc, const int32_t *bc_kind,
+                       const double *bc_val, double *t, double *dt_last);
+
 /* ------------------------------------------------------------------ BCSR */
 typedef struct subfv_bcsr subfv_bcsr;
 /* bcsr_matrix_type: m block rows, b x b blocks, 1-based row(m+1), col(nnzb),

@@ -226,6 +226,31 @@ class Solver:
         self._check(_lib.lib().subfv_compute_rhs_ns(self.h))
         return self.get("rhs"), self.get("sum_lambda")
 
+    def compute_rhs_lagrange(self, sol, mass, b2d: bool = True):
+        """compute_rhs_lagrange on the device: (rhs (C, 5), vp (V, 3))."""
+        C_, V_ = self.n_elems, self.n_vert
+        sol = _lib.f64(sol).reshape(C_, 5)
+        mass = _lib.f64(mass).reshape(C_)
+        rhs = np.zeros((C_, 5))
+        vp = np.zeros((V_, 3))
+        self._check(_lib.lib().subfv_compute_rhs_lagrange(self.h, _lib.dptr(sol), _lib.dptr(mass), int(b2d),
+                                                          _lib.dptr(rhs), _lib.dptr(vp)))
+        return rhs, vp
+
+    def acoustic_run(self, sol, n_steps: int, cfl: float, bc_kind=(), bc_val=(), t: float = 0.0):
+        """n_steps of the acoustic loop on sol (C, 4) = (p, u, v, w);
+        bc_kind per tag: 0 wall, 1 freestream, 2 outflow.  Returns (sol, t, dt)."""
+        out = _lib.f64(sol).reshape(self.n_elems, 4).copy()
+        nb = len(bc_kind)
+        kinds = _lib.i32(np.asarray(bc_kind, np.int32).reshape(nb)) if nb else np.zeros(1, np.int32)
+        vals = np.zeros((max(nb, 1), 4))
+        for i in range(min(nb, len(bc_val))):
+            vals[i] = bc_val[i]
+        tt, dt = C.c_double(t), C.c_double(0.0)
+        self._check(_lib.lib().subfv_acoustic_run(self.h, _lib.dptr(out), int(n_steps), float(cfl), nb,
+                                                  _lib.iptr(kinds), _lib.dptr(vals), C.byref(tt), C.byref(dt)))
+        return out, tt.value, dt.value
+
     def profile(self, on: bool = True):
         self._check(_lib.lib().subfv_profile_enable(self.h, int(on)))
 

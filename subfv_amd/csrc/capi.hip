@@ -16,6 +16,7 @@
 #include "kernels_diff.hpp"
 #include "kernels_flux.hpp"
 #include "kernels_geom.hpp"
+#include "kernels_sweeps.hpp"
 #include "mesh_build.hpp"
 
 using namespace sfv;
@@ -929,6 +930,129 @@ int subfv_compute_rhs_ns(subfv_ctx *c) {
                        (const double *)c->rdiff);
     HIPCHK(c, hipGetLastError());
   }
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// device scratch of one sweep call, freed on every exit path
+struct Scratch {
+  std::vector<void *> p;
+  ~Scratch() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  hipError_t get(T **out, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) p.push_back(q);
+    *out = (T *)q;
+    return e;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+// compute_rhs_lagrange (src/lagrange/lagrange_module.F90:14-147)
+int subfv_compute_rhs_lagrange(subfv_ctx *c, const double *sol, const double *mass, int32_t b2d, double *rhs,
+                               double *vp) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  if (!sol || !mass || !rhs || !vp) return fail(c, SUBFV_E_INVALID, "null array");
+  HIPCHK(c, hipSetDevice(c->device));
+  const DevMesh &m = c->dm;
+  Scratch sc;
+  double *dsol, *dmass, *T, *dvp, *drhs;
+  HIPCHK(c, sc.get(&dsol, 5 * m.C));
+  HIPCHK(c, sc.get(&dmass, m.C));
+  HIPCHK(c, sc.get(&T, (size_t)kSweepTerm * kSlotSrc * m.SE));
+  HIPCHK(c, sc.get(&dvp, 3 * m.V));
+  HIPCHK(c, sc.get(&drhs, 5 * m.C));
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(dsol, sol, 5 * m.C * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dmass, mass, m.C * sizeof(double), hipMemcpyHostToDevice, s));
+  LagrangeArgs a;
+  a.m = m;
+  a.sol = dsol;
+  a.mass = dmass;
+  a.T = T;
+  a.vp = dvp;
+  a.b2d = b2d ? 1 : 0;
+  a.nblocks = m.ngroups;
+  a.per_xcd = (m.ngroups + 7) / 8;
+  hipLaunchKernelGGL(k_lagrange_group, dim3(8 * a.per_xcd), dim3(kGrpThreads), 0, s, a);
+  hipLaunchKernelGGL(k_sweep_gather<5>, dim3(nblk(m.C, kCellThreads)), dim3(kCellThreads), 0, s, m,
+                     (const double *)T, drhs);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(rhs, drhs, 5 * m.C * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(vp, dvp, 3 * m.V * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return 0;
+}
+
+// n_steps of the acoustic loop (src/acoustic/acoustic_main.F90:171-290)
+int subfv_acoustic_run(subfv_ctx *c, double *sol, int32_t n_steps, double cfl, int32_t n_bc, const int32_t *bc_kind,
+                       const double *bc_val, double *t, double *dt_last) {
+  if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  if (!sol || !t || n_steps < 0 || n_bc < 0 || (n_bc > 0 && (!bc_kind || !bc_val)))
+    return fail(c, SUBFV_E_INVALID, "bad acoustic arguments");
+  for (int i = 0; i < n_bc; i++)
+    if (bc_kind[i] < 0 || bc_kind[i] > 2)
+      return fail(c, SUBFV_E_BAD_BC, "acoustic boundary kinds are 0 wall, 1 freestream, 2 outflow");
+  HIPCHK(c, hipSetDevice(c->device));
+  const DevMesh &m = c->dm;
+  Scratch sc;
+  double *dsol, *T, *blk, *dt_t, *dval;
+  int *dkind, *derr;
+  const int ng = m.ngroups;
+  HIPCHK(c, sc.get(&dsol, 4 * m.C));
+  HIPCHK(c, sc.get(&T, (size_t)kSweepTerm * kSlotSrc * m.SE));
+  HIPCHK(c, sc.get(&blk, ng));
+  HIPCHK(c, sc.get(&dt_t, 2));
+  HIPCHK(c, sc.get(&dkind, n_bc + 1));
+  HIPCHK(c, sc.get(&dval, 4 * (n_bc + 1)));
+  HIPCHK(c, sc.get(&derr, 1));
+  std::vector<int> hk(n_bc + 1, 0);
+  std::vector<double> hv(4 * (n_bc + 1), 0.0);
+  for (int i = 0; i < n_bc; i++) {
+    hk[i + 1] = bc_kind[i];
+    for (int k = 0; k < 4; k++) hv[4 * (i + 1) + k] = bc_val[4 * i + k];
+  }
+  const double ht[2] = {0.0, *t};
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(dsol, sol, 4 * m.C * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dkind, hk.data(), hk.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dval, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dt_t, ht, sizeof(ht), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(derr, 0, sizeof(int), s));
+  AcousticArgs a;
+  a.m = m;
+  a.sol = dsol;
+  a.T = T;
+  a.blk = blk;
+  a.bc_kind = dkind;
+  a.bc_val = dval;
+  a.n_bc = n_bc;
+  a.err = derr;
+  a.nblocks = ng;
+  a.per_xcd = (ng + 7) / 8;
+  for (int it = 0; it < n_steps; it++) {
+    hipLaunchKernelGGL(k_acoustic_group, dim3(8 * a.per_xcd), dim3(kGrpThreads), 0, s, a);
+    hipLaunchKernelGGL(k_acoustic_dt, dim3(1), dim3(kCellThreads), 0, s, (const double *)blk, ng, dt_t);
+    hipLaunchKernelGGL(k_acoustic_update, dim3(nblk(m.C, kCellThreads)), dim3(kCellThreads), 0, s, m,
+                       (const double *)T, dsol, cfl, (const double *)dt_t);
+  }
+  HIPCHK(c, hipGetLastError());
+  double hdt[2];
+  int herr = 0;
+  HIPCHK(c, hipMemcpyAsync(hdt, dt_t, sizeof(hdt), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(sol, dsol, 4 * m.C * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (herr) return fail(c, SUBFV_E_BAD_BC, "acoustic: a boundary tag has no boundary kind");
+  *t = hdt[1];
+  if (dt_last) *dt_last = hdt[0];
   return 0;
 }
 
