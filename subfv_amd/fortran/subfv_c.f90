@@ -177,6 +177,26 @@ module subfv_c
       real(c_double), intent(out) :: ms_total
       integer(c_int64_t), intent(out) :: count
     end function
+    integer(c_int) function subfv_compute_rhs_lagrange(ctx, sol, mass, b2d, rhs, vp) &
+        bind(C, name="subfv_compute_rhs_lagrange")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: sol(*), mass(*)
+      integer(c_int32_t), value :: b2d
+      real(c_double), intent(out) :: rhs(*), vp(*)
+    end function
+    integer(c_int) function subfv_acoustic_run(ctx, sol, n_steps, cfl, n_bc, bc_kind, bc_val, t, dt_last) &
+        bind(C, name="subfv_acoustic_run")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: sol(*)
+      integer(c_int32_t), value :: n_steps, n_bc
+      real(c_double), value :: cfl
+      integer(c_int32_t), intent(in) :: bc_kind(*)
+      real(c_double), intent(in) :: bc_val(*)
+      real(c_double), intent(inout) :: t
+      real(c_double), intent(out) :: dt_last
+    end function
     integer(c_int) function subfv_bcsr_create(ctx, A, m, b, nnzb, row, col, v) bind(C, name="subfv_bcsr_create")
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx

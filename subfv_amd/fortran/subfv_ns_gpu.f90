@@ -283,6 +283,31 @@ contains
     if (omega < 1.0_DP .and. me == 0 .and. num_procs >= 1) print *, "OMEGA", omega
   end subroutine
 
+  ! ------------------------------------------------- lagrange / acoustic
+  ! compute_rhs_lagrange(mesh, sol, vp, dt, rhs, n_bc, bc_type, bc_val, b2d, mass)
+  ! on the context's mesh: sol(5, n_elems), mass(n_elems) -> rhs(5, n_elems),
+  ! vp(3, n_vert)
+  subroutine compute_rhs_lagrange(ctx, sol, vp, rhs, b2d, mass)
+    type(c_ptr), intent(in) :: ctx
+    real(DP), intent(in) :: sol(:, :), mass(:)
+    real(DP), intent(out) :: vp(:, :), rhs(:, :)
+    logical, intent(in) :: b2d
+    call check(ctx, subfv_compute_rhs_lagrange(ctx, sol, mass, merge(1_c_int32_t, 0_c_int32_t, b2d), rhs, vp))
+  end subroutine
+
+  ! n_steps of the acoustic loop on sol(4, n_elems) = (p, u, v, w); bc_kind(t)
+  ! for tag t: 0 wall, 1 freestream (bc_val(:, t)), 2 outflow
+  subroutine acoustic_run(ctx, sol, n_steps, cfl, bc_kind, bc_val, t, dt)
+    type(c_ptr), intent(in) :: ctx
+    real(DP), intent(inout) :: sol(:, :), t
+    integer, intent(in) :: n_steps
+    real(DP), intent(in) :: cfl, bc_val(:, :)
+    integer(c_int32_t), intent(in) :: bc_kind(:)
+    real(DP), intent(out) :: dt
+    call check(ctx, subfv_acoustic_run(ctx, sol, int(n_steps, c_int32_t), cfl, int(size(bc_kind), c_int32_t), &
+      bc_kind, bc_val, t, dt))
+  end subroutine
+
   ! ------------------------------------------------------------ BCSR
   type(c_ptr) function bcsr_create(ctx, m, b, nnzb, row, col, v) result(A)
     type(c_ptr), intent(in) :: ctx
