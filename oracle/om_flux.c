@@ -33,6 +33,39 @@ static double dmax(double a, double b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
 static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+/* sol_potential_flow_2d / _3d (ns_euler_exact_sol_module.F90:105-170): the
+ * primitive state of potential flow around a cylinder (2-D, r0 = 0.5 inside an
+ * outer radius 5.5, through atan2 / cos / sin) or a sphere (3-D, r0 = 0.5) at
+ * Mach 1e-4; integer powers as products (r^5 = ((r r)(r r)) r). */
+void om_potential_flow(int three_d, const double *x, double *w) {
+  const double rhoinf = 1.4, mach = 1e-4, pinf = 1.0, r0 = 0.5;
+  const double vinf = mach * sqrt(GAMMA * pinf / rhoinf);
+  double ux, uy, uz = 0.0, vmag2;
+  if (!three_d) {
+    const double r1 = 5.5;
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+    const double theta = atan2(x[1], x[0]);
+    const double k = (r1 * r1) / (r1 * r1 - r0 * r0), q = (r0 / r) * (r0 / r);
+    const double vr = k * (1.0 - q) * cos(theta);
+    const double vt = -k * (1.0 + q) * sin(theta);
+    ux = vinf * (cos(theta) * vr - sin(theta) * vt);
+    uy = vinf * (sin(theta) * vr + cos(theta) * vt);
+    vmag2 = ux * ux + uy * uy;
+  } else {
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double f = (r0 * r0 * r0) / (2.0 * (((r * r) * (r * r)) * r));
+    ux = vinf * (1.0 + f * (r * r - 3.0 * (x[0] * x[0])));
+    uy = -3.0 * vinf * f * x[0] * x[1];
+    uz = -3.0 * vinf * f * x[0] * x[2];
+    vmag2 = ux * ux + uy * uy + uz * uz;
+  }
+  w[0] = rhoinf;
+  w[1] = ux;
+  w[2] = uy;
+  w[3] = uz;
+  w[4] = pinf + 0.5 * rhoinf * (vinf * vinf - vmag2);
+}
+
 /* ------------------------------------------------------------- primitives */
 static void primit_to_conserv(const double *w, double *u) {
   u[0] = w[0];
@@ -151,6 +184,13 @@ static int compute_right_state(const om_mesh *m, const om_config *cfg, int sf, i
         primit_to_conserv(wa, sol_r);
       else
         wall_reflect(n, sol_l, sol_r);
+      return OM_OK;
+    }
+    case OM_BC_POTENTIAL_FLOW_2D:
+    case OM_BC_POTENTIAL_FLOW_3D: {
+      double w[5];
+      om_potential_flow(id == OM_BC_POTENTIAL_FLOW_3D, &FC(m, m->sf_face[sf], 0), w);
+      primit_to_conserv(w, sol_r);
       return OM_OK;
     }
     default:

@@ -39,7 +39,9 @@ enum {
   OM_BC_OUTFLOWSUPERSONIC = 4,
   OM_BC_INFLOW_POND = 5,
   OM_BC_INOUT_DOUBLE_MACH = 6,
-  OM_BC_DOUBLE_MACH_BOTTOM = 7
+  OM_BC_DOUBLE_MACH_BOTTOM = 7,
+  OM_BC_POTENTIAL_FLOW_2D = 8,
+  OM_BC_POTENTIAL_FLOW_3D = 9
 };
 
 /* Error codes shared with the product's C-ABI numbering. */

@@ -36,6 +36,8 @@ BCS = {
     "inflow_pond": 5,
     "inout_double_mach": 6,
     "double_mach_bottom": 7,
+    "potential_flow_2d": 8,
+    "potential_flow_3d": 9,
 }
 
 
@@ -148,6 +150,7 @@ def lib():
         L.om_add_diffusion.argtypes = [P, cp, dp, dp, dp]
         L.om_test_inv.argtypes = [C.c_int, dp, dp]
         L.om_inverse_3_by_3.argtypes = [dp, dp]
+        L.om_potential_flow.argtypes = [C.c_int, dp, dp]
         L.om_compute_rhs_lagrange.argtypes = [P, dp, dp, C.c_int, dp, dp]
         L.om_acoustic_step.argtypes = [P, dp, C.c_int, ip, dp, C.c_double, C.POINTER(C.c_double)]
         _lib = L
@@ -416,6 +419,13 @@ def bcsr_pcgm(m, b, row, col, v, rhs, tol, x0=None):
     info = np.zeros(2)
     conv = lib().om_bcsr_pcgm(m, b, _i32(row), _i32(col), _f64(v), x, _f64(rhs), tol, info)
     return x, int(conv), dict(iterations=int(info[0]), rho=info[1])
+
+
+def potential_flow(x, three_d: bool) -> np.ndarray:
+    """sol_potential_flow_2d / _3d at the point x: primitive (rho, u, v, w, p)."""
+    w = np.zeros(5)
+    lib().om_potential_flow(int(three_d), np.ascontiguousarray(x, np.float64), w)
+    return w
 
 
 def inverse_3_by_3(A: np.ndarray) -> np.ndarray:

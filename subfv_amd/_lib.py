@@ -41,6 +41,8 @@ BCS = {
     "inflow_pond": 5,
     "inout_double_mach": 6,
     "double_mach_bottom": 7,
+    "potential_flow_2d": 8,
+    "potential_flow_3d": 9,
 }
 
 

@@ -769,7 +769,7 @@ int subfv_set_config(subfv_ctx *c, const subfv_config *cfg) {
   if (cfg->n_bc < 0 || cfg->n_bc > SUBFV_MNBC) return fail(c, SUBFV_E_INVALID, "n_bc out of range");
   if (cfg->method < 0 || cfg->method > 6) return fail(c, SUBFV_E_INVALID, "Method for second order unknown !");
   for (int i = 0; i < cfg->n_bc; i++)
-    if (cfg->bc_euler_id[i] < 1 || cfg->bc_euler_id[i] > 7)
+    if (cfg->bc_euler_id[i] < 1 || cfg->bc_euler_id[i] > 9)
       return fail(c, SUBFV_E_INVALID, "ERROR: bc_type unrecognized");
   c->cfg = *cfg;
   BCParams &b = c->bc;

@@ -30,7 +30,9 @@ enum BCKind : int {
   kBCOutflowSupersonic = 4,
   kBCInflowPond = 5,
   kBCInoutDoubleMach = 6,
-  kBCDoubleMachBottom = 7
+  kBCDoubleMachBottom = 7,
+  kBCPotentialFlow2D = 8,
+  kBCPotentialFlow3D = 9
 };
 enum Err : int { kErrNegStarDensity = 1, kErrBadOmega = 2, kErrBadBC = 4 };
 
@@ -90,6 +92,39 @@ SFV_DEV void reflect(const double *n, const double *ul, double *ur) {
 }
 
 // compute_right_state, ns_euler_recon_module.F90:331-409 (conserved in/out)
+// sol_potential_flow_2d / _3d (ns_euler_exact_sol_module.F90:105-170):
+// potential flow around a cylinder (2-D, r0 = 0.5, outer radius 5.5, through
+// atan2 / cos / sin) or a sphere (3-D, r0 = 0.5) at Mach 1e-4, primitive state
+// at the face centroid x; integer powers as products (r^5 = ((r r)(r r)) r).
+SFV_DEV void potential_flow(bool three_d, const double *x, double *w) {
+  const double rhoinf = 1.4, mach = 1e-4, pinf = 1.0, r0 = 0.5;
+  const double vinf = mach * sqrt(kGamma * pinf / rhoinf);
+  double ux, uy, uz = 0.0, vmag2;
+  if (!three_d) {
+    const double r1 = 5.5;
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+    const double theta = atan2(x[1], x[0]);
+    const double k = (r1 * r1) / (r1 * r1 - r0 * r0), q = (r0 / r) * (r0 / r);
+    const double vr = k * (1.0 - q) * cos(theta);
+    const double vt = -k * (1.0 + q) * sin(theta);
+    ux = vinf * (cos(theta) * vr - sin(theta) * vt);
+    uy = vinf * (sin(theta) * vr + cos(theta) * vt);
+    vmag2 = ux * ux + uy * uy;
+  } else {
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double f = (r0 * r0 * r0) / (2.0 * (((r * r) * (r * r)) * r));
+    ux = vinf * (1.0 + f * (r * r - 3.0 * (x[0] * x[0])));
+    uy = -3.0 * vinf * f * x[0] * x[1];
+    uz = -3.0 * vinf * f * x[0] * x[2];
+    vmag2 = ux * ux + uy * uy + uz * uz;
+  }
+  w[0] = rhoinf;
+  w[1] = ux;
+  w[2] = uy;
+  w[3] = uz;
+  w[4] = pinf + 0.5 * rhoinf * (vinf * vinf - vmag2);
+}
+
 SFV_DEV int ghost_state(const BCParams &bc, int re, const double *n, const double *fc, double t,
                         const double *ul, double *ur) {
   if (re == -1) {
@@ -137,6 +172,13 @@ SFV_DEV int ghost_state(const BCParams &bc, int re, const double *n, const doubl
         to_cons(wa, ur);
       else
         reflect(n, ul, ur);
+      return 0;
+    }
+    case kBCPotentialFlow2D:
+    case kBCPotentialFlow3D: {
+      double w[5];
+      potential_flow(bc.bc_id[ib] == kBCPotentialFlow3D, fc, w);
+      to_cons(w, ur);
       return 0;
     }
     default:

@@ -111,6 +111,8 @@ program subfvns_gpu
     case ("inflow_pond"); cfg%bc_euler_id(i) = 5
     case ("inout_double_mach"); cfg%bc_euler_id(i) = 6
     case ("double_mach_bottom"); cfg%bc_euler_id(i) = 7
+    case ("potential_flow_2d"); cfg%bc_euler_id(i) = 8
+    case ("potential_flow_3d"); cfg%bc_euler_id(i) = 9
     case default
       print *, "BC TYPE NOT RECOGNIZED !"
       error stop

@@ -385,3 +385,32 @@ def test_bcsr_gmres_matches_direct_solve():
     assert conv
     assert np.linalg.norm(rhs - A @ x) < tol
     assert np.allclose(x, np.linalg.solve(A, rhs), rtol=1e-8, atol=1e-9)
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_potential_flow_states(three_d):
+    """sol_potential_flow_2d/_3d: no flow through the body (r = 0.5), the
+    field is divergence-free, Bernoulli holds, and the 3-D field tends to the
+    free stream far away."""
+    vinf = 1e-4
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        d = rng.standard_normal(3)
+        if not three_d:
+            d[2] = 0.0
+        d /= np.linalg.norm(d)
+        w = O.potential_flow(0.5 * d, three_d)
+        assert abs(w[1:4] @ d) < 1e-12 * vinf
+        assert abs(w[4] + 0.5 * 1.4 * (w[1:4] @ w[1:4]) - (1.0 + 0.5 * 1.4 * vinf * vinf)) < 1e-15
+        # divergence by central differences at a point off the body
+        x = (0.8 + rng.random()) * d
+        h = 1e-5
+        div = 0.0
+        for k in range(3 if three_d else 2):
+            e = np.zeros(3)
+            e[k] = h
+            div += (O.potential_flow(x + e, three_d)[1 + k] - O.potential_flow(x - e, three_d)[1 + k]) / (2 * h)
+        assert abs(div) < 1e-8 * vinf
+    if three_d:
+        far = O.potential_flow(np.array([0.0, 0.0, 1e4]), True)
+        assert np.allclose(far[1:4], [vinf, 0.0, 0.0], rtol=1e-9, atol=1e-18)
