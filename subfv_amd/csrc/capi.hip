@@ -380,7 +380,9 @@ int launch_gather(subfv_ctx *c, const double *P, const double *P2, double *rhs, 
   g.fixed_dt = fixed_dt;
   Prof pf(c, "gather");
   const int nb = (int)nblk(c->dm.C, kCellThreads);
-  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kCellThreads), 0, c->stream, g);
+  g.nblocks = nb;
+  g.per_xcd = (nb + 7) / 8;
+  hipLaunchKernelGGL(k_gather, dim3(8 * g.per_xcd), dim3(kCellThreads), 0, c->stream, g);
   hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(kCellThreads), 0, c->stream, (const double *)c->blk, nb, c->st,
                      fixed_dt);
   HIPCHK(c, hipGetLastError());
@@ -400,7 +402,9 @@ int launch_grad(subfv_ctx *c, const double *U, int method) {
   g.nblocks = nb;
   Prof pf(c, "grad");
   hipLaunchKernelGGL(k_nodal_grad, dim3(8 * g.per_xcd), dim3(kGrpThreads), 0, c->stream, g);
-  hipLaunchKernelGGL(k_cell_grad, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, g);
+  g.cblocks = (int)nblk(c->dm.C, kCellThreads);
+  g.cper_xcd = (g.cblocks + 7) / 8;
+  hipLaunchKernelGGL(k_cell_grad, dim3(8 * g.cper_xcd), dim3(kCellThreads), 0, c->stream, g);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -498,8 +502,12 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
   a.rdiff = c->rdiff;
   a.diag = c->diag;
   Prof pf(c, "diffusion");
-  hipLaunchKernelGGL(k_diff_apply, dim3(nblk(c->dm.V, kDiffWaves)), dim3(64 * kDiffWaves), 0, c->stream, a);
-  hipLaunchKernelGGL(k_diff_gather, dim3(nblk(c->dm.C, 256)), dim3(256), 0, c->stream, a);
+  a.vblocks = (int)nblk(c->dm.V, kDiffWaves);
+  a.vper_xcd = (a.vblocks + 7) / 8;
+  a.cblocks = (int)nblk(c->dm.C, 256);
+  a.cper_xcd = (a.cblocks + 7) / 8;
+  hipLaunchKernelGGL(k_diff_apply, dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0, c->stream, a);
+  hipLaunchKernelGGL(k_diff_gather, dim3(8 * a.cper_xcd), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }

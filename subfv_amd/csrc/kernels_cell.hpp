@@ -113,6 +113,8 @@ struct GatherArgs {
   double *blk;       // per-block minima
   StepState *st;
   int fixed_dt;      // keep st->dt (SSP-RK3 stages 2 and 3)
+  int per_xcd;       // logical cell blocks per XCD slice
+  int nblocks;       // logical cell blocks
 };
 
 // rhs(:,c) = -sum of the partials of c's vertices in increasing vertex id,
@@ -147,7 +149,13 @@ SFV_DEV void gather_block(const GatherArgs &a, long bidx) {
   }
   block_min_store(local, a.blk, (int)bidx);
 }
-__global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) { gather_block(a, blockIdx.x); }
+// Consecutive cells share the cache lines of their vertices' slots, so each
+// XCD takes a contiguous range of cell blocks.
+__global__ __launch_bounds__(kCellThreads) void k_gather(GatherArgs a) {
+  const int lb = xcd_block(a.per_xcd);
+  if (lb >= a.nblocks) return;  // uniform over the workgroup
+  gather_block(a, lb);
+}
 
 struct UpdateArgs {
   DevMesh m;
@@ -270,7 +278,8 @@ struct GradArgs {
   double *VMM;       // [V][4] rho_min, rho_max, p_min, p_max over the vertex's cells
   double *G;         // [C][15]
   int method;
-  int per_xcd, nblocks;
+  int per_xcd, nblocks;    // vertex groups (k_nodal_grad)
+  int cper_xcd, cblocks;   // cell blocks (k_cell_grad)
 };
 
 struct GradLds {
@@ -371,7 +380,9 @@ SFV_DEV double oi3(const double *g) { return sqrt(g[0] * g[0] + g[1] * g[1] + g[
 // (:129-164) and relaxed_maximum_principle (:61-127)
 __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
   const DevMesh &m = a.m;
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block(a.cper_xcd);  // neighbouring cells share their vertices' gradients
+  if (lb >= a.cblocks) return;
+  long e = (long)lb * blockDim.x + threadIdx.x;
   if (e >= m.C) return;
   const long C = m.C;
   double g[15];

@@ -19,6 +19,7 @@
 // diag[C][25] (the point-implicit block).
 #pragma once
 #include "device_mesh.hpp"
+#include "kernels_flux.hpp"
 #include "physics.hpp"
 
 namespace sfv {
@@ -58,6 +59,8 @@ struct DiffArgs {
   double *PD;         // [SE][18] node-sorted slot partials
   double *rdiff;      // [C][4]
   double *diag;       // [C][25]
+  int vper_xcd, vblocks;  // XCD slices of the vertex blocks (apply)
+  int cper_xcd, cblocks;  // and of the cell blocks (gather)
 };
 
 SFV_DEV void wsync_d() {
@@ -404,7 +407,9 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
   __shared__ DiffApplyLds lds_all[kDiffWaves];
   const DevMesh &m = a.m;
   const int wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long v = (long)blockIdx.x * kDiffWaves + wib;
+  const int lb = xcd_block(a.vper_xcd);  // neighbouring vertices share cells
+  if (lb >= a.vblocks) return;
+  const long v = (long)lb * kDiffWaves + wib;
   if (v >= m.V) return;  // whole wavefront
   DiffApplyLds &L = lds_all[wib];
   const int sf0 = m.sf_off[v], nf = m.sf_off[v + 1] - sf0;
@@ -516,7 +521,9 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
 // rdiff and diag of each cell: its slots' rows in increasing vertex id
 __global__ __launch_bounds__(256) void k_diff_gather(DiffArgs a) {
   const DevMesh &m = a.m;
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block(a.cper_xcd);
+  if (lb >= a.cblocks) return;
+  const long c = (long)lb * blockDim.x + threadIdx.x;
   if (c >= m.C) return;
   const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
   double r[4] = {0, 0, 0, 0}, d[14];
