@@ -380,6 +380,10 @@ SFV_DEV void dtheta_du(const double *u, double cv, double *P) {
 }
 
 constexpr int kDiffWaves = 4;
+// slot-term data staged in LDS per vertex: sb and blcs (12 ns), snsh, sbh and
+// b1 (ns^2 + 2 ns), the diagonal 3x3 blocks of S~^T N^-1 S (9 ns); vertices
+// with more sub-elements read them from the block in HBM instead
+constexpr int kDiffStage = 320;
 
 struct DiffApplyLds {
   double Ve[kDiffMaxF], Vf[kDiffMaxF], SVe[kDiffMaxF];
@@ -388,6 +392,7 @@ struct DiffApplyLds {
   int le[kDiffMaxSub];         // left cell of each sub-face
   int cell[kDiffMaxSub];       // cell of each slot
   int ssf[kDiffMaxSub][kSlotSrc + 1];  // local sub-faces of each slot's sub-element, count first
+  double pc[kDiffStage];       // the slot terms' block data, staged while the row pass streams
 };
 
 // One wavefront per vertex: the scatter terms of
@@ -446,6 +451,24 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
     for (int k = 0; k < 4; k++) L.nrm[lane][k] = m.sfn[k * m.SF + s];
     L.le[lane] = m.sf_le[s];
   }
+  // stage the slot terms' block data: the loads overlap the row pass below
+  const bool staged = ns * ns + 23 * ns <= kDiffStage;
+  const int o_tail = 12 * ns, o_diag = 12 * ns + ns * ns + 2 * ns;
+  if (staged) {
+    for (int t = lane; t < 12 * ns; t += 64) L.pc[t] = sb[t];  // sb then blcs, contiguous
+    for (int t = lane; t < ns * ns + 2 * ns; t += 64) L.pc[o_tail + t] = snsh[t];  // snsh, sbh, b1
+    for (int t = lane; t < 9 * ns; t += 64) {
+      const int l = t / 9, ij = t - 9 * l, i = ij % 3, j = ij / 3;
+      L.pc[o_diag + t] = sns[(3 * l + i) + (long)E * (3 * l + j)];
+    }
+  }
+  const double *c_sb = staged ? L.pc : sb, *c_blcs = staged ? L.pc + 3 * ns : blcs;
+  const double *c_snsh = staged ? L.pc + o_tail : snsh;
+  const double *c_sbh = c_snsh + (long)ns * ns, *c_b1 = c_sbh + ns;
+  // diagonal block entry (i, j) of slot l
+  auto sdiag = [&](int l, int i, int j) -> double {
+    return staged ? L.pc[o_diag + 9 * l + i + 3 * j] : sns[(3 * l + i) + (long)E * (3 * l + j)];
+  };
   wsync_d();
   // ---- (b)
   for (int r = lane; r < F + E; r += 64) {
@@ -466,20 +489,20 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
     const int l = t / 9, ij = t - 9 * l, i = ij % 3, j = ij / 3;
     const double rho = L.rho[l];
     a.PD[(long)kDiffSlot * (se0 + l) + 4 + ij] =
-        -sns[(3 * l + i) + (long)E * (3 * l + j)] / rho + blcs[9 * l + j + 3 * i] / rho;
+        -sdiag(l, i, j) / rho + c_blcs[9 * l + j + 3 * i] / rho;
   }
   // momentum rhs, 3 per slot
   for (int t = lane; t < 3 * ns; t += 64) {
     const int l = t / 3, i = t - 3 * l;
     double mv = 0.0;
-    for (int j = 0; j < 3; j++) mv = mv + sns[(3 * l + i) + (long)E * (3 * l + j)] * L.Ve[3 * l + j];
-    a.PD[(long)kDiffSlot * (se0 + l) + i] = (L.SVe[3 * l + i] - mv) + sb[3 * l + i];
+    for (int j = 0; j < 3; j++) mv = mv + sdiag(l, i, j) * L.Ve[3 * l + j];
+    a.PD[(long)kDiffSlot * (se0 + l) + i] = (L.SVe[3 * l + i] - mv) + c_sb[3 * l + i];
   }
   // heat row of the diagonal, 5 per slot
   for (int t = lane; t < 5 * ns; t += 64) {
     const int l = t / 5, k = t - 5 * l;
-    const double sll = snsh[l + (long)ns * l];
-    a.PD[(long)kDiffSlot * (se0 + l) + 13 + k] = (b1[l] - sll) * L.P[l][k];
+    const double sll = c_snsh[l + (long)ns * l];
+    a.PD[(long)kDiffSlot * (se0 + l) + 13 + k] = (c_b1[l] - sll) * L.P[l][k];
   }
   // viscous work and heat rhs, one lane per slot
   if (lane < ns) {
@@ -510,10 +533,10 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
       work = work + L.nrm[lo][3] * (sv[0] * nl[0] + sv[1] * nl[1] + sv[2] * nl[2]);
     }
     double h = 0.0;
-    for (int p = 0; p < ns; p++) h = h + snsh[l + (long)ns * p] * L.th[p];
-    const double sll = snsh[l + (long)ns * l];
+    for (int p = 0; p < ns; p++) h = h + c_snsh[l + (long)ns * p] * L.th[p];
+    const double sll = c_snsh[l + (long)ns * l];
     const double pu = L.pu[l];
-    h = h - sll * pu + (b1[l] - sll) * pu - b1[l] * L.th[l] + sbh[l];
+    h = h - sll * pu + (c_b1[l] - sll) * pu - c_b1[l] * L.th[l] + c_sbh[l];
     a.PD[(long)kDiffSlot * (se0 + l) + 3] = work + h;
   }
 }
