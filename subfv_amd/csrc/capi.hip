@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -21,8 +22,19 @@
 
 using namespace sfv;
 
+// Device-only tables derived from a host mesh (right-neighbour codes, slot
+// cells, the cells' slots in vertex order, vertex groups and their slot
+// sources), built once with the host mesh so an upload only copies.
+struct DevTables {
+  std::vector<int> sf_re, se_cell, c_slot, grp;
+  std::vector<uint32_t> se_src;
+  int ngroups = 0;
+  std::string err;
+};
+
 struct subfv_host_mesh {
-  subfv::HostMesh hm;
+  std::shared_ptr<subfv::HostMesh> hm;
+  std::shared_ptr<DevTables> tab;
 };
 
 // A batch of contexts advanced together by multi-instance launches.
@@ -46,11 +58,15 @@ struct subfv_ctx {
   std::string err;
   bool has_mesh = false;
   bool b2d = false;
-  subfv::HostMesh hm;
+  std::shared_ptr<const subfv::HostMesh> hmp;  // shared with the host mesh handle it came from
+  std::shared_ptr<const DevTables> tab;
   DevMesh dm;
   BCParams bc{};
   subfv_config cfg{};
-  std::vector<void *> allocs;  // mesh-lifetime device buffers
+  // mesh-lifetime device buffers: bump allocations in device chunks that
+  // stay allocated when the mesh changes (an upload re-uses them)
+  std::vector<std::pair<char *, size_t>> chunks;
+  size_t chunk_i = 0, chunk_off = 0;
   // state and work arrays
   double *U[3] = {nullptr, nullptr, nullptr};
   int cur = 0;
@@ -85,14 +101,25 @@ int fail(subfv_ctx *c, int code, const std::string &msg) {
       return fail((c), SUBFV_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// Mesh-lifetime buffer of n elements, 256-byte aligned, taken from the
+// context's chunks (a new chunk of at least 64 MiB when none has room).
 template <class T>
 int dalloc(subfv_ctx *c, T **p, size_t n) {
-  void *q = nullptr;
-  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess) return fail(c, SUBFV_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-  c->allocs.push_back(q);
-  *p = (T *)q;
+  const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+  while (c->chunk_i < c->chunks.size() && c->chunk_off + bytes > c->chunks[c->chunk_i].second) {
+    c->chunk_i++;
+    c->chunk_off = 0;
+  }
+  if (c->chunk_i == c->chunks.size()) {
+    const size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, sz);
+    if (e != hipSuccess) return fail(c, SUBFV_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->chunks.push_back({(char *)q, sz});
+    c->chunk_off = 0;
+  }
+  *p = (T *)(c->chunks[c->chunk_i].first + c->chunk_off);
+  c->chunk_off += bytes;
   return 0;
 }
 
@@ -104,9 +131,12 @@ int upload(subfv_ctx *c, T **p, const std::vector<T> &v) {
   return 0;
 }
 
+// Forget the mesh; its chunks stay allocated for the next one, so the work
+// already queued on the stream must finish first.
 void free_mesh(subfv_ctx *c) {
-  for (void *p : c->allocs) (void)hipFree(p);
-  c->allocs.clear();
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->chunk_i = 0;
+  c->chunk_off = 0;
   c->dm = DevMesh();
   c->has_mesh = false;
   for (auto &u : c->U) u = nullptr;
@@ -191,8 +221,35 @@ std::string pack_groups(const subfv::HostMesh &hm, const std::vector<int> &secel
   return "";
 }
 
+// the device-only tables of a host mesh (see DevTables)
+std::shared_ptr<DevTables> make_tables(const subfv::HostMesh &h) {
+  auto t = std::make_shared<DevTables>();
+  t->sf_re.resize(h.nsf);
+  for (int s = 0; s < h.nsf; s++) t->sf_re[s] = h.sf_re[s] - 1;  // device right-neighbour code
+  t->se_cell.resize(h.nse);
+  for (int q = 0; q < h.nse; q++) t->se_cell[q] = h.se_elem[h.v_se[q]];
+  // each cell's sub-elements in increasing vertex id (the scatter order of
+  // compute_euler_flux, ns_euler_module.F90:321-336), as node-sorted slots:
+  // the gather reads the node sweep's per-slot partials through this table
+  std::vector<int> slot_of(h.nse);
+  for (int q = 0; q < h.nse; q++) slot_of[h.v_se[q]] = q;
+  t->c_slot.resize(h.nse);
+  for (int e = 0; e < h.ne; e++) {
+    int b = h.e_voff[e], n = h.e_voff[e + 1] - b;
+    std::pair<int, int> pv[8];
+    for (int j = 0; j < n; j++) pv[j] = {h.se_vert[b + j], slot_of[b + j]};
+    std::sort(pv, pv + n);
+    for (int j = 0; j < n; j++) t->c_slot[b + j] = pv[j].second;
+  }
+  t->err = pack_groups(h, t->se_cell, t->grp, t->se_src);
+  t->ngroups = (int)(t->grp.size() / kGrpHdr);
+  return t;
+}
+
 int upload_mesh(subfv_ctx *c) {
-  const subfv::HostMesh &h = c->hm;
+  const subfv::HostMesh &h = *c->hmp;
+  const DevTables &t = *c->tab;
+  if (!t.err.empty()) return fail(c, SUBFV_E_INVALID, t.err);
   DevMesh &d = c->dm;
   d.V = h.nv;
   d.C = h.ne;
@@ -217,44 +274,19 @@ int upload_mesh(subfv_ctx *c) {
   UP(sf_vert, h.sf_vert);
   UP(sf_face, h.sf_face);
   UP(sf_le, h.sf_le);
-  std::vector<int> re(h.nsf);
-  for (int s = 0; s < h.nsf; s++) re[s] = h.sf_re[s] - 1;  // device right-neighbour code
-  UP(sf_re, re);
+  UP(sf_re, t.sf_re);
   UP(se_sfoff, h.se_sfoff);
   UP(se_sf, h.se_sf);
   UP(se_elem, h.se_elem);
-  std::vector<int> secell(h.nse);
-  for (int q = 0; q < h.nse; q++) secell[q] = h.se_elem[h.v_se[q]];
-  UP(se_cell, secell);
+  UP(se_cell, t.se_cell);
   UP(e_voff, h.e_voff);
   UP(e_vert, h.e_vert);
   UP(e_foff, h.e_foff);
   UP(e_face, h.e_face);
-  // each cell's sub-elements in increasing vertex id (the scatter order of
-  // compute_euler_flux, ns_euler_module.F90:321-336), as node-sorted slots:
-  // the gather reads the node sweep's per-slot partials through this table
-  {
-    std::vector<int> slot_of(h.nse);
-    for (int q = 0; q < h.nse; q++) slot_of[h.v_se[q]] = q;
-    std::vector<int> cslot(h.nse);
-    for (int e = 0; e < h.ne; e++) {
-      int b = h.e_voff[e], n = h.e_voff[e + 1] - b;
-      std::pair<int, int> pv[8];
-      for (int j = 0; j < n; j++) pv[j] = {h.se_vert[b + j], slot_of[b + j]};
-      std::sort(pv, pv + n);
-      for (int j = 0; j < n; j++) cslot[b + j] = pv[j].second;
-    }
-    UP(c_slot, cslot);
-  }
-  {
-    std::vector<int> grp;
-    std::vector<uint32_t> src;
-    std::string e = pack_groups(h, secell, grp, src);
-    if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
-    d.ngroups = (int)(grp.size() / kGrpHdr);
-    UP(grp, grp);
-    UP(se_src, src);
-  }
+  UP(c_slot, t.c_slot);
+  d.ngroups = t.ngroups;
+  UP(grp, t.grp);
+  UP(se_src, t.se_src);
 #undef UP
   if ((r = dalloc(c, &d.vvol, h.nv))) return r;
   if ((r = dalloc(c, &d.vbound, h.nv))) return r;
@@ -449,7 +481,7 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
 // buffers, and the setup kernel (nodal systems, inverses, products).
 int ensure_diffusion(subfv_ctx *c) {
   if (c->diff_ready) return 0;
-  const subfv::HostMesh &h = c->hm;
+  const subfv::HostMesh &h = *c->hmp;
   std::vector<long> off(h.nv + 1, 0);
   for (int v = 0; v < h.nv; v++) {
     const int nf = h.sf_off[v + 1] - h.sf_off[v], ns = h.v_seoff[v + 1] - h.v_seoff[v];
@@ -661,6 +693,8 @@ void subfv_destroy(subfv_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_mesh(c);
+  for (auto &ch : c->chunks) (void)hipFree(ch.first);
+  c->chunks.clear();
   if (c->derr) (void)hipFree(c->derr);
   if (c->st) (void)hipFree(c->st);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -674,20 +708,22 @@ int subfv_host_mesh_build(subfv_host_mesh **out, int32_t n_vert, const double *c
                           const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag) {
   if (!out || !coord || !kind || !elem_vert) return SUBFV_E_INVALID;
   auto *m = new subfv_host_mesh();
-  std::string e = subfv::build_host_mesh(m->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
+  m->hm = std::make_shared<subfv::HostMesh>();
+  std::string e = subfv::build_host_mesh(*m->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
                                          bf_vert, bf_tag);
   if (!e.empty()) {
     delete m;
     *out = nullptr;
     return SUBFV_E_INVALID;
   }
+  m->tab = make_tables(*m->hm);
   *out = m;
   return 0;
 }
 
 int64_t subfv_host_mesh_get_i32(subfv_host_mesh *m, const char *name, int32_t *out, int64_t cap) {
   if (!m || !name || !out) return -1;
-  return mesh_i32(m->hm, name, out, cap);
+  return mesh_i32(*m->hm, name, out, cap);
 }
 
 void subfv_host_mesh_free(subfv_host_mesh *m) { delete m; }
@@ -699,9 +735,12 @@ int subfv_mesh_build(subfv_ctx *c, int32_t n_vert, const double *coord, int32_t 
   if (n_bfaces > 0 && (!bf_nvert || !bf_vert || !bf_tag)) return fail(c, SUBFV_E_INVALID, "null boundary array");
   HIPCHK(c, hipSetDevice(c->device));
   free_mesh(c);
-  std::string e = subfv::build_host_mesh(c->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
+  auto hm = std::make_shared<subfv::HostMesh>();
+  std::string e = subfv::build_host_mesh(*hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
                                          bf_vert, bf_tag);
   if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
+  c->tab = make_tables(*hm);
+  c->hmp = hm;
   return finish_mesh(c, boundary_2d);
 }
 
@@ -709,19 +748,20 @@ int subfv_mesh_upload(subfv_ctx *c, const subfv_host_mesh *hm, int32_t boundary_
   if (!c || !hm) return fail(c, SUBFV_E_INVALID, "null argument");
   HIPCHK(c, hipSetDevice(c->device));
   free_mesh(c);
-  c->hm = hm->hm;
+  c->hmp = hm->hm;
+  c->tab = hm->tab;
   return finish_mesh(c, boundary_2d);
 }
 
 int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
   if (!c || !c->has_mesh || !what) return -1;
   std::string w(what);
-  if (w == "n_vert") return c->hm.nv;
-  if (w == "n_elems") return c->hm.ne;
-  if (w == "n_faces") return c->hm.nf;
-  if (w == "n_sub_faces") return c->hm.nsf;
-  if (w == "n_sub_elems") return c->hm.nse;
-  if (w == "n_max_sub_faces") return c->hm.nmax_sfn;
+  if (w == "n_vert") return c->hmp->nv;
+  if (w == "n_elems") return c->hmp->ne;
+  if (w == "n_faces") return c->hmp->nf;
+  if (w == "n_sub_faces") return c->hmp->nsf;
+  if (w == "n_sub_elems") return c->hmp->nse;
+  if (w == "n_max_sub_faces") return c->hmp->nmax_sfn;
   if (w == "n_groups") return c->dm.ngroups;
   return -1;
 }
@@ -729,13 +769,13 @@ int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
 int64_t subfv_mesh_get_i32(subfv_ctx *c, const char *name, int32_t *out, int64_t cap) {
   if (!c || !c->has_mesh || !name || !out) return -1;
   if (std::string(name) == "vert_is_bound") {
-    std::vector<uint8_t> b(c->hm.nv);
+    std::vector<uint8_t> b(c->hmp->nv);
     if (hipMemcpy(b.data(), c->dm.vbound, b.size(), hipMemcpyDeviceToHost) != hipSuccess) return -1;
-    if (cap < c->hm.nv) return -1;
-    for (int i = 0; i < c->hm.nv; i++) out[i] = b[i];
-    return c->hm.nv;
+    if (cap < c->hmp->nv) return -1;
+    for (int i = 0; i < c->hmp->nv; i++) out[i] = b[i];
+    return c->hmp->nv;
   }
-  return mesh_i32(c->hm, name, out, cap);
+  return mesh_i32(*c->hmp, name, out, cap);
 }
 
 int64_t subfv_mesh_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) {
@@ -749,7 +789,7 @@ int64_t subfv_mesh_get_f64(subfv_ctx *c, const char *name, double *out, int64_t 
   };
   if (n == "vert_coord") {
     if (3L * d.V > cap) return -1;
-    std::copy(c->hm.vcoord.begin(), c->hm.vcoord.end(), out);
+    std::copy(c->hmp->vcoord.begin(), c->hmp->vcoord.end(), out);
     return 3L * d.V;
   }
   if (n == "vert_volume") return cp(d.vvol, d.V);
