@@ -90,11 +90,19 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24):
     upload, geometry, initial state, the steps and the read-back are timed."""
     from subfv_amd import serve as S
 
+    from subfv_amd.api import Solver
+
     reqs = S.prepare(S.make_requests(n_req, rate=rate, steps=steps))
     calib = S.prepare(S.make_requests(calib_n, rate=1e9, steps=30, seed=S.SEED + 1))
     best, trials = S.autotune(calib)
-    plain = S.summarize(*S.serve_plain(reqs))
-    served = S.summarize(*S.serve(reqs, best))
+    # contexts are created before the timed runs, as a server holds them
+    one = Solver(0)
+    plain = S.summarize(*S.serve_plain(reqs, solver=one))
+    one.close()
+    pool = [Solver(0) for _ in range(best.max_batch)]
+    served = S.summarize(*S.serve(reqs, best, pool=pool))
+    for s in pool:
+        s.close()
     cells = sum(r.cells for r in reqs)
     return {
         "requests": n_req, "steps_per_request": steps, "arrival_rate_per_s": rate,

@@ -132,6 +132,10 @@ int subfv_set_config(subfv_ctx *ctx, const subfv_config *cfg);
 int subfv_set_time(subfv_ctx *ctx, double t);
 
 int subfv_state_put(subfv_ctx *ctx, const double *sol);
+/* init_sol with init_1drp (src/ns/ns_euler_module.F90:36-43) on the device:
+ * the state of each cell is primit_to_conserv(w_left) where its centroid's x
+ * is below x1drp, primit_to_conserv(w_right) elsewhere ((rho, u, v, w, p)). */
+int subfv_init_1drp(subfv_ctx *ctx, double x1drp, const double *w_left, const double *w_right);
 int subfv_state_get(subfv_ctx *ctx, double *sol);
 
 /* Device arrays in reference layout, by name: "sol", "rhs", "rhs_o2",

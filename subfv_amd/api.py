@@ -148,6 +148,12 @@ class Solver:
         s = _lib.f64(sol).reshape(self.n_elems, 5)
         self._check(_lib.lib().subfv_state_put(self.h, _lib.dptr(s)))
 
+    def init_1drp(self, x1drp: float, wl, wr):
+        """init_1drp on the device: left/right primitive states by centroid x."""
+        wl = _lib.f64(wl).reshape(5)
+        wr = _lib.f64(wr).reshape(5)
+        self._check(_lib.lib().subfv_init_1drp(self.h, float(x1drp), _lib.dptr(wl), _lib.dptr(wr)))
+
     def get_sol(self) -> np.ndarray:
         s = np.zeros((self.n_elems, 5))
         self._check(_lib.lib().subfv_state_get(self.h, _lib.dptr(s)))

@@ -102,7 +102,7 @@ int fail(subfv_ctx *c, int code, const std::string &msg) {
   } while (0)
 
 // Mesh-lifetime buffer of n elements, 256-byte aligned, taken from the
-// context's chunks (a new chunk of at least 64 MiB when none has room).
+// context's chunks (a new chunk of at least 16 MiB when none has room).
 template <class T>
 int dalloc(subfv_ctx *c, T **p, size_t n) {
   const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
@@ -111,7 +111,7 @@ int dalloc(subfv_ctx *c, T **p, size_t n) {
     c->chunk_off = 0;
   }
   if (c->chunk_i == c->chunks.size()) {
-    const size_t sz = std::max<size_t>(bytes, (size_t)64 << 20);
+    const size_t sz = std::max<size_t>(bytes, (size_t)16 << 20);
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, sz);
     if (e != hipSuccess) return fail(c, SUBFV_E_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -846,6 +846,22 @@ int subfv_set_time(subfv_ctx *c, double t) {
 int subfv_state_put(subfv_ctx *c, const double *sol) {
   if (!c || !c->has_mesh || !sol) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
   return host_to_dev_soa(c, sol, 1, 5 * c->dm.C, c->U[c->cur]);
+}
+
+// init_sol with init_1drp (ns_euler_module.F90:36-43) on the device: each
+// cell takes the left or right primitive state by its centroid's x
+int subfv_init_1drp(subfv_ctx *c, double x1drp, const double *w_left, const double *w_right) {
+  if (!c || !c->has_mesh || !w_left || !w_right) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
+  Init1drp a;
+  a.x1drp = x1drp;
+  for (int k = 0; k < 5; k++) {
+    a.wl[k] = w_left[k];
+    a.wr[k] = w_right[k];
+  }
+  hipLaunchKernelGGL(k_init_1drp, dim3(nblk(c->dm.C, kCellThreads)), dim3(kCellThreads), 0, c->stream, c->dm, a,
+                     c->U[c->cur]);
+  HIPCHK(c, hipGetLastError());
+  return 0;
 }
 
 int subfv_state_get(subfv_ctx *c, double *sol) {

@@ -26,6 +26,8 @@ struct InstDesc {
   int pad;
 };
 
+typedef const InstDesc __attribute__((address_space(4))) CInstDesc;
+
 constexpr int kTabShift = 20;  // table entry = instance << 20 | block within the instance
 constexpr int kTabMask = (1 << kTabShift) - 1;
 
@@ -36,14 +38,18 @@ __global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux_multi(const InstDe
   const int lb = xcd_block(per_xcd);
   if (lb >= total) return;
   const int e = gtab[lb];
-  const InstDesc &D = d[e >> kTabShift];
+  // the descriptors do not change during a launch: reading them through the
+  // constant address space lets the compiler keep them across the sweep's
+  // barriers instead of re-reading them
+  const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
   node_flux_group<SCHEME>(D.fa, D.U[(D.cur0 + phase) % 3], e & kTabMask, L);
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_gather_multi(const InstDesc *d, const int *ctab, int total) {
   if ((int)blockIdx.x >= total) return;
   const int e = ctab[blockIdx.x];
-  gather_block(d[e >> kTabShift].ga, e & kTabMask);
+  const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
+  gather_block(D.ga, e & kTabMask);
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_finalize_dt_multi(const InstDesc *d) {
@@ -55,7 +61,7 @@ __global__ __launch_bounds__(kCellThreads) void k_update_multi(const InstDesc *d
                                                                int phase) {
   if ((int)blockIdx.x >= total) return;
   const int e = ctab[blockIdx.x];
-  const InstDesc &D = d[e >> kTabShift];
+  const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
   update_block<false>(D.ua, D.U[(D.cur0 + phase) % 3], D.U[(D.cur0 + phase + 1) % 3], e & kTabMask);
 }
 
@@ -68,7 +74,7 @@ __global__ __launch_bounds__(kCellThreads) void k_update_fixup_multi(const InstD
                                                                      int phase) {
   if ((int)blockIdx.x >= total) return;
   const int e = ctab[blockIdx.x];
-  const InstDesc &D = d[e >> kTabShift];
+  const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
   update_fixup_block<false>(D.ua, D.U[(D.cur0 + phase) % 3], D.U[(D.cur0 + phase + 1) % 3], e & kTabMask);
 }
 

@@ -103,6 +103,20 @@ __global__ __launch_bounds__(kCellThreads) void k_finalize_omega(const double *b
   finalize_omega(blk, nblk, st, advance_t);
 }
 
+struct Init1drp {
+  double x1drp, wl[5], wr[5];
+};
+
+// init_1drp (ns_euler_module.F90:36-43): U = primit_to_conserv of the left
+// state where the centroid's x < x1drp, of the right state elsewhere
+__global__ __launch_bounds__(kCellThreads) void k_init_1drp(DevMesh m, Init1drp a, double *U) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  double u[5];
+  to_cons(m.ccoord[3 * c] < a.x1drp ? a.wl : a.wr, u);
+  for (int k = 0; k < 5; k++) U[5 * c + k] = u[k];
+}
+
 struct GatherArgs {
   DevMesh m;
   const double *P;   // first-order partials [SE][6], node-sorted slots

@@ -91,9 +91,7 @@ def admit(s: Solver, r: Request):
     """Upload the mesh, set the configuration and the init_1drp state."""
     s.upload_mesh(r.host, r.boundary_2d)
     s.set_config(config_for(r))
-    xc = s.elem_coord
-    w = np.where((xc[:, 0] < r.x_split)[:, None], r.wl[None, :], r.wr[None, :])
-    s.put_sol(mg.primit_to_conserv(w))
+    s.init_1drp(r.x_split, r.wl, r.wr)
     s.set_time(0.0)
 
 
@@ -144,11 +142,14 @@ class Params:
 
 
 def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep: bool = False,
-          realtime: bool = True) -> tuple[list[Result], float]:
+          realtime: bool = True, pool: list | None = None) -> tuple[list[Result], float]:
     """Continuous batching.  Returns the results (latency = finish - arrival)
-    and the makespan in seconds."""
+    and the makespan in seconds.  `pool`: idle contexts to admit into (a
+    server keeps them between calls; they are returned to it); without one,
+    contexts are created as needed and closed at the end."""
     batch = Batch(device)
-    pool: list[Solver] = []
+    own = pool is None
+    pool = [] if own else pool
     pending = sorted(reqs, key=lambda r: r.arrival)
     pi = 0
     active = []  # [request, solver, remaining]
@@ -204,17 +205,20 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
             active = still
     finally:
         batch.close()
-        for s in pool:
-            s.close()
+        makespan = time.perf_counter() - t0
         for a in active + staged:
-            a[1].close()
-    return results, time.perf_counter() - t0
+            pool.append(a[1])
+        if own:
+            for s in pool:
+                s.close()
+    return results, makespan
 
 
 def serve_plain(reqs: list[Request], device: int = 0, keep: bool = False,
-                realtime: bool = True) -> tuple[list[Result], float]:
-    """One solve at a time in arrival order, each with its own launches."""
-    s = Solver(device)
+                realtime: bool = True, solver: Solver | None = None) -> tuple[list[Result], float]:
+    """One solve at a time in arrival order, each with its own launches
+    (on `solver` if given, kept open)."""
+    s = solver if solver is not None else Solver(device)
     results = []
     t0 = time.perf_counter()
     try:
@@ -229,8 +233,10 @@ def serve_plain(reqs: list[Request], device: int = 0, keep: bool = False,
             t = time.perf_counter() - t0
             results.append(Result(r.rid, t - (r.arrival if realtime else 0.0), t, sol))
     finally:
-        s.close()
-    return results, time.perf_counter() - t0
+        makespan = time.perf_counter() - t0
+        if solver is None:
+            s.close()
+    return results, makespan
 
 
 def summarize(results: list[Result], makespan: float) -> dict:
@@ -246,9 +252,12 @@ def autotune(sample: list[Request], device: int = 0, candidates=None) -> tuple[P
         candidates = [Params(chunk=c, max_batch=b, use_graph=g)
                       for c in (15, 30, 60) for b in (64, 256) for g in (False, True)]
     trials = []
+    pool = [Solver(device) for _ in range(max(p.max_batch for p in candidates))]
     for p in candidates:
-        _, ms = serve(sample, p, device, realtime=False)
+        _, ms = serve(sample, p, device, realtime=False, pool=pool)
         trials.append({"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
                        "solves_per_s": len(sample) / ms})
+    for s in pool:
+        s.close()
     best = max(range(len(trials)), key=lambda i: trials[i]["solves_per_s"])
     return candidates[best], trials

@@ -5,7 +5,9 @@ instance's arithmetic is that of subfv_run)."""
 import numpy as np
 import pytest
 
+from subfv_amd import meshgen as mg
 from subfv_amd import serve as S
+from subfv_amd.api import Solver
 
 pytestmark = pytest.mark.gpu
 
@@ -37,3 +39,16 @@ def test_autotuner_picks_a_candidate(requests):
                                                          S.Params(chunk=9, max_batch=4, use_graph=True)])
     assert len(trials) == 2
     assert best.chunk in (3, 9)
+
+
+def test_init_1drp_on_the_device_equals_the_host_state():
+    print("[gpu] init_1drp", flush=True)
+    mesh = mg.tri_mesh(9)
+    s = Solver(0)
+    s.build_mesh(mesh)
+    wl, wr = np.array([1.0, 0.1, -0.2, 0.0, 1.0]), np.array([0.125, 0.0, 0.3, 0.0, 0.1])
+    s.init_1drp(0.45, wl, wr)
+    got = s.get_sol()
+    ref = mg.init_1drp(s.elem_coord, 0.45, wl, wr)
+    s.close()
+    assert np.array_equal(got, ref)
