@@ -81,7 +81,7 @@ def cpu_baseline(n=192, steps=1, viscous=True):
                       + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid")}
 
 
-def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24):
+def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cmax=512 ** 2):
     """Served mixed-size first-order solves (quad/tri 50/50, 32^2..512^2
     cells log-uniform, seeded discontinuities, `steps` steps each, Poisson
     arrivals at `rate`/s): continuous batching with autotuned parameters
@@ -92,8 +92,8 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24):
 
     from subfv_amd.api import Solver
 
-    reqs = S.prepare(S.make_requests(n_req, rate=rate, steps=steps))
-    calib = S.prepare(S.make_requests(calib_n, rate=1e9, steps=30, seed=S.SEED + 1))
+    reqs = S.prepare(S.make_requests(n_req, rate=rate, steps=steps, cmin=cmin, cmax=cmax))
+    calib = S.prepare(S.make_requests(calib_n, rate=1e9, steps=30, seed=S.SEED + 1, cmin=cmin, cmax=cmax))
     best, trials = S.autotune(calib)
     # contexts are created before the timed runs, as a server holds them
     one = Solver(0)
@@ -106,6 +106,7 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24):
     cells = sum(r.cells for r in reqs)
     return {
         "requests": n_req, "steps_per_request": steps, "arrival_rate_per_s": rate,
+        "cells_range": [cmin, cmax],
         "mean_cells": cells / n_req, "cell_steps": cells * steps,
         "served": served, "plain": plain,
         "throughput_vs_plain": served["solves_per_s"] / plain["solves_per_s"],
@@ -233,6 +234,9 @@ def main():
     s.close()
     if args.served_requests > 0 and not args.first_order:
         out["served"] = served_bench(args.served_requests)
+        # the same loop on small meshes (32^2 .. 96^2 cells), where one solve
+        # alone cannot fill the GPU and launches dominate the plain path
+        out["served_small"] = served_bench(2 * args.served_requests, rate=4000.0, calib_n=96, cmax=96 ** 2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, viscous=viscous)
     print(json.dumps(out), flush=True)

@@ -139,6 +139,7 @@ class Params:
     max_batch: int = 256     # instances resident at once
     use_graph: bool = True
     budget: int = 96 << 30   # HBM bytes for admitted instances
+    admit_workers: int = 8   # host threads uploading admitted requests
 
 
 def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep: bool = False,
@@ -169,12 +170,18 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
             r = pending[pi]
             pi += 1
             s = pool.pop() if pool else Solver(device)
-            admit(s, r)
             out.append([r, s, r.steps])
             used += device_bytes(r)
+        # each admission works on its own context and stream; the C-ABI calls
+        # release the interpreter lock, so several uploads proceed together
+        if len(out) > 1:
+            list(ex.map(lambda a: admit(a[1], a[0]), out))
+        elif out:
+            admit(out[0][1], out[0][0])
         return out
 
     staged = []
+    ex = ThreadPoolExecutor(max_workers=params.admit_workers)
     try:
         while pi < len(pending) or active or staged:
             active += staged
@@ -204,6 +211,7 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
                     still.append(a)
             active = still
     finally:
+        ex.shutdown()
         batch.close()
         makespan = time.perf_counter() - t0
         for a in active + staged:
