@@ -32,7 +32,7 @@ constexpr int kTabShift = 20;  // table entry = instance << 20 | block within th
 constexpr int kTabMask = (1 << kTabShift) - 1;
 
 template <int SCHEME>
-__global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux_multi(const InstDesc *d, const int *gtab, int total,
+__global__ __launch_bounds__(kGrpThreads, 3) void k_node_flux_multi(const InstDesc *d, const int *gtab, int total,
                                                                    int per_xcd, int phase) {
   __shared__ FluxLds L;
   const int lb = xcd_block(per_xcd);

@@ -370,7 +370,7 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
 }
 
 template <int SCHEME>
-__global__ __launch_bounds__(kGrpThreads, 4) void k_node_flux(FluxArgs a) {
+__global__ __launch_bounds__(kGrpThreads, 3) void k_node_flux(FluxArgs a) {
   __shared__ FluxLds L;
   const int lb = xcd_block(a.per_xcd);
   if (lb >= a.nblocks) return;  // uniform over the workgroup
