@@ -110,7 +110,8 @@ int subfv_mesh_build(subfv_ctx *ctx, int32_t n_vert, const double *coord, int32_
 /* The same from a connectivity built beforehand on the host
  * (subfv_host_mesh_build): upload, device geometry and BC data only. */
 int subfv_mesh_upload(subfv_ctx *ctx, const subfv_host_mesh *hm, int32_t boundary_2d);
-/* "n_vert", "n_elems", "n_faces", "n_sub_faces", "n_sub_elems", "n_max_sub_faces", "n_groups" */
+/* "n_vert", "n_elems", "n_faces", "n_sub_faces", "n_sub_elems", "n_max_sub_faces", "n_groups",
+ * "diff_block_doubles" (size of the packed viscous blocks, 0 before the first viscous residual) */
 int64_t subfv_mesh_count(const subfv_ctx *ctx, const char *what);
 /* Connectivity in the reference numbering (same names as the mesh_type
  * components, e.g. "vert_sub_face", "sub_face_left_elem"); returns the number
@@ -124,6 +125,14 @@ int64_t subfv_mesh_get_f64(subfv_ctx *ctx, const char *name, double *out, int64_
 int subfv_host_mesh_build(subfv_host_mesh **out, int32_t n_vert, const double *coord, int32_t n_elems,
                           const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
                           const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag);
+/* periodic_mesh = .true. of build_mesh (src/mesh/mesh_connectivity_module.F90:44-54):
+ * stitch_periodic_faces (:976-1073) pairs the faces on x, y, z = -0.5 / 0.5
+ * whose other centre coordinates agree within 1e-12, and build_periodic_stencil
+ * (:1075-1328) appends each paired vertex's partner sub-faces and sub-elements to
+ * its lists.  Applies to a built host mesh before subfv_mesh_upload; contexts
+ * that uploaded it before keep the non-periodic one.  The viscous terms and the
+ * Lagrange / acoustic sweeps return SUBFV_E_INVALID on a periodic mesh. */
+int subfv_host_mesh_make_periodic(subfv_host_mesh *m);
 int64_t subfv_host_mesh_get_i32(subfv_host_mesh *m, const char *name, int32_t *out, int64_t cap);
 void subfv_host_mesh_free(subfv_host_mesh *m);
 
@@ -183,7 +192,8 @@ int subfv_run(subfv_ctx *ctx, int32_t n_steps, int32_t time_scheme, double *t, d
 
 /* Kernel timing (no counterpart in the reference): with profiling on, HIP
  * events bracket each phase ("node_flux", "node_flux_o2", "grad", "gather",
- * "update") on the context's stream; read returns the summed milliseconds
+ * "diffusion" and, inside it, "diff_apply", "update") on the context's
+ * stream; read returns the summed milliseconds
  * and the number of launches since profiling was (re)enabled. */
 int subfv_profile_enable(subfv_ctx *ctx, int32_t on);
 int subfv_profile_read(subfv_ctx *ctx, const char *phase, double *ms_total, int64_t *count);

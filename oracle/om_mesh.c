@@ -3,7 +3,8 @@
  *
  * Restatement of the mesh path: the msh 2.2 element-to-face step of
  * src/mesh/mesh_reading_module.F90:1122-1321, build_mesh
- * (src/mesh/mesh_connectivity_module.F90:12-59, non-periodic, one rank) and
+ * (src/mesh/mesh_connectivity_module.F90:12-59, one rank, with the periodic
+ * stitch and stencil of :44-54, 976-1328 when asked for) and
  * compute_geometry_mesh (src/mesh/mesh_geometry_module.F90:16-57).
  * Loop orders and expression forms follow the reference so that ids come
  * out identical and geometry is reproduced operation for operation.
@@ -678,6 +679,186 @@ static void find_if_vert_is_bound(om_mesh *m, int b2d) {
   }
 }
 
+/* ----------------------------------------------------------- periodic */
+#define OM_PTOL 1e-12
+static int on_plane(double x, double c) { return x < c + OM_PTOL && x > c - OM_PTOL; }
+
+/* stitch_periodic_faces, mesh_connectivity_module.F90:976-1073, with the
+ * face centres taken as vertex averages as build_mesh does first (:44-52):
+ * faces on x = -0.5 and x = 0.5 (then y, then z) whose other two centre
+ * coordinates agree within 1e-12 become each other's right neighbours. */
+static int stitch_periodic_faces(om_mesh *m) {
+  double *fc = (double *)malloc(sizeof(double) * 3 * (size_t)(m->nf + 1));
+  for (int i = 1; i <= m->nf; i++) {
+    for (int d = 0; d < 3; d++) fc[3 * i + d] = 0.0;
+    for (int j = 1; j <= m->f_nv[i]; j++)
+      for (int d = 0; d < 3; d++) fc[3 * i + d] = fc[3 * i + d] + VC(m, m->f_vert[i][j], d);
+    for (int d = 0; d < 3; d++) fc[3 * i + d] = fc[3 * i + d] / (double)m->f_nv[i];
+  }
+  int cnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int i = 1; i <= m->nf; i++)
+    for (int d = 0; d < 3; d++) {
+      if (on_plane(fc[3 * i + d], -0.5)) cnt[d][0]++;
+      if (on_plane(fc[3 * i + d], 0.5)) cnt[d][1]++;
+    }
+  for (int d = 0; d < 3; d++)
+    if (cnt[d][0] != cnt[d][1]) { /* "Error periodic boundaries !" */
+      free(fc);
+      return 0;
+    }
+  int *lst[3][2];
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 2; k++) {
+      lst[d][k] = (int *)malloc(sizeof(int) * (size_t)(cnt[d][k] + 1));
+      cnt[d][k] = 0;
+    }
+  for (int i = 1; i <= m->nf; i++)
+    for (int d = 0; d < 3; d++) {
+      if (on_plane(fc[3 * i + d], -0.5)) lst[d][0][++cnt[d][0]] = i;
+      if (on_plane(fc[3 * i + d], 0.5)) lst[d][1][++cnt[d][1]] = i;
+    }
+  for (int d = 0; d < 3; d++) {
+    const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;  /* the two other axes */
+    for (int i = 1; i <= cnt[d][0]; i++)
+      for (int j = 1; j <= cnt[d][1]; j++) {
+        int ii = lst[d][0][i], jj = lst[d][1][j];
+        double da = fc[3 * ii + a] - fc[3 * jj + a], db = fc[3 * ii + b] - fc[3 * jj + b];
+        if (sqrt(da * da + db * db) < OM_PTOL) {
+          m->f_right[ii] = m->f_left[jj];
+          m->f_right[jj] = m->f_left[ii];
+        }
+      }
+  }
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 2; k++) free(lst[d][k]);
+  free(fc);
+  for (int i = 1; i <= m->nsf; i++) {
+    m->sf_le[i] = m->f_left[m->sf_face[i]];
+    m->sf_re[i] = m->f_right[m->sf_face[i]];
+  }
+  return 1;
+}
+
+/* two sub-faces join the same pair of cells, in either orientation */
+static int same_cells(const om_mesh *m, int a, int b) {
+  return (m->sf_le[a] == m->sf_re[b] && m->sf_re[a] == m->sf_le[b]) ||
+         (m->sf_le[a] == m->sf_le[b] && m->sf_re[a] == m->sf_re[b]);
+}
+
+/* process_pair of build_periodic_stencil, mesh_connectivity_module.F90:1223-1326:
+ * v1 takes v2 as its periodic partner.  Returns 0 when the extended list
+ * cannot be filled (a case the reference leaves undefined). */
+static int process_pair(om_mesh *m, int v1, int v2) {
+  /* step 1: right sub-element of v1's stitched sub-faces, from v2's list */
+  for (int ki = 1; ki <= m->v_nsfn[v1]; ki++) {
+    int sfki = m->v_sf[v1][ki], re = m->sf_re[sfki];
+    if (re > 0 && m->sf_rse[sfki] == 0)
+      for (int k = 1; k <= m->v_nsen[v2]; k++) {
+        int id_se = m->v_se[v2][k];
+        if (m->se_elem[id_se] == re) {
+          m->sf_rse[sfki] = id_se;
+          break;
+        }
+      }
+  }
+  /* step 2: v2's sub-faces appended to v1's list, without repeats */
+  int n1 = m->v_nsfn[v1], n2 = m->v_nsfn[v2], nm = 0;
+  for (int ki = 1; ki <= n1; ki++)
+    for (int kii = 1; kii <= n2; kii++)
+      if (same_cells(m, m->v_sf[v1][ki], m->v_sf[v2][kii])) nm++;
+  int st = n1 + n2 - nm;
+  if (st < n1) return 0;
+  int *nl = ialloc(m, st);
+  for (int ki = 1; ki <= n1; ki++) nl[ki] = m->v_sf[v1][ki];
+  for (int ki = n1 + 1; ki <= st; ki++) nl[ki] = -1;
+  for (int ki = 1; ki <= st; ki++) {
+    if (nl[ki] >= 0) continue;
+    for (int kii = 1; kii <= n2; kii++) {
+      int sfkii = m->v_sf[v2][kii], found = 0;
+      for (int p = 1; p <= ki - 1; p++) {
+        if (nl[p] < 0) continue;
+        if (same_cells(m, nl[p], sfkii)) found = 1;
+      }
+      if (!found) {
+        nl[ki] = sfkii;
+        break;
+      }
+    }
+    if (nl[ki] < 0) return 0;
+  }
+  m->v_sf[v1] = nl;
+  m->v_nsfn[v1] = st;
+  /* step 3: v2's own sub-elements appended to v1's list */
+  int ne1 = m->v_nsen[v1], add = 0;
+  for (int k = 1; k <= m->v_nsen[v2]; k++)
+    if (m->se_vert[m->v_se[v2][k]] == v2) add++;
+  int *el = ialloc(m, ne1 + add);
+  for (int k = 1; k <= ne1; k++) el[k] = m->v_se[v1][k];
+  add = 0;
+  for (int k = 1; k <= m->v_nsen[v2]; k++) {
+    int id_se = m->v_se[v2][k];
+    if (m->se_vert[id_se] == v2) el[ne1 + ++add] = id_se;
+  }
+  m->v_se[v1] = el;
+  m->v_nsen[v1] = ne1 + add;
+  return 1;
+}
+
+/* build_periodic_stencil, mesh_connectivity_module.F90:1075-1221: vertex
+ * pairs across x, y and z, then the xy, xz and yz edge pairs and the xyz
+ * corner pair, each processed both ways, in that order */
+static int build_periodic_stencil(om_mesh *m) {
+  int *lst[3][2], cnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 2; k++) lst[d][k] = (int *)malloc(sizeof(int) * (size_t)(m->nv + 1));
+  for (int i = 1; i <= m->nv; i++)
+    for (int d = 0; d < 3; d++) {
+      if (on_plane(VC(m, i, d), -0.5)) lst[d][0][++cnt[d][0]] = i;
+      if (on_plane(VC(m, i, d), 0.5)) lst[d][1][++cnt[d][1]] = i;
+    }
+  int ok = 1;
+#define PAIR(a, b)                                              \
+  do {                                                          \
+    ok = ok && process_pair(m, (a), (b)) && process_pair(m, (b), (a)); \
+  } while (0)
+  for (int d = 0; d < 3; d++) {
+    const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+    for (int i = 1; i <= cnt[d][0]; i++)
+      for (int j = 1; j <= cnt[d][1]; j++) {
+        int ii = lst[d][0][i], jj = lst[d][1][j];
+        double da = VC(m, ii, a) - VC(m, jj, a), db = VC(m, ii, b) - VC(m, jj, b);
+        if (sqrt(da * da + db * db) < OM_PTOL) PAIR(ii, jj);
+      }
+  }
+  /* edges: (d1, d2) both on their lower planes <-> both on their upper
+   * planes, matched by the remaining axis; listed from the d1 sets */
+  static const int edge[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+  for (int e = 0; e < 3; e++) {
+    const int d1 = edge[e][0], d2 = edge[e][1], r = edge[e][2];
+    for (int i = 1; i <= cnt[d1][0]; i++) {
+      int ii = lst[d1][0][i];
+      if (!on_plane(VC(m, ii, d2), -0.5)) continue;
+      for (int j = 1; j <= cnt[d1][1]; j++) {
+        int jj = lst[d1][1][j];
+        if (!on_plane(VC(m, jj, d2), 0.5)) continue;
+        if (fabs(VC(m, ii, r) - VC(m, jj, r)) < OM_PTOL) PAIR(ii, jj);
+      }
+    }
+  }
+  for (int i = 1; i <= cnt[0][0]; i++) {
+    int ii = lst[0][0][i];
+    if (!(on_plane(VC(m, ii, 1), -0.5) && on_plane(VC(m, ii, 2), -0.5))) continue;
+    for (int j = 1; j <= cnt[0][1]; j++) {
+      int jj = lst[0][1][j];
+      if (on_plane(VC(m, jj, 1), 0.5) && on_plane(VC(m, jj, 2), 0.5)) PAIR(ii, jj);
+    }
+  }
+#undef PAIR
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 2; k++) free(lst[d][k]);
+  return ok;
+}
+
 /* ----------------------------------------------------------- geometry */
 static void cross3(const double *a, const double *b, double *c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
@@ -895,9 +1076,9 @@ static int compute_geometry_mesh(om_mesh *m, int b2d) {
 }
 
 /* --------------------------------------------------------- entry point */
-om_mesh *om_mesh_build(int nv, const double *coord, int ne, const int *kind, const int *evert,
-                       int nbf, const int *bf_nv, const int *bf_vert, const int *bf_tag, int b2d,
-                       int *status) {
+om_mesh *om_mesh_build_p(int nv, const double *coord, int ne, const int *kind, const int *evert,
+                         int nbf, const int *bf_nv, const int *bf_vert, const int *bf_tag, int b2d,
+                         int periodic, int *status) {
   *status = OM_OK;
   for (int i = 0; i < ne; i++)
     if (n_vert_kind(kind[i]) == 0) {
@@ -921,8 +1102,24 @@ om_mesh *om_mesh_build(int nv, const double *coord, int ne, const int *kind, con
   for (int i = 1; i <= m->nv; i++)
     if (m->v_nsfn[i] > m->nmax_sfn) m->nmax_sfn = m->v_nsfn[i];
   compute_neigh_by_vert(m);
+  if (periodic) {
+    if (!stitch_periodic_faces(m) || !build_periodic_stencil(m)) {
+      *status = OM_ERR_BAD_INPUT;
+      om_mesh_free(m);
+      return NULL;
+    }
+    /* work arrays of the sweeps are sized by the extended lists */
+    for (int i = 1; i <= m->nv; i++)
+      if (m->v_nsfn[i] > m->nmax_sfn) m->nmax_sfn = m->v_nsfn[i];
+  }
   if (!compute_geometry_mesh(m, b2d)) *status = OM_ERR_MESH_GEOMETRY;
   return m;
+}
+
+om_mesh *om_mesh_build(int nv, const double *coord, int ne, const int *kind, const int *evert,
+                       int nbf, const int *bf_nv, const int *bf_vert, const int *bf_tag, int b2d,
+                       int *status) {
+  return om_mesh_build_p(nv, coord, ne, kind, evert, nbf, bf_nv, bf_vert, bf_tag, b2d, 0, status);
 }
 
 /* counts: nv, ne, nf, nsf, nse, nmax_sfn */

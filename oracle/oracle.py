@@ -122,6 +122,9 @@ def lib():
         L.om_mesh_build.restype = P
         L.om_mesh_build.argtypes = [C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, ip, C.c_int,
                                     C.POINTER(C.c_int)]
+        L.om_mesh_build_p.restype = P
+        L.om_mesh_build_p.argtypes = [C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, ip, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int)]
         L.om_mesh_free.argtypes = [P]
         L.om_mesh_counts.argtypes = [P, ip]
         L.om_mesh_get_i.restype = C.c_long
@@ -173,16 +176,18 @@ _DBL_NAMES = ["vert_coord", "vert_volume", "elem_coord", "elem_volume", "face_co
 class OracleMesh:
     """build_mesh + compute_geometry_mesh on the CPU."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, periodic: bool = False):
+        """periodic: stitch the faces on x, y, z = -0.5 / 0.5 and extend the
+        stencils of the vertices there (build_mesh with periodic_mesh)."""
         L = lib()
         st = C.c_int(0)
-        self.h = L.om_mesh_build(mesh.n_vert, np.ascontiguousarray(mesh.coord, np.float64).ravel(),
-                                 mesh.n_elems, np.ascontiguousarray(mesh.kind, np.int32),
-                                 np.ascontiguousarray(mesh.elem_vert, np.int32), int(mesh.bf_nvert.size),
-                                 np.ascontiguousarray(mesh.bf_nvert, np.int32),
-                                 np.ascontiguousarray(mesh.bf_vert, np.int32),
-                                 np.ascontiguousarray(mesh.bf_tag, np.int32), int(mesh.boundary_2d),
-                                 C.byref(st))
+        self.h = L.om_mesh_build_p(mesh.n_vert, np.ascontiguousarray(mesh.coord, np.float64).ravel(),
+                                   mesh.n_elems, np.ascontiguousarray(mesh.kind, np.int32),
+                                   np.ascontiguousarray(mesh.elem_vert, np.int32), int(mesh.bf_nvert.size),
+                                   np.ascontiguousarray(mesh.bf_nvert, np.int32),
+                                   np.ascontiguousarray(mesh.bf_vert, np.int32),
+                                   np.ascontiguousarray(mesh.bf_tag, np.int32), int(mesh.boundary_2d),
+                                   int(periodic), C.byref(st))
         self.status = st.value
         if not self.h:
             raise ValueError(f"oracle mesh build failed (status {self.status})")

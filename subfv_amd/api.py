@@ -26,7 +26,9 @@ class SubfvError(RuntimeError):
 class HostMesh:
     """Connectivity of build_mesh computed on the host only (no device)."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, periodic: bool = False):
+        """periodic: build_mesh with periodic_mesh (faces on x, y, z = -0.5 / 0.5
+        stitched to their partners, subfv_host_mesh_make_periodic)."""
         L = _lib.lib()
         h = C.c_void_p()
         coord = _lib.f64(mesh.coord).ravel()
@@ -39,6 +41,10 @@ class HostMesh:
             raise SubfvError(st, "host mesh build failed")
         self.h = h
         self._cap = 64 * (ev.size + mesh.n_vert + mesh.n_elems) + 1024
+        if periodic:
+            st = L.subfv_host_mesh_make_periodic(self.h)
+            if st:
+                raise SubfvError(st, "Error periodic boundaries !")
 
     def geti(self, name: str) -> np.ndarray:
         out = np.zeros(self._cap, np.int32)
@@ -85,8 +91,12 @@ class Solver:
             raise SubfvError(st, msg.decode() if msg else "")
 
     # ------------------------------------------------------------- mesh
-    def build_mesh(self, mesh):
-        """read_mesh_msh_2 + build_mesh + compute_geometry_mesh."""
+    def build_mesh(self, mesh, periodic: bool = False):
+        """read_mesh_msh_2 + build_mesh + compute_geometry_mesh (periodic:
+        with periodic_mesh, through a host connectivity)."""
+        if periodic:
+            self.upload_mesh(HostMesh(mesh, periodic=True), mesh.boundary_2d)
+            return
         L = _lib.lib()
         coord = _lib.f64(mesh.coord).ravel()
         kind = _lib.i32(mesh.kind)

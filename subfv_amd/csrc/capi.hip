@@ -30,6 +30,10 @@ struct DevTables {
   std::vector<uint32_t> se_src;
   int ngroups = 0;
   std::string err;
+  // periodic meshes: the per-sub-face arrays by list entry (sf_re above is
+  // then by entry too), the face and sub-element sub-face lists as entries,
+  // each sub-face's first entry and each vertex's own sub-face count
+  std::vector<int> ent_vert, ent_face, ent_le, ent_fsf, ent_sesf, prim, n0;
 };
 
 struct subfv_host_mesh {
@@ -79,6 +83,7 @@ struct subfv_ctx {
   DiffParams dparams{};
   long *doff = nullptr;
   double *pre = nullptr, *PD = nullptr, *rdiff = nullptr;
+  long pre_n = 0;  // doubles in the packed viscous blocks
   int *derr = nullptr;
   StepState *st = nullptr;
   double host_t = 0.0;
@@ -187,34 +192,41 @@ int check_err(subfv_ctx *c) {
 // <= kGrpSlots slots) and, per node-sorted slot, the group-local indices of
 // the sub-faces of its vertex that touch its cell, in increasing order (the
 // order compute_rhs_around_vert accumulates them, ns_euler_module.F90:506-517)
-std::string pack_groups(const subfv::HostMesh &hm, const std::vector<int> &secell, std::vector<int> &grp,
-                        std::vector<uint32_t> &src) {
+std::string pack_groups(const subfv::HostMesh &hm, const std::vector<int> &off, const std::vector<int> &ent,
+                        const std::vector<int> &secell, std::vector<int> &grp, std::vector<uint32_t> &src) {
   grp.clear();
   src.assign(hm.nse, 0xFFFFFFFFu);
   int v = 0;
   while (v < hm.nv) {
-    const int vb = v, sf0 = hm.sf_off[v], se0 = hm.v_seoff[v];
-    while (v < hm.nv && v - vb < kGrpVert && hm.sf_off[v + 1] - sf0 <= kGrpSub &&
-           hm.v_seoff[v + 1] - se0 <= kGrpSlots)
+    const int vb = v, sf0 = off[v], se0 = hm.v_seoff[v];
+    while (v < hm.nv && v - vb < kGrpVert && off[v + 1] - sf0 <= kGrpSub && hm.v_seoff[v + 1] - se0 <= kGrpSlots)
       v++;
     if (v == vb) return "vertex " + std::to_string(v + 1) + " has more than " + std::to_string(kGrpSub) +
                         " sub-faces or " + std::to_string(kGrpSlots) + " sub-elements";
-    const int hdr[kGrpHdr] = {vb, v - vb, sf0, hm.sf_off[v] - sf0, se0, hm.v_seoff[v] - se0, 0, 0};
+    const int hdr[kGrpHdr] = {vb, v - vb, sf0, off[v] - sf0, se0, hm.v_seoff[v] - se0, 0, 0};
     grp.insert(grp.end(), hdr, hdr + kGrpHdr);
     for (int u = vb; u < v; u++)
       for (int q = hm.v_seoff[u]; q < hm.v_seoff[u + 1]; q++) {
         const int cell = secell[q];
         int cnt = 0;
         uint32_t code = 0xFFFFFFFFu;
-        for (int s = hm.sf_off[u]; s < hm.sf_off[u + 1]; s++)
-          if (hm.sf_le[s] == cell || hm.sf_re[s] - 1 == cell) {
+        for (int e = off[u]; e < off[u + 1]; e++) {
+          const int s = ent.empty() ? e : ent[e];
+          // sums go to the vertex's own sub-elements only, the right one only
+          // with the left (compute_rhs_around_vert, ns_euler_module.F90:506-517)
+          const int lse = hm.sf_lse[s], rse = hm.sf_rse[s];
+          if (lse < 0 || hm.se_vert[lse] != u) continue;
+          const bool left = hm.sf_le[s] == cell;
+          const bool right = hm.sf_re[s] - 1 == cell && rse >= 0 && hm.se_vert[rse] == u;
+          if (left || right) {
             if (cnt == kSlotSrc)
               return "a cell has more than " + std::to_string(kSlotSrc) + " sub-faces at vertex " +
                      std::to_string(u + 1);
             code &= ~(0xFFu << (8 * cnt));
-            code |= (uint32_t)(s - sf0) << (8 * cnt);
+            code |= (uint32_t)(e - sf0) << (8 * cnt);
             cnt++;
           }
+        }
         src[q] = code;
       }
   }
@@ -241,7 +253,33 @@ std::shared_ptr<DevTables> make_tables(const subfv::HostMesh &h) {
     std::sort(pv, pv + n);
     for (int j = 0; j < n; j++) t->c_slot[b + j] = pv[j].second;
   }
-  t->err = pack_groups(h, t->se_cell, t->grp, t->se_src);
+  if (h.periodic) {
+    const int ne_ = h.x_sfoff[h.nv];
+    t->prim.resize(h.nsf);
+    t->n0.resize(h.nv);
+    for (int v = 0; v < h.nv; v++) {
+      t->n0[v] = h.sf_off[v + 1] - h.sf_off[v];
+      for (int s = h.sf_off[v]; s < h.sf_off[v + 1]; s++) t->prim[s] = h.x_sfoff[v] + (s - h.sf_off[v]);
+    }
+    t->ent_vert.resize(ne_);
+    t->ent_face.resize(ne_);
+    t->ent_le.resize(ne_);
+    t->sf_re.resize(ne_);
+    for (int e = 0; e < ne_; e++) {
+      const int s = h.x_sf[e];
+      t->ent_vert[e] = h.sf_vert[s];
+      t->ent_face[e] = h.sf_face[s];
+      t->ent_le[e] = h.sf_le[s];
+      t->sf_re[e] = h.sf_re[s] - 1;
+    }
+    t->ent_fsf.resize(h.f_sf.size());
+    for (size_t q = 0; q < h.f_sf.size(); q++) t->ent_fsf[q] = t->prim[h.f_sf[q]];
+    t->ent_sesf.resize(h.se_sf.size());
+    for (size_t q = 0; q < h.se_sf.size(); q++) t->ent_sesf[q] = t->prim[h.se_sf[q]];
+    t->err = pack_groups(h, h.x_sfoff, h.x_sf, t->se_cell, t->grp, t->se_src);
+  } else {
+    t->err = pack_groups(h, h.sf_off, {}, t->se_cell, t->grp, t->se_src);
+  }
   t->ngroups = (int)(t->grp.size() / kGrpHdr);
   return t;
 }
@@ -251,16 +289,24 @@ int upload_mesh(subfv_ctx *c) {
   const DevTables &t = *c->tab;
   if (!t.err.empty()) return fail(c, SUBFV_E_INVALID, t.err);
   DevMesh &d = c->dm;
+  const bool per = h.periodic;
   d.V = h.nv;
   d.C = h.ne;
   d.F = h.nf;
-  d.SF = h.nsf;
+  d.SF = per ? h.x_sfoff[h.nv] : h.nsf;  // list entries on a periodic mesh
   d.SE = h.nse;
   int r;
 #define UP(field, vec) \
   if ((r = upload(c, &d.field, vec))) return r
   UP(vcoord, h.vcoord);
-  UP(sf_off, h.sf_off);
+  UP(sf_off, per ? h.x_sfoff : h.sf_off);
+  if (per) {
+    UP(sf_n0, t.n0);
+    UP(vx_seoff, h.x_seoff);
+    UP(vx_se, h.x_se);
+  } else {
+    d.sf_n0 = d.vx_seoff = d.vx_se = nullptr;
+  }
   UP(se_off, h.v_seoff);
   UP(v_eoff, h.v_eoff);
   UP(v_elem, h.v_elem);
@@ -270,13 +316,13 @@ int upload_mesh(subfv_ctx *c) {
   UP(f_left, h.f_left);
   UP(f_right, h.f_right);
   UP(f_sfoff, h.f_sfoff);
-  UP(f_sf, h.f_sf);
-  UP(sf_vert, h.sf_vert);
-  UP(sf_face, h.sf_face);
-  UP(sf_le, h.sf_le);
+  UP(f_sf, per ? t.ent_fsf : h.f_sf);
+  UP(sf_vert, per ? t.ent_vert : h.sf_vert);
+  UP(sf_face, per ? t.ent_face : h.sf_face);
+  UP(sf_le, per ? t.ent_le : h.sf_le);
   UP(sf_re, t.sf_re);
   UP(se_sfoff, h.se_sfoff);
-  UP(se_sf, h.se_sf);
+  UP(se_sf, per ? t.ent_sesf : h.se_sf);
   UP(se_elem, h.se_elem);
   UP(se_cell, t.se_cell);
   UP(e_voff, h.e_voff);
@@ -294,7 +340,7 @@ int upload_mesh(subfv_ctx *c) {
   if ((r = dalloc(c, &d.fcoord, 3L * h.nf))) return r;
   if ((r = dalloc(c, &d.fnorm, 3L * h.nf))) return r;
   if ((r = dalloc(c, &d.farea, h.nf))) return r;
-  if ((r = dalloc(c, &d.sfn, 4L * h.nsf))) return r;
+  if ((r = dalloc(c, &d.sfn, 4L * d.SF))) return r;
   if ((r = dalloc(c, &d.se_vol, h.nse))) return r;
   if ((r = dalloc(c, &d.ccoord, 3L * h.ne))) return r;
   if ((r = dalloc(c, &d.cvol, h.ne))) return r;
@@ -482,6 +528,7 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
 int ensure_diffusion(subfv_ctx *c) {
   if (c->diff_ready) return 0;
   const subfv::HostMesh &h = *c->hmp;
+  if (h.periodic) return fail(c, SUBFV_E_INVALID, "viscous terms are not available on periodic meshes");
   std::vector<long> off(h.nv + 1, 0);
   for (int v = 0; v < h.nv; v++) {
     const int nf = h.sf_off[v + 1] - h.sf_off[v], ns = h.v_seoff[v + 1] - h.v_seoff[v];
@@ -499,6 +546,7 @@ int ensure_diffusion(subfv_ctx *c) {
   int r;
   if ((r = upload(c, &c->doff, off))) return r;
   if ((r = dalloc(c, &c->pre, (size_t)off[h.nv]))) return r;
+  c->pre_n = off[h.nv];
   if ((r = dalloc(c, &c->PD, (size_t)kDiffSlot * h.nse))) return r;
   if ((r = dalloc(c, &c->rdiff, 4ull * h.ne))) return r;
   if (!c->diag && (r = dalloc(c, &c->diag, 25ull * h.ne))) return r;
@@ -538,7 +586,10 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
   a.vper_xcd = (a.vblocks + 7) / 8;
   a.cblocks = (int)nblk(c->dm.C, 256);
   a.cper_xcd = (a.cblocks + 7) / 8;
-  hipLaunchKernelGGL(k_diff_apply, dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0, c->stream, a);
+  {
+    Prof pa(c, "diff_apply");
+    hipLaunchKernelGGL(k_diff_apply, dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0, c->stream, a);
+  }
   hipLaunchKernelGGL(k_diff_gather, dim3(8 * a.cper_xcd), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -577,6 +628,17 @@ int mesh_i32(const subfv::HostMesh &h, const char *name, int32_t *out, int64_t c
   };
   if (n == "vert_nto") list({}, h.vert_nto, 1);
   else if (n == "elem_nto") list({}, h.elem_nto, 1);
+  else if (n == "vert_n_sub_faces" && h.periodic)
+    for (int i = 0; i < h.nv; i++) v.push_back(h.x_sfoff[i + 1] - h.x_sfoff[i]);
+  else if (n == "vert_n_sub_elems" && h.periodic)
+    for (int i = 0; i < h.nv; i++)
+      v.push_back(h.v_seoff[i + 1] - h.v_seoff[i] + h.x_seoff[i + 1] - h.x_seoff[i]);
+  else if (n == "vert_sub_face" && h.periodic) list({}, h.x_sf, 1);
+  else if (n == "vert_sub_elem" && h.periodic)
+    for (int i = 0; i < h.nv; i++) {
+      for (int q = h.v_seoff[i]; q < h.v_seoff[i + 1]; q++) v.push_back(h.v_se[q] + 1);
+      for (int q = h.x_seoff[i]; q < h.x_seoff[i + 1]; q++) v.push_back(h.x_se[q] + 1);
+    }
   else if (n == "vert_n_sub_faces") for (int i = 0; i < h.nv; i++) v.push_back(h.sf_off[i + 1] - h.sf_off[i]);
   else if (n == "vert_n_sub_elems") for (int i = 0; i < h.nv; i++) v.push_back(h.v_seoff[i + 1] - h.v_seoff[i]);
   else if (n == "vert_n_elems") for (int i = 0; i < h.nv; i++) v.push_back(h.v_eoff[i + 1] - h.v_eoff[i]);
@@ -721,6 +783,18 @@ int subfv_host_mesh_build(subfv_host_mesh **out, int32_t n_vert, const double *c
   return 0;
 }
 
+int subfv_host_mesh_make_periodic(subfv_host_mesh *m) {
+  if (!m || !m->hm) return SUBFV_E_INVALID;
+  // a new connectivity: contexts that uploaded the old one keep it
+  auto hm = std::make_shared<subfv::HostMesh>(*m->hm);
+  if (!subfv::make_periodic(*hm).empty()) return SUBFV_E_INVALID;
+  auto tab = make_tables(*hm);
+  if (!tab->err.empty()) return SUBFV_E_INVALID;
+  m->hm = hm;
+  m->tab = tab;
+  return 0;
+}
+
 int64_t subfv_host_mesh_get_i32(subfv_host_mesh *m, const char *name, int32_t *out, int64_t cap) {
   if (!m || !name || !out) return -1;
   return mesh_i32(*m->hm, name, out, cap);
@@ -763,6 +837,7 @@ int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
   if (w == "n_sub_elems") return c->hmp->nse;
   if (w == "n_max_sub_faces") return c->hmp->nmax_sfn;
   if (w == "n_groups") return c->dm.ngroups;
+  if (w == "diff_block_doubles") return c->diff_ready ? c->pre_n : 0;
   return -1;
 }
 
@@ -799,6 +874,21 @@ int64_t subfv_mesh_get_f64(subfv_ctx *c, const char *name, double *out, int64_t 
   if (n == "face_norm") return cp(d.fnorm, 3 * d.F);
   if (n == "face_area") return cp(d.farea, d.F);
   if (n == "sub_elem_volume") return cp(d.se_vol, d.SE);
+  if (c->hmp->periodic && (n == "sub_face_area" || n == "sub_face_norm")) {
+    // by sub-face id: each sub-face's first list entry
+    const long nsf = c->hmp->nsf, k = n == "sub_face_area" ? 1 : 3;
+    if (k * nsf > cap) return -1;
+    std::vector<double> all(4 * d.SF);
+    if (hipMemcpy(all.data(), d.sfn, all.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    const std::vector<int> &prim = c->tab->prim;
+    for (long q = 0; q < nsf; q++) {
+      if (k == 1)
+        out[q] = all[3 * d.SF + prim[q]];
+      else
+        for (int j = 0; j < 3; j++) out[3 * q + j] = all[j * d.SF + prim[q]];
+    }
+    return k * nsf;
+  }
   if (n == "sub_face_area") return cp(d.sfn + 3 * d.SF, d.SF);
   if (n == "sub_face_norm") {
     if (3 * d.SF > cap) return -1;
@@ -1023,6 +1113,7 @@ extern "C" {
 int subfv_compute_rhs_lagrange(subfv_ctx *c, const double *sol, const double *mass, int32_t b2d, double *rhs,
                                double *vp) {
   if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  if (c->hmp->periodic) return fail(c, SUBFV_E_INVALID, "not available on periodic meshes");
   if (!sol || !mass || !rhs || !vp) return fail(c, SUBFV_E_INVALID, "null array");
   HIPCHK(c, hipSetDevice(c->device));
   const DevMesh &m = c->dm;
@@ -1059,6 +1150,7 @@ int subfv_compute_rhs_lagrange(subfv_ctx *c, const double *sol, const double *ma
 int subfv_acoustic_run(subfv_ctx *c, double *sol, int32_t n_steps, double cfl, int32_t n_bc, const int32_t *bc_kind,
                        const double *bc_val, double *t, double *dt_last) {
   if (!c || !c->has_mesh) return fail(c, SUBFV_E_INVALID, "no mesh");
+  if (c->hmp->periodic) return fail(c, SUBFV_E_INVALID, "not available on periodic meshes");
   if (!sol || !t || n_steps < 0 || n_bc < 0 || (n_bc > 0 && (!bc_kind || !bc_val)))
     return fail(c, SUBFV_E_INVALID, "bad acoustic arguments");
   for (int i = 0; i < n_bc; i++)

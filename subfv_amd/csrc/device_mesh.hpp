@@ -25,6 +25,11 @@ struct DevMesh {
   double *vvol = nullptr;     // [V]
   uint8_t *vbound = nullptr;  // [V]
   int *sf_off = nullptr;      // [V+1] sub-faces of v: [sf_off[v], sf_off[v+1])
+  // Periodic meshes list a vertex's own sub-faces first, then those of its
+  // periodic partners; every per-sub-face array below is then indexed by
+  // list entry (a partner's sub-face appears once more, as a copy).
+  int *sf_n0 = nullptr;       // [V] periodic: the vertex's own sub-faces (else null)
+  int *vx_seoff = nullptr, *vx_se = nullptr;  // [V+1] periodic: partners' sub-elements
   int *se_off = nullptr;      // [V+1] node-sorted sub-element slots of v
   int *v_eoff = nullptr, *v_elem = nullptr;  // vertex -> cells (reference order)
   int *v_seid = nullptr;      // [SE] node-sorted slot -> cell-major sub-element id

@@ -350,15 +350,29 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   }
   for (int p = tid; p < nv * 4; p += kGrpThreads) {
     const int vv = p >> 2, c = p & 3;
-    const int jb = L.vtab[vv], je = L.vtab[vv + 1];
-    const double *row = L.red[9 + c];
-    double mm = row[jb];
-    if (c & 1)
-      for (int j = jb + 1; j < je; j++) mm = dmax(mm, row[j]);
-    else
-      for (int j = jb + 1; j < je; j++) mm = dmin(mm, row[j]);
-    // the bounds start from 1e100 / 0.0 (ns_euler_recon_module.F90:83-86)
-    mm = (c & 1) ? dmax(0.0, mm) : dmin(1e100, mm);
+    double mm;
+    if (m.sf_n0) {
+      // periodic mesh: a stitched sub-face reaches a cell of the partner
+      // vertex, so the bounds come from the vertex's own cells
+      mm = (c & 1) ? 0.0 : 1e100;
+      for (int q = m.v_eoff[vb + vv]; q < m.v_eoff[vb + vv + 1]; q++) {
+        double u[5], w[5];
+        for (int k = 0; k < 5; k++) u[k] = a.U[5L * m.v_elem[q] + k];
+        to_prim(u, w);
+        const double x = (c < 2) ? w[0] : w[4];
+        mm = (c & 1) ? dmax(mm, x) : dmin(mm, x);
+      }
+    } else {
+      const int jb = L.vtab[vv], je = L.vtab[vv + 1];
+      const double *row = L.red[9 + c];
+      mm = row[jb];
+      if (c & 1)
+        for (int j = jb + 1; j < je; j++) mm = dmax(mm, row[j]);
+      else
+        for (int j = jb + 1; j < je; j++) mm = dmin(mm, row[j]);
+      // the bounds start from 1e100 / 0.0 (ns_euler_recon_module.F90:83-86)
+      mm = (c & 1) ? dmax(0.0, mm) : dmin(1e100, mm);
+    }
     a.VMM[4L * (vb + vv) + c] = mm;
   }
   __syncthreads();

@@ -219,10 +219,13 @@ __global__ void k_vert_volume_bound(DevMesh m, int b2d) {
   if (v >= m.V) return;
   double vol = 0.0;
   for (int q = m.se_off[v]; q < m.se_off[v + 1]; q++) vol = vol + m.se_vol[m.v_seid[q]];
+  if (m.vx_seoff)  // periodic: the partners' sub-elements follow (build_periodic_stencil step 3)
+    for (int q = m.vx_seoff[v]; q < m.vx_seoff[v + 1]; q++) vol = vol + m.se_vol[m.vx_se[q]];
   m.vvol[v] = vol;
   uint8_t bnd = 0;
-  // faces around v are exactly the faces of its sub-faces
-  for (int s = m.sf_off[v]; s < m.sf_off[v + 1]; s++) {
+  // faces around v are exactly the faces of its own sub-faces
+  const int s_end = m.sf_n0 ? m.sf_off[v] + m.sf_n0[v] : m.sf_off[v + 1];
+  for (int s = m.sf_off[v]; s < s_end; s++) {
     int f = m.sf_face[s];
     int re = m.f_right[f];
     if (b2d) {

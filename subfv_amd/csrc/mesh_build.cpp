@@ -4,6 +4,7 @@
 #include "mesh_build.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace subfv {
@@ -416,6 +417,185 @@ std::string build_host_mesh(HostMesh &m, int nv, const double *coord, int ne, co
     m.nmax_sfn = std::max(m.nmax_sfn, m.sf_off[v + 1] - m.sf_off[v]);
     m.nmax_sen = std::max(m.nmax_sen, m.v_seoff[v + 1] - m.v_seoff[v]);
   }
+  return "";
+}
+
+// ---- periodic_mesh: stitch_periodic_faces (:976-1073) and
+//      build_periodic_stencil (:1075-1328)
+namespace {
+
+constexpr double kPerTol = 1e-12;
+inline bool on_plane(double x, double c) { return x < c + kPerTol && x > c - kPerTol; }
+
+// Vertex lists that grow as partners are processed; a vertex not yet touched
+// reads its built CSR range.
+struct PeriodicLists {
+  const HostMesh &m;
+  std::vector<std::vector<int>> sf, se;
+  std::vector<char> own;
+  explicit PeriodicLists(const HostMesh &mesh) : m(mesh), sf(mesh.nv), se(mesh.nv), own(mesh.nv, 0) {}
+  void touch(int v) {
+    if (own[v]) return;
+    own[v] = 1;
+    sf[v].clear();
+    for (int s = m.sf_off[v]; s < m.sf_off[v + 1]; s++) sf[v].push_back(s);
+    se[v].assign(m.v_se.begin() + m.v_seoff[v], m.v_se.begin() + m.v_seoff[v + 1]);
+  }
+};
+
+}  // namespace
+
+std::string make_periodic(HostMesh &m) {
+  if (m.periodic) return "the mesh is periodic already";
+  // face centres as vertex averages, as build_mesh takes them (:44-52)
+  std::vector<double> fc(3L * m.nf);
+  for (int f = 0; f < m.nf; f++) {
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int q = m.f_voff[f]; q < m.f_voff[f + 1]; q++)
+      for (int d = 0; d < 3; d++) c[d] = c[d] + m.vcoord[3L * m.f_vert[q] + d];
+    const double n = (double)(m.f_voff[f + 1] - m.f_voff[f]);
+    for (int d = 0; d < 3; d++) fc[3L * f + d] = c[d] / n;
+  }
+  // the two other axes of direction d, in the reference's order
+  auto axes = [](int d, int &a, int &b) {
+    a = d == 0 ? 1 : 0;
+    b = d == 2 ? 1 : 2;
+  };
+  std::vector<int> lo[3], hi[3];
+  for (int f = 0; f < m.nf; f++)
+    for (int d = 0; d < 3; d++) {
+      if (on_plane(fc[3L * f + d], -0.5)) lo[d].push_back(f);
+      if (on_plane(fc[3L * f + d], 0.5)) hi[d].push_back(f);
+    }
+  for (int d = 0; d < 3; d++)
+    if (lo[d].size() != hi[d].size()) return "Error periodic boundaries ! (unmatched face counts)";
+  for (int d = 0; d < 3; d++) {
+    int a, b;
+    axes(d, a, b);
+    for (int ii : lo[d])
+      for (int jj : hi[d]) {
+        const double da = fc[3L * ii + a] - fc[3L * jj + a], db = fc[3L * ii + b] - fc[3L * jj + b];
+        if (std::sqrt(da * da + db * db) < kPerTol) {
+          m.f_right[ii] = m.f_left[jj] + 1;
+          m.f_right[jj] = m.f_left[ii] + 1;
+        }
+      }
+  }
+  for (int s = 0; s < m.nsf; s++) {
+    m.sf_le[s] = m.f_left[m.sf_face[s]];
+    m.sf_re[s] = m.f_right[m.sf_face[s]];
+  }
+  PeriodicLists L(m);
+  // sub-faces joining the same two cells, in either orientation (left cells
+  // compared one-based, as the reference stores them)
+  auto same_cells = [&](int p, int q) {
+    const int lp = m.sf_le[p] + 1, rp = m.sf_re[p], lq = m.sf_le[q] + 1, rq = m.sf_re[q];
+    return (lp == rq && rp == lq) || (lp == lq && rp == rq);
+  };
+  std::string err;
+  auto process_pair = [&](int v1, int v2) {
+    L.touch(v1);
+    L.touch(v2);
+    std::vector<int> &s1 = L.sf[v1], &s2 = L.sf[v2];
+    // step 1: right sub-element of v1's stitched sub-faces, from v2's list
+    for (int sfki : s1) {
+      const int re = m.sf_re[sfki];
+      if (re > 0 && m.sf_rse[sfki] < 0)
+        for (int id_se : L.se[v2])
+          if (m.se_elem[id_se] == re - 1) {
+            m.sf_rse[sfki] = id_se;
+            break;
+          }
+    }
+    // step 2: v2's sub-faces appended, without repeats
+    const int n1 = (int)s1.size(), n2 = (int)s2.size();
+    int nm = 0;
+    for (int p : s1)
+      for (int q : s2)
+        if (same_cells(p, q)) nm++;
+    const int st = n1 + n2 - nm;
+    if (st < n1) {
+      err = "periodic stencil of vertex " + std::to_string(v1 + 1) + " cannot be formed";
+      return;
+    }
+    std::vector<int> nl(s1);
+    nl.resize(st, -1);
+    for (int ki = n1; ki < st; ki++)
+      for (int q : s2) {
+        bool found = false;
+        for (int p = 0; p < ki; p++)
+          if (nl[p] >= 0 && same_cells(nl[p], q)) found = true;
+        if (!found) {
+          nl[ki] = q;
+          break;
+        }
+      }
+    for (int x : nl)
+      if (x < 0) {
+        err = "periodic stencil of vertex " + std::to_string(v1 + 1) + " cannot be filled";
+        return;
+      }
+    s1.swap(nl);
+    // step 3: v2's own sub-elements appended
+    std::vector<int> add;
+    for (int id_se : L.se[v2])
+      if (m.se_vert[id_se] == v2) add.push_back(id_se);
+    L.se[v1].insert(L.se[v1].end(), add.begin(), add.end());
+  };
+  auto pair = [&](int a, int b) {
+    if (err.empty()) process_pair(a, b);
+    if (err.empty()) process_pair(b, a);
+  };
+  std::vector<int> vlo[3], vhi[3];
+  for (int v = 0; v < m.nv; v++)
+    for (int d = 0; d < 3; d++) {
+      if (on_plane(m.vcoord[3L * v + d], -0.5)) vlo[d].push_back(v);
+      if (on_plane(m.vcoord[3L * v + d], 0.5)) vhi[d].push_back(v);
+    }
+  const double *X = m.vcoord.data();
+  for (int d = 0; d < 3; d++) {
+    int a, b;
+    axes(d, a, b);
+    for (int ii : vlo[d])
+      for (int jj : vhi[d]) {
+        const double da = X[3L * ii + a] - X[3L * jj + a], db = X[3L * ii + b] - X[3L * jj + b];
+        if (std::sqrt(da * da + db * db) < kPerTol) pair(ii, jj);
+      }
+  }
+  // edges: lower on d1 and d2 <-> upper on both, matched along the third axis
+  const int edge[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+  for (const auto &e : edge)
+    for (int ii : vlo[e[0]]) {
+      if (!on_plane(X[3L * ii + e[1]], -0.5)) continue;
+      for (int jj : vhi[e[0]])
+        if (on_plane(X[3L * jj + e[1]], 0.5) && std::fabs(X[3L * ii + e[2]] - X[3L * jj + e[2]]) < kPerTol)
+          pair(ii, jj);
+    }
+  for (int ii : vlo[0]) {
+    if (!(on_plane(X[3L * ii + 1], -0.5) && on_plane(X[3L * ii + 2], -0.5))) continue;
+    for (int jj : vhi[0])
+      if (on_plane(X[3L * jj + 1], 0.5) && on_plane(X[3L * jj + 2], 0.5)) pair(ii, jj);
+  }
+  if (!err.empty()) return err;
+  // flatten: the sweep's lists and the partners' sub-elements
+  m.x_sfoff.assign(m.nv + 1, 0);
+  m.x_seoff.assign(m.nv + 1, 0);
+  m.x_sf.clear();
+  m.x_se.clear();
+  for (int v = 0; v < m.nv; v++) {
+    if (L.own[v]) {
+      m.x_sf.insert(m.x_sf.end(), L.sf[v].begin(), L.sf[v].end());
+      const int n0 = m.v_seoff[v + 1] - m.v_seoff[v];
+      m.x_se.insert(m.x_se.end(), L.se[v].begin() + n0, L.se[v].end());
+    } else {
+      for (int s = m.sf_off[v]; s < m.sf_off[v + 1]; s++) m.x_sf.push_back(s);
+    }
+    m.x_sfoff[v + 1] = (int)m.x_sf.size();
+    m.x_seoff[v + 1] = (int)m.x_se.size();
+    m.nmax_sfn = std::max(m.nmax_sfn, m.x_sfoff[v + 1] - m.x_sfoff[v]);
+    m.nmax_sen = std::max(m.nmax_sen, m.v_seoff[v + 1] - m.v_seoff[v] + m.x_seoff[v + 1] - m.x_seoff[v]);
+  }
+  m.periodic = true;
   return "";
 }
 

@@ -1,6 +1,6 @@
 // Host-side connectivity of the subface mesh, numbered exactly as subfv's
-// build_mesh numbers it (src/mesh/mesh_connectivity_module.F90:12-59, one rank,
-// non-periodic).  All ids are 0-based here; the reference's 1-based numbers
+// build_mesh numbers it (src/mesh/mesh_connectivity_module.F90:12-59, one rank;
+// the periodic stitch is a separate step, make_periodic).  All ids are 0-based here; the reference's 1-based numbers
 // are these plus one.  The right neighbour of a face keeps the reference
 // encoding: >0 one-based cell id, 0 default wall, <0 minus the BC index.
 #pragma once
@@ -35,6 +35,12 @@ struct HostMesh {
   std::vector<int> e_tag;
   // cells sharing a vertex with the cell (neigh_by_vert)
   std::vector<int> e_nbvoff, e_nbv;
+  // periodic meshes (make_periodic): each vertex's sub-face list as the node
+  // sweep reads it, its own sub-faces first, then those its periodic partners
+  // bring (x_sf holds sub-face ids); and the sub-elements the partners bring,
+  // after the vertex's own (x_se, cell-major ids).  Empty otherwise.
+  bool periodic = false;
+  std::vector<int> x_sfoff, x_sf, x_seoff, x_se;
 };
 
 // kind: gmsh kinds (4 tet, 5 hex, 6 prism, 7 pyramid); elem_vert: 1-based
@@ -43,6 +49,12 @@ struct HostMesh {
 std::string build_host_mesh(HostMesh &m, int nv, const double *coord, int ne, const int32_t *kind,
                             const int32_t *elem_vert, int nbf, const int32_t *bf_nvert,
                             const int32_t *bf_vert, const int32_t *bf_tag);
+
+// The periodic part of build_mesh (src/mesh/mesh_connectivity_module.F90:44-54)
+// on a built connectivity: faces on x, y, z = -0.5 and 0.5 are stitched to
+// their partners, and the vertices there take their partners' sub-faces and
+// sub-elements into their lists.  Returns an empty string or an error.
+std::string make_periodic(HostMesh &m);
 
 int gmsh_n_vert(int kind);
 int gmsh_n_face(int kind);
