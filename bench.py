@@ -12,7 +12,9 @@ the state resident in HBM.  One JSON line is printed.
 The roofline block times the dominant kernel with HIP events recorded on
 the context's stream during the timed region and divides its algorithmic
 bytes per launch (mesh/state bytes it must read, intermediates excluded) by
-the average launch time.  The CPU baseline times the oracle (single core)
+the average launch time: the viscous apply k_diff_apply when the viscous
+terms are on (``roofline_node_sweep`` then gives the node sweep's), the node
+sweep k_node_flux otherwise.  The CPU baseline times the oracle (single core)
 on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -41,22 +43,37 @@ def algorithmic_bytes(C, V, SF, SE, second_order):
     return b
 
 
+def diff_apply_bytes(C, V, SF, SE, block_doubles):
+    """Bytes the viscous apply must move per launch: the packed per-vertex
+    blocks (N^-1 S, S~^T N^-1 S, their right-hand sides and the heat-system
+    counterparts, built once per mesh), each sub-face's unit normal, area and
+    left cell, each slot's cell, sub-element id, sub-element volume, offset
+    and sub-face list (three entries for a prism corner), each cell's state
+    once, the vertex offsets, and the 18-double slot rows it writes."""
+    return block_doubles * 8 + SF * 36 + SE * (4 + 4 + 8 + 4 + 12) + C * 40 + V * 16 + SE * 144
+
+
 def stage_bytes_m1(C, V, SF, SE):
     """First-order residual + update per stage: U in and out, volume, the
     sub-face data, sub-element -> cell ids and vertex offsets."""
     return C * 88 + SF * 40 + SE * 4 + V * 4
 
 
-def read_traffic(kernel_tag):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel factor), if a summary exists."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel_tag}.txt")
+def read_value(summary, key):
+    """A figure recorded in a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"{summary}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel factor), if a summary exists."""
+    return read_value(f"pmc_{kernel_tag}", "traffic_bytes_per_launch")
 
 
 def cpu_baseline(n=192, steps=1, viscous=True):
@@ -178,16 +195,16 @@ def main():
     tf, dtf = s.run(args.steps, scheme)  # ends with a stream synchronisation
     elapsed = time.perf_counter() - t1
     phases = {}
-    for ph in ("node_flux", "node_flux_o2", "grad", "gather", "diffusion", "update"):
+    for ph in ("node_flux", "node_flux_o2", "grad", "gather", "diffusion", "diff_apply", "update"):
         ms, cnt = s.profile_read(ph)
         if cnt:
             phases[ph] = {"ms_total": ms, "launches": cnt, "us_avg": 1e3 * ms / cnt}
     s.profile(False)
 
     value = C * stages * args.steps / elapsed
-    # dominant kernel: the node sweep k_node_flux, launched once per stage for
-    # the first-order residual and, in second order, once more for the
-    # reconstructed one; the roofline averages over both launch kinds
+    # node sweep k_node_flux, launched once per stage for the first-order
+    # residual and, in second order, once more for the reconstructed one; its
+    # roofline averages over both launch kinds
     sweeps = [(p, p == "node_flux_o2") for p in ("node_flux", "node_flux_o2") if p in phases]
     n_launch = sum(phases[p]["launches"] for p, _ in sweeps)
     sweep_us = 1e3 * sum(phases[p]["ms_total"] for p, _ in sweeps) / n_launch
@@ -195,6 +212,25 @@ def main():
     achieved = abytes / (sweep_us * 1e-6) / 1e9
     tag = f"tri{args.n}_" + ("o1_fe" if args.first_order else "o2_rk3")
     traffic = read_traffic(f"k_node_flux_{tag}")
+    node_roof = {
+        "bound": "hbm", "kernel": "k_node_flux<multi_point>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "algorithmic_bytes_per_launch": abytes, "us_per_launch": sweep_us, "launches": n_launch,
+        "valu_issue_frac": read_value(f"valu_k_node_flux_{tag}", "valu_issue_frac"),
+    }
+    roofline, extra = node_roof, {}
+    if "diff_apply" in phases:
+        # with the viscous terms the dominant kernel is the viscous apply
+        # (longest launch, and more time per stage than both node sweeps)
+        da = phases["diff_apply"]
+        dbytes = diff_apply_bytes(C, V, SF, SE, s.count("diff_block_doubles"))
+        dach = dbytes / (da["us_avg"] * 1e-6) / 1e9
+        roofline = {
+            "bound": "hbm", "kernel": "k_diff_apply", "achieved": dach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dach / HBM_PEAK_GBS, "traffic": read_traffic(f"k_diff_apply_{tag}"),
+            "algorithmic_bytes_per_launch": dbytes, "us_per_launch": da["us_avg"], "launches": da["launches"],
+        }
+        extra = {"roofline_node_sweep": node_roof}
     stage_us = 1e6 * elapsed / (args.steps * stages)
     m1 = stage_bytes_m1(C, V, SF, SE)
     out = {
@@ -218,11 +254,8 @@ def main():
             "cells": C, "stages_per_step": stages, "viscous": viscous, "mu": 1e-3 if viscous else 0.0,
             "prandtl": 0.72 if viscous else None,
         },
-        "roofline": {
-            "bound": "hbm", "kernel": "k_node_flux<multi_point>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": abytes, "us_per_launch": sweep_us, "launches": n_launch,
-        },
+        "roofline": roofline,
+        **extra,
         "stage": {
             "us_per_stage": stage_us,
             "m1_bytes": m1,
