@@ -89,6 +89,15 @@ module subfv_c
       import :: c_ptr
       type(c_ptr), value :: hm
     end subroutine
+    integer(c_int) function subfv_host_mesh_make_periodic(hm) bind(C, name="subfv_host_mesh_make_periodic")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: hm
+    end function
+    integer(c_int) function subfv_mesh_upload(ctx, hm, boundary_2d) bind(C, name="subfv_mesh_upload")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, hm
+      integer(c_int32_t), value :: boundary_2d
+    end function
     integer(c_int) function subfv_set_config(ctx, cfg) bind(C, name="subfv_set_config")
       import :: c_ptr, c_int, subfv_config
       type(c_ptr), value :: ctx

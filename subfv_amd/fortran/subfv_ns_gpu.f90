@@ -146,15 +146,32 @@ contains
     end do
   end subroutine
 
-  ! build_mesh + compute_geometry_mesh on the device (one rank, sub-entities)
-  subroutine build_mesh(ctx, mesh, boundary_2d)
+  ! build_mesh + compute_geometry_mesh on the device (one rank, sub-entities);
+  ! periodic_mesh stitches x, y, z = -0.5 / 0.5 on the host connectivity first
+  subroutine build_mesh(ctx, mesh, boundary_2d, periodic_mesh)
     type(c_ptr), intent(in) :: ctx
     type(raw_mesh), intent(in) :: mesh
     logical, intent(in) :: boundary_2d
+    logical, intent(in), optional :: periodic_mesh
     integer(c_int32_t) :: b2d
+    type(c_ptr) :: hm
+    logical :: per
     b2d = merge(1, 0, boundary_2d)
-    call check(ctx, subfv_mesh_build(ctx, mesh%n_vert, mesh%coord, mesh%n_elems, mesh%kind, mesh%elem_vert, &
-      mesh%n_bfaces, mesh%bf_nvert, mesh%bf_vert, mesh%bf_tag, b2d))
+    per = .false.
+    if (present(periodic_mesh)) per = periodic_mesh
+    if (.not. per) then
+      call check(ctx, subfv_mesh_build(ctx, mesh%n_vert, mesh%coord, mesh%n_elems, mesh%kind, mesh%elem_vert, &
+        mesh%n_bfaces, mesh%bf_nvert, mesh%bf_vert, mesh%bf_tag, b2d))
+      return
+    end if
+    if (subfv_host_mesh_build(hm, mesh%n_vert, mesh%coord, mesh%n_elems, mesh%kind, mesh%elem_vert, &
+        mesh%n_bfaces, mesh%bf_nvert, mesh%bf_vert, mesh%bf_tag) /= 0) error stop "build_mesh failed"
+    if (subfv_host_mesh_make_periodic(hm) /= 0) then
+      print *, "Error periodic boundaries !"
+      error stop
+    end if
+    call check(ctx, subfv_mesh_upload(ctx, hm, b2d))
+    call subfv_host_mesh_free(hm)
   end subroutine
 
   integer function mesh_count(ctx, what)

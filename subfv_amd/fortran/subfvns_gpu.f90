@@ -127,12 +127,12 @@ program subfvns_gpu
   cfg%method = method
   cfg%cfl = cfl
   cfg%tmax = tmax
-  if (periodic_mesh .or. rescale .or. use_cylinder_map) error stop "periodic/rescaled/mapped meshes are not supported"
+  if (rescale .or. use_cylinder_map) error stop "rescaled/mapped meshes are not supported"
 
   ! ---- mesh and initial state
   call check(c_null_ptr, subfv_create(ctx, 0))
   call read_mesh_msh_2(mesh, meshfile_path, meshfile, n_bc, bc_name)
-  call build_mesh(ctx, mesh, boundary_2d)
+  call build_mesh(ctx, mesh, boundary_2d, periodic_mesh)
   call check(ctx, subfv_set_config(ctx, cfg))
   if (activate_diffusion) then
     ! viscous terms with constant viscosity; Prandtl is 0.71 in the reference
