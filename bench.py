@@ -76,9 +76,10 @@ def read_traffic(kernel_tag):
     return read_value(f"pmc_{kernel_tag}", "traffic_bytes_per_launch")
 
 
-def cpu_baseline(n=192, steps=1, viscous=True):
-    """Oracle (single core) on an n x n tri mesh, second order, SSP-RK3, with
-    the same viscous terms as the timed workload."""
+def cpu_baseline(n=192, steps=1, viscous=True, first_order=False):
+    """Oracle (single core) on an n x n tri mesh with the timed workload's
+    scheme: second order, SSP-RK3 and the same viscous terms, or the
+    inviscid first-order forward-Euler stage (`steps` stages then)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -86,16 +87,21 @@ def cpu_baseline(n=192, steps=1, viscous=True):
 
     om = O.OracleMesh(mg.tri_mesh(n))
     sol = mg.vortex_state(om.elem_coord)
-    cfg = O.make_config("multi_point", second_order=True, method=4, cfl=0.4, diffusion=viscous, mu=1e-3,
-                        prandtl=0.72)
+    cfg = O.make_config("multi_point", second_order=not first_order, method=4, cfl=0.4,
+                        diffusion=viscous and not first_order, mu=1e-3, prandtl=0.72)
+    stages = 1 if first_order else 3
     t0 = time.perf_counter()
-    st, _, _ = om.run(cfg, sol, steps, 1)
+    st, _, _ = om.run(cfg, sol, steps, 0 if first_order else 1)
     dt = time.perf_counter() - t0
     if st:
         raise RuntimeError(f"oracle status {st}")
-    return {"value": om.n_elems * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"tri {n}x{n} ({om.n_elems} prisms), {steps} SSP-RK3 step(s), second order, multi_point"
-                      + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid")}
+    if first_order:
+        what = f"{steps} forward-Euler step(s), first order, multi_point, inviscid"
+    else:
+        what = (f"{steps} SSP-RK3 step(s), second order, multi_point"
+                + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid"))
+    return {"value": om.n_elems * stages * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"tri {n}x{n} ({om.n_elems} prisms), " + what}
 
 
 def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cmax=512 ** 2):
@@ -271,7 +277,11 @@ def main():
         # alone cannot fill the GPU and launches dominate the plain path
         out["served_small"] = served_bench(2 * args.served_requests, rate=4000.0, calib_n=96, cmax=96 ** 2)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, viscous=viscous)
+        # samples of about 10-15 s of single-core work
+        if args.first_order:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_n, steps=40, viscous=False, first_order=True)
+        else:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_n, steps=2, viscous=viscous)
     print(json.dumps(out), flush=True)
 
 

@@ -89,6 +89,12 @@ module subfv_c
       import :: c_ptr
       type(c_ptr), value :: hm
     end subroutine
+    integer(c_int) function subfv_init_1drp(ctx, x1drp, w_left, w_right) bind(C, name="subfv_init_1drp")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: x1drp
+      real(c_double), intent(in) :: w_left(5), w_right(5)
+    end function
     integer(c_int) function subfv_host_mesh_make_periodic(hm) bind(C, name="subfv_host_mesh_make_periodic")
       import :: c_ptr, c_int
       type(c_ptr), value :: hm

@@ -149,19 +149,22 @@ program subfvns_gpu
   n_vert = mesh_count(ctx, "n_vert")
   allocate (sol(5, n_elems), xc(3, n_elems), vol(n_elems))
   call elem_geometry(ctx, n_elems, xc, vol)
-  do i = 1, n_elems
-    if (init_uniform) then
-      w = sol_uniform
-    else if (init_1drp) then
-      w = merge(sol_w_1drp_l, sol_w_1drp_r, xc(1, i) < x1drp)
-    else
-      error stop "only init_uniform and init_1drp are supported"
-    end if
-    sol(1, i) = w(1)
-    sol(2:4, i) = w(1) * w(2:4)
-    sol(5, i) = w(5) / (gamma - 1.0_DP) + 0.5_DP * w(1) * (w(2)**2 + w(3)**2 + w(4)**2)
-  end do
-  call check(ctx, subfv_state_put(ctx, sol))
+  if (init_1drp .and. .not. init_uniform) then
+    ! init_sol's init_1drp branch on the device (ns_euler_module.F90:36-43)
+    call check(ctx, subfv_init_1drp(ctx, x1drp, sol_w_1drp_l, sol_w_1drp_r))
+  else
+    do i = 1, n_elems
+      if (init_uniform) then
+        w = sol_uniform
+      else
+        error stop "only init_uniform and init_1drp are supported"
+      end if
+      sol(1, i) = w(1)
+      sol(2:4, i) = w(1) * w(2:4)
+      sol(5, i) = w(5) / (gamma - 1.0_DP) + 0.5_DP * w(1) * (w(2)**2 + w(3)**2 + w(4)**2)
+    end do
+    call check(ctx, subfv_state_put(ctx, sol))
+  end if
   call check(ctx, subfv_set_time(ctx, 0.0_DP))
   if (final_res > 0.0_DP) print *, "note: the device loop runs n_max_iter+1 steps or to tmax (final_res unused)"
 
