@@ -306,9 +306,10 @@ struct GradLds {
 // compute_nodal_grad (ns_euler_recon_module.F90:453-490) over the vertex
 // groups of the node sweep, thread = sub-face, and the per-vertex bounds of
 // relaxed_maximum_principle (:82-97) taken over the cells on either side of
-// the vertex's sub-faces (exactly the cells around the vertex; min and max do
-// not depend on the order).  Sums run in sub-face order (seg_sum); the 3x3
-// pseudo-inverses of the group run in the first wavefront.
+// the vertex's sub-faces (exactly the cells around the vertex on a
+// non-periodic mesh; min and max do not depend on the order).  Sums run in
+// sub-face order (seg_sum); the 3x3 pseudo-inverses of the group run in the
+// first wavefront.
 __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   __shared__ GradLds L;
   const DevMesh &m = a.m;
