@@ -184,3 +184,27 @@ def test_bad_omega_is_reported(solver):
     with pytest.raises(SubfvError) as e:
         solver.run(2)
     assert e.value.code == 3
+
+
+@pytest.mark.parametrize("case", [("quad8lr", 0, False, 25), ("tri7", 0, True, 10), ("tet3", 0, True, 6)])
+def test_local_time_step_loop_bit_exact(solver, case):
+    """local_time_step = .true. (ns_euler_module.F90:557-560): dt stays the
+    global CFL minimum but is not clamped to tmax, the point-implicit
+    diagonal keeps a1*sum_lambda, and the positivity back-off does not
+    shorten dt; the pseudo-time mode of the potential-flow tests."""
+    name, ts, o2, nsteps = case
+    print(f"[gpu] local time step {case}", flush=True)
+    mesh, om = _pair(solver, name)
+    bcs = _bcs_for(name)
+    kw = dict(boundary_2d=mesh.boundary_2d, bcs=bcs, second_order=o2, cfl=0.8, local_time_step=True, tmax=0.05)
+    solver.set_config(make_config("multi_point", **kw))
+    solver.set_time(0.0)
+    cfg = O.make_config("multi_point", **kw)
+    sol = mg.init_1drp(om.elem_coord, 0.5, [1, 0.1, 0, 0, 1], [0.125, 0, 0, 0, 0.1])
+    solver.put_sol(sol)
+    t, dt = solver.run(nsteps, ts)
+    got = solver.get_sol()
+    st, ref, info = om.run(cfg, sol, nsteps, ts)
+    assert st == 0
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    assert t == info["t"] and dt == info["dt"]

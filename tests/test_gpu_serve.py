@@ -52,3 +52,26 @@ def test_init_1drp_on_the_device_equals_the_host_state():
     ref = mg.init_1drp(s.elem_coord, 0.45, wl, wr)
     s.close()
     assert np.array_equal(got, ref)
+
+
+def test_periodic_instances_in_a_batch_equal_solves_alone(requests):
+    """Instances on periodic meshes (extended vertex lists) batched with
+    ordinary ones."""
+    print("[gpu] serve periodic", flush=True)
+    from subfv_amd.api import HostMesh
+    per = []
+    for i, (kind, n) in enumerate((("quad", 10), ("tri", 8))):
+        mesh = (mg.quad_mesh if kind == "quad" else mg.tri_mesh)(n, x0=-0.5, y0=-0.5, seed=40 + i)
+        r = S.Request(100 + i, kind, n, 0.0, np.array([1.0, 0.2, 0.1, 0.0, 1.0]),
+                      np.array([0.3, -0.1, 0.0, 0.0, 0.4]), 0.1 * i, 15)
+        r.host = HostMesh(mesh, periodic=True)
+        r.boundary_2d = True
+        r.cells = mesh.n_elems
+        per.append(r)
+    reqs = per + list(requests[:3])
+    alone, _ = S.serve_plain(reqs, keep=True, realtime=False)
+    mixed, _ = S.serve(reqs, S.Params(chunk=6, max_batch=4, use_graph=True), keep=True, realtime=False)
+    a = {r.rid: r.sol for r in alone}
+    m = {r.rid: r.sol for r in mixed}
+    for rid in a:
+        assert np.array_equal(a[rid], m[rid]), rid
