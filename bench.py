@@ -122,7 +122,7 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cm
     one = Solver(0)
     plain = S.summarize(*S.serve_plain(reqs, solver=one))
     one.close()
-    pool = [Solver(0) for _ in range(best.max_batch)]
+    pool = [Solver(0) for _ in range(best.max_batch + best.prefetch)]
     served = S.summarize(*S.serve(reqs, best, pool=pool))
     for s in pool:
         s.close()

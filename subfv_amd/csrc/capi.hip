@@ -71,6 +71,9 @@ struct subfv_ctx {
   // stay allocated when the mesh changes (an upload re-uses them)
   std::vector<std::pair<char *, size_t>> chunks;
   size_t chunk_i = 0, chunk_off = 0;
+  // pinned host blocks that stage a mesh upload's copies (reused per upload)
+  std::vector<std::pair<char *, size_t>> pins;
+  size_t pin_i = 0, pin_off = 0;
   // state and work arrays
   double *U[3] = {nullptr, nullptr, nullptr};
   int cur = 0;
@@ -84,7 +87,7 @@ struct subfv_ctx {
   long *doff = nullptr;
   double *pre = nullptr, *PD = nullptr, *rdiff = nullptr;
   long pre_n = 0;  // doubles in the packed viscous blocks
-  int *derr = nullptr;
+  int *derr = nullptr;  // [2]: error bits of the path, failed check_mesh cells
   StepState *st = nullptr;
   double host_t = 0.0;
   // optional per-phase kernel timing with HIP events on the context stream
@@ -134,6 +137,38 @@ int upload(subfv_ctx *c, T **p, const std::vector<T> &v) {
   if (r) return r;
   if (!v.empty()) HIPCHK(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return 0;
+}
+
+// Host-to-device copy queued on the context's stream from its pinned
+// staging blocks (a plain memcpy into pinned memory, then an asynchronous
+// DMA copy); the blocks are reused from the start by the next upload, after
+// the synchronisation that ends a mesh upload.
+int stage_copy(subfv_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!bytes) return 0;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  while (c->pin_i < c->pins.size() && c->pin_off + need > c->pins[c->pin_i].second) {
+    c->pin_i++;
+    c->pin_off = 0;
+  }
+  if (c->pin_i == c->pins.size()) {
+    const size_t sz = std::max(need, (size_t)1 << 20);
+    void *q = nullptr;
+    HIPCHK(c, hipHostMalloc(&q, sz, hipHostMallocDefault));
+    c->pins.push_back({(char *)q, sz});
+    c->pin_off = 0;
+  }
+  char *h = c->pins[c->pin_i].first + c->pin_off;
+  std::memcpy(h, src, bytes);
+  c->pin_off += need;
+  HIPCHK(c, hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+template <class T>
+int upload_staged(subfv_ctx *c, T **p, const std::vector<T> &v) {
+  int r = dalloc(c, p, v.size());
+  if (r) return r;
+  return stage_copy(c, *p, v.data(), v.size() * sizeof(T));
 }
 
 // Forget the mesh; its chunks stay allocated for the next one, so the work
@@ -290,6 +325,8 @@ int upload_mesh(subfv_ctx *c) {
   if (!t.err.empty()) return fail(c, SUBFV_E_INVALID, t.err);
   DevMesh &d = c->dm;
   const bool per = h.periodic;
+  c->pin_i = 0;  // the previous upload's copies finished (free_mesh synchronised)
+  c->pin_off = 0;
   d.V = h.nv;
   d.C = h.ne;
   d.F = h.nf;
@@ -297,7 +334,7 @@ int upload_mesh(subfv_ctx *c) {
   d.SE = h.nse;
   int r;
 #define UP(field, vec) \
-  if ((r = upload(c, &d.field, vec))) return r
+  if ((r = upload_staged(c, &d.field, vec))) return r
   UP(vcoord, h.vcoord);
   UP(sf_off, per ? h.x_sfoff : h.sf_off);
   if (per) {
@@ -376,15 +413,13 @@ int geometry(subfv_ctx *c) {
   hipLaunchKernelGGL(k_subface_norm, dim3(nblk(d.SF, T)), dim3(T), 0, s, d);
   hipLaunchKernelGGL(k_face_norm, dim3(nblk(d.F, T)), dim3(T), 0, s, d);
   hipLaunchKernelGGL(k_subelem_volume, dim3(nblk(d.SE, T)), dim3(T), 0, s, d);
-  int *bad = nullptr;
-  HIPCHK(c, hipMallocAsync((void **)&bad, sizeof(int), s));
+  int *bad = c->derr + 1;
   HIPCHK(c, hipMemsetAsync(bad, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_elem_volume_check, dim3(nblk(d.C, T)), dim3(T), 0, s, d, bad);
   hipLaunchKernelGGL(k_vert_volume_bound, dim3(nblk(d.V, T)), dim3(T), 0, s, d, (int)c->b2d);
   HIPCHK(c, hipGetLastError());
   int hb = 0;
   HIPCHK(c, hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipFreeAsync(bad, s));
   HIPCHK(c, hipStreamSynchronize(s));
   if (hb) return fail(c, SUBFV_E_MESH, "MESH GEOMETRY IS NOT OK (" + std::to_string(hb) + " cells)");
   return 0;
@@ -732,13 +767,20 @@ int subfv_create(subfv_ctx **out, int device) {
   subfv_ctx *c = new subfv_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipMalloc((void **)&c->derr, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&c->derr, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&c->st, sizeof(StepState)) != hipSuccess) {
     delete c;
     return SUBFV_E_DEVICE;
   }
-  (void)hipMemset(c->derr, 0, sizeof(int));
+  (void)hipMemset(c->derr, 0, 2 * sizeof(int));
   (void)hipMemset(c->st, 0, sizeof(StepState));
+  // the first device chunk and the upload staging hold a mesh of up to about
+  // 10^4 cells, so a context admits such a mesh without allocating; larger
+  // meshes add chunks and staging blocks
+  void *q = nullptr;
+  if (hipMalloc(&q, (size_t)48 << 20) == hipSuccess) c->chunks.push_back({(char *)q, (size_t)48 << 20});
+  q = nullptr;
+  if (hipHostMalloc(&q, (size_t)8 << 20, hipHostMallocDefault) == hipSuccess) c->pins.push_back({(char *)q, (size_t)8 << 20});
   c->cfg.scheme_id = SUBFV_SCHEME_MULTI_POINT;
   c->cfg.bc_style = 2;
   c->cfg.method = 4;
@@ -757,6 +799,8 @@ void subfv_destroy(subfv_ctx *c) {
   free_mesh(c);
   for (auto &ch : c->chunks) (void)hipFree(ch.first);
   c->chunks.clear();
+  for (auto &ph : c->pins) (void)hipHostFree(ph.first);
+  c->pins.clear();
   if (c->derr) (void)hipFree(c->derr);
   if (c->st) (void)hipFree(c->st);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -21,7 +21,8 @@ from __future__ import annotations
 import ctypes as C
 import math
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
+from concurrent.futures import wait as wait_futures
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -140,20 +141,28 @@ class Params:
     use_graph: bool = True
     budget: int = 96 << 30   # HBM bytes for admitted instances
     admit_workers: int = 8   # host threads uploading admitted requests
+    prefetch: int = 256      # instances uploaded ahead of free batch capacity
+    first: int = 64          # uploads to wait for before the first chunk of an idle server
 
 
 def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep: bool = False,
           realtime: bool = True, pool: list | None = None) -> tuple[list[Result], float]:
     """Continuous batching.  Returns the results (latency = finish - arrival)
-    and the makespan in seconds.  `pool`: idle contexts to admit into (a
-    server keeps them between calls; they are returned to it); without one,
-    contexts are created as needed and closed at the end."""
+    and the makespan in seconds.  Arrived requests are uploaded into idle
+    contexts by a pool of host threads in the background, up to the free
+    batch capacity plus `params.prefetch` ahead, and join the batch at the
+    next chunk boundary; an idle server waits for `params.first` uploads
+    (or all that are under way) before it starts.  `pool`: idle contexts to
+    admit into (a server keeps them between calls; they are returned to it);
+    without one, contexts are created as needed and closed at the end."""
     batch = Batch(device)
     own = pool is None
     pool = [] if own else pool
     pending = sorted(reqs, key=lambda r: r.arrival)
     pi = 0
-    active = []  # [request, solver, remaining]
+    active = []     # [request, solver, remaining], in the batch
+    staged = []     # uploaded, waiting for batch capacity
+    uploading = {}  # future -> entry
     used = 0
     results = []
     t0 = time.perf_counter()
@@ -161,40 +170,49 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
     def now():
         return time.perf_counter() - t0 if realtime else float("inf")
 
-    def admit_due(limit):
+    def start_uploads(limit):
         nonlocal pi, used
-        out = []
         t = now()
-        while (pi < len(pending) and pending[pi].arrival <= t and limit - len(out) > 0
+        n = 0
+        while (pi < len(pending) and pending[pi].arrival <= t and n < limit
                and used + device_bytes(pending[pi]) <= params.budget):
             r = pending[pi]
             pi += 1
             s = pool.pop() if pool else Solver(device)
-            out.append([r, s, r.steps])
             used += device_bytes(r)
-        # each admission works on its own context and stream; the C-ABI calls
-        # release the interpreter lock, so several uploads proceed together
-        if len(out) > 1:
-            list(ex.map(lambda a: admit(a[1], a[0]), out))
-        elif out:
-            admit(out[0][1], out[0][0])
-        return out
+            # each upload works on its own context and stream; the C-ABI calls
+            # release the interpreter lock, so uploads overlap the batch
+            uploading[ex.submit(admit, s, r)] = [r, s, r.steps]
+            n += 1
 
-    staged = []
+    def collect(wait_for=0):
+        if wait_for > 0 and uploading:
+            done = set()
+            while len(done) < min(wait_for, len(uploading)):
+                d, _ = wait_futures(list(set(uploading) - done), return_when=FIRST_COMPLETED)
+                done |= d
+        for f in [f for f in uploading if f.done()]:
+            entry = uploading.pop(f)
+            f.result()  # an upload error ends the serve
+            staged.append(entry)
+
     ex = ThreadPoolExecutor(max_workers=params.admit_workers)
     try:
-        while pi < len(pending) or active or staged:
-            active += staged
-            staged = []
+        while pi < len(pending) or active or staged or uploading:
+            collect()
+            room = params.max_batch - len(active)
+            active += staged[:room]
+            del staged[:max(room, 0)]
             if not active:
-                active = admit_due(params.max_batch)
-                if not active:
+                start_uploads(params.max_batch + params.prefetch - len(uploading))
+                if not uploading:
                     time.sleep(max(0.0, pending[pi].arrival - (time.perf_counter() - t0)))
-                    continue
+                else:
+                    collect(wait_for=params.first)
+                continue
             steps = min(params.chunk, min(a[2] for a in active))
             batch.run([a[1] for a in active], steps, params.use_graph)
-            # arrivals are uploaded on their own streams while the batch runs
-            staged = admit_due(params.max_batch - len(active))
+            start_uploads(params.max_batch - len(active) + params.prefetch - len(staged) - len(uploading))
             batch.wait()
             done_t = time.perf_counter() - t0
             still = []
@@ -211,10 +229,10 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
                     still.append(a)
             active = still
     finally:
-        ex.shutdown()
+        ex.shutdown(wait=True)
         batch.close()
         makespan = time.perf_counter() - t0
-        for a in active + staged:
+        for a in active + staged + list(uploading.values()):
             pool.append(a[1])
         if own:
             for s in pool:
@@ -260,7 +278,7 @@ def autotune(sample: list[Request], device: int = 0, candidates=None) -> tuple[P
         candidates = [Params(chunk=c, max_batch=b, use_graph=g)
                       for c in (15, 30, 60) for b in (64, 256) for g in (False, True)]
     trials = []
-    pool = [Solver(device) for _ in range(max(p.max_batch for p in candidates))]
+    pool = [Solver(device) for _ in range(max(p.max_batch + p.prefetch for p in candidates))]
     for p in candidates:
         _, ms = serve(sample, p, device, realtime=False, pool=pool)
         trials.append({"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
