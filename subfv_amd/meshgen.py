@@ -12,6 +12,7 @@ Every random number comes from a counter-based generator keyed by
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -313,3 +314,61 @@ def write_msh(mesh: Mesh, path: str, bc_names=()) -> None:
     lines.append("$EndElements")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
+
+
+_GMSH_NV = {2: 3, 3: 4, 4: 4, 5: 8, 6: 6, 7: 5, 15: 1, 1: 2}
+
+
+def read_msh(path: str, bc_names=(), boundary_2d: bool = False) -> Mesh:
+    """Reads a gmsh 2.2 ASCII mesh as read_mesh_msh_2 does
+    (src/mesh/mesh_reading_module.F90:989-1324): with BC names given, the
+    physical group named bc_names[j-1] tags its boundary faces j and every
+    other boundary face 0 (the default wall); triangles and quadrangles are
+    boundary faces, gmsh kinds 4-7 the 3-D elements, other items are skipped.
+    Raises ValueError where the reference stops ("No Physical names inside
+    mesh!!!", "No 3D elements in mesh !")."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    i = 0
+    bc_tag = {}
+    if bc_names:
+        while not lines[i].startswith("$PhysicalNames"):
+            if lines[i].startswith("$Nodes"):
+                raise ValueError("No Physical names inside mesh!!!")
+            i += 1
+        n = int(lines[i + 1].split()[0])
+        for line in lines[i + 2:i + 2 + n]:
+            parts = line.split(None, 2)
+            tag, name = int(parts[1]), parts[2].strip().strip('"')
+            for j, nm in enumerate(bc_names, start=1):
+                if nm.strip() == name:
+                    bc_tag[j] = tag
+    while not lines[i].startswith("$Nodes"):
+        i += 1
+    nv = int(lines[i + 1].split()[0])
+    coord = np.array([[float(x) for x in line.split()[1:4]] for line in lines[i + 2:i + 2 + nv]], np.float64)
+    while not lines[i].startswith("$Elements"):
+        i += 1
+    n_items = int(lines[i + 1].split()[0])
+    kinds, ev, bnv, bv, btag = [], [], [], [], []
+    for line in lines[i + 2:i + 2 + n_items]:
+        f = [int(x) for x in line.split()]
+        kind, ntag = f[1], f[2]
+        ptag = f[3] if ntag > 0 else 0
+        verts = f[3 + ntag:3 + ntag + _GMSH_NV.get(kind, 0)]
+        if kind in (2, 3):
+            bnv.append(len(verts))
+            bv.extend(verts)
+            t = 0
+            for j in range(1, len(bc_names) + 1):
+                if bc_tag.get(j) == ptag:
+                    t = j
+            btag.append(t)
+        elif 4 <= kind <= 7:
+            kinds.append(kind)
+            ev.extend(verts)
+    if not kinds:
+        raise ValueError("[-] No 3D elements in mesh !")
+    return Mesh(coord, np.array(kinds, np.int32), np.array(ev, np.int32), np.array(bnv, np.int32),
+                np.array(bv, np.int32), np.array(btag, np.int32), boundary_2d=boundary_2d,
+                name=os.path.basename(path))

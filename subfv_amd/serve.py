@@ -47,6 +47,8 @@ class Request:
     host: HostMesh | None = None
     boundary_2d: bool = True
     cells: int = 0
+    mesh_path: str | None = None  # a msh 2.2 file instead of the generated mesh
+    bc_names: tuple = ()
 
 
 def make_requests(n_req: int = 4096, rate: float = 200.0, steps: int = 300, seed: int = SEED,
@@ -71,10 +73,15 @@ def make_requests(n_req: int = 4096, rate: float = 200.0, steps: int = 300, seed
 
 
 def prepare(reqs: list[Request], workers: int = 16) -> list[Request]:
-    """Client side: generate each mesh and build its connectivity (threads;
-    the C-ABI call releases the interpreter lock)."""
+    """Client side: generate each mesh (or read its msh 2.2 file) and build
+    its connectivity (threads; the C-ABI call releases the interpreter lock)."""
     def one(r: Request):
-        mesh = mg.quad_mesh(r.n, seed=SEED + r.rid) if r.kind == "quad" else mg.tri_mesh(r.n, seed=SEED + r.rid)
+        if r.mesh_path:
+            mesh = mg.read_msh(r.mesh_path, r.bc_names, r.boundary_2d)
+        elif r.kind == "quad":
+            mesh = mg.quad_mesh(r.n, seed=SEED + r.rid)
+        else:
+            mesh = mg.tri_mesh(r.n, seed=SEED + r.rid)
         r.host = HostMesh(mesh)
         r.boundary_2d = mesh.boundary_2d
         r.cells = mesh.n_elems

@@ -35,6 +35,10 @@ def test_fortran_reader_and_host_build(tmp_path, name, mesh, bcs):
         arrays.append(raw[off + 1:off + 1 + n])
         off += 1 + n
     hm = HostMesh(mesh)
+    # the Python reader gives the library the same mesh
+    hm_py = HostMesh(mg.read_msh(path, bcs, mesh.boundary_2d))
+    for k, name in enumerate(("sub_face_left_elem", "elem_vert", "face_right")):
+        assert np.array_equal(arrays[k], hm_py.geti(name)), name
     assert np.array_equal(arrays[0], hm.geti("sub_face_left_elem"))
     assert np.array_equal(arrays[1], hm.geti("elem_vert"))
     # face_right carries minus the boundary index the reader gave each face
