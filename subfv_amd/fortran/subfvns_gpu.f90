@@ -185,5 +185,22 @@ program subfvns_gpu
   write (funit) int(n_elems, c_int32_t)
   write (funit) sol
   close (funit)
+  ! the final write_sol of ns_main.F90 (iaff = -1): write_sol_dat
+  ! (ns_io_module.F90:50-96), the cells strictly inside the *_dat box
+  if (plot_solution_dat) then
+    open (newunit=funit, file="output_-1.dat", status="replace")
+    write (funit, *) ""
+    do i = 1, n_elems
+      if ((xc(1, i) < xmax_dat .and. xc(1, i) > xmin_dat) .and. &
+          (xc(2, i) < ymax_dat .and. xc(2, i) > ymin_dat) .and. &
+          (xc(3, i) < zmax_dat .and. xc(3, i) > zmin_dat)) then
+        w(1) = sol(1, i)
+        w(2:4) = sol(2:4, i) / sol(1, i)
+        w(5) = (gamma - 1) * (sol(5, i) - 0.5_DP * w(1) * (w(2)**2 + w(3)**2 + w(4)**2))
+        write (funit, *) xc(:, i), sol(:, i), w(:)
+      end if
+    end do
+    close (funit)
+  end if
   call subfv_destroy(ctx)
 end program subfvns_gpu

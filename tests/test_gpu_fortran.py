@@ -39,6 +39,12 @@ final_res = -1.0
 n_max_iter = {iters}
 n_iter_write_sol = 0
 plot_solution_dat = .true.
+xmin_dat = -10.0
+xmax_dat = 10.0
+ymin_dat = -10.0
+ymax_dat = 10.0
+zmin_dat = -10.0
+zmax_dat = 10.0
 /
 """
 
@@ -76,3 +82,16 @@ def test_fortran_driver_matches_oracle(tmp_path, scheme, second, periodic):
     sol = raw[4:].view(np.float64).reshape(n, 5)
     assert st == 0
     assert np.array_equal(sol, ref), np.abs(sol - ref).max()
+    # write_sol_dat of the final write_sol: centre, conservative and
+    # primitive state of every cell inside the box (here all of them)
+    # list-directed records may wrap over several lines: read the numbers
+    vals = np.array((tmp_path / "output_-1.dat").read_text().split(), dtype=np.float64)
+    assert vals.size == 13 * n
+    dat = vals.reshape(n, 13)
+    scale = np.abs(ref).max()
+    assert np.allclose(dat[:, :3], om.elem_coord, rtol=0, atol=1e-15)
+    assert np.allclose(dat[:, 3:8], ref, rtol=0, atol=1e-15 * scale)
+    w = ref.copy()
+    w[:, 1:4] = ref[:, 1:4] / ref[:, :1]
+    w[:, 4] = 0.4 * (ref[:, 4] - 0.5 * w[:, 0] * (w[:, 1] ** 2 + w[:, 2] ** 2 + w[:, 3] ** 2))
+    assert np.allclose(dat[:, 8:], w, rtol=0, atol=1e-14 * scale)
