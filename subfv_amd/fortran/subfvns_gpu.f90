@@ -3,7 +3,8 @@
 ! ns_global_data_module.F90:177-213; every entry is accepted, the ones the
 ! explicit path and its viscous terms use are honoured) and its msh 2.2 mesh,
 ! builds the mesh on
-! the device, sets the initial state (init_uniform or init_1drp,
+! the device, sets the initial state (init_uniform, init_1drp,
+! init_isentropic_vortex or init_gresho,
 ! ns_euler_module.F90:20-70), runs the time loop of ns_main.F90:83-107 with
 ! the state resident in HBM, prints the final time and the wall time of the
 ! loop (as ns_main.F90:113-126 does) and writes the final sol(5, n_elems) to
@@ -153,11 +154,16 @@ program subfvns_gpu
     ! init_sol's init_1drp branch on the device (ns_euler_module.F90:36-43)
     call check(ctx, subfv_init_1drp(ctx, x1drp, sol_w_1drp_l, sol_w_1drp_r))
   else
+    ! the other init_sol branches in the reference's order (ns_euler_module.F90:32-53)
     do i = 1, n_elems
       if (init_uniform) then
         w = sol_uniform
+      else if (init_isentropic_vortex) then
+        w = isentropic_vortex(xc(:, i))
+      else if (init_gresho) then
+        w = gresho_state(xc(:, i), 1.0e-5_DP)
       else
-        error stop "only init_uniform and init_1drp are supported"
+        error stop "only init_uniform, init_1drp, init_isentropic_vortex and init_gresho are supported"
       end if
       sol(1, i) = w(1)
       sol(2:4, i) = w(1) * w(2:4)
@@ -203,4 +209,44 @@ program subfvns_gpu
     close (funit)
   end if
   call subfv_destroy(ctx)
+
+contains
+
+  ! sol_gresho_mach (ns_euler_exact_sol_module.F90:7-44): the Gresho vortex
+  ! of radius 0.4 around the origin, background pressure 1/(gamma Mach^2)
+  function gresho_state(x, mach) result(w)
+    real(DP), intent(in) :: x(3), mach
+    real(DP) :: w(5), p0, r
+    p0 = 1.0_DP / (gamma * mach**2)
+    r = norm2(x(1:2))
+    w(1) = 1.0_DP
+    w(4) = 0.0_DP
+    if (r < 0.2_DP) then
+      w(2) = -5.0_DP * x(2)
+      w(3) = 5.0_DP * x(1)
+      w(5) = p0 + 12.5_DP * r**2
+    else if (r > 0.2_DP .and. r < 0.4_DP) then
+      w(2) = (5.0_DP - 2.0_DP / r) * x(2)
+      w(3) = (-5.0_DP + 2.0_DP / r) * x(1)
+      w(5) = p0 + 12.5_DP * r**2 + 4.0_DP - 20.0_DP * r + 4.0_DP * log(5.0_DP * r)
+    else
+      w(2) = 0.0_DP
+      w(3) = 0.0_DP
+      w(5) = p0 - 2.0_DP + 4.0_DP * log(2.0_DP)
+    end if
+  end function
+
+  ! sol_isentropic_vortex at t = 0 (ns_euler_exact_sol_module.F90:47-74):
+  ! strength 5 at the origin, no advection
+  function isentropic_vortex(x) result(w)
+    real(DP), intent(in) :: x(3)
+    real(DP) :: w(5), r
+    real(DP), parameter :: beta = 5.0_DP, pi = 4.0_DP * atan(1.0_DP)
+    r = x(1)**2 + x(2)**2
+    w(1) = (1.0_DP - ((gamma - 1) * beta**2) / (8.0_DP * gamma * pi**2) * exp(1.0_DP - r))**(1.0_DP / (gamma - 1.0_DP))
+    w(2) = -x(2) * beta / (2.0_DP * pi) * exp(0.5_DP * (1.0_DP - r))
+    w(3) = x(1) * beta / (2.0_DP * pi) * exp(0.5_DP * (1.0_DP - r))
+    w(4) = 0.0_DP
+    w(5) = w(1)**gamma
+  end function
 end program subfvns_gpu
