@@ -162,7 +162,7 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
     (or all that are under way) before it starts.  `pool`: idle contexts to
     admit into (a server keeps them between calls; they are returned to it);
     without one, contexts are created as needed and closed at the end."""
-    batch = Batch(device)
+    batch = None    # created with the first chunk
     own = pool is None
     pool = [] if own else pool
     pending = sorted(reqs, key=lambda r: r.arrival)
@@ -200,7 +200,11 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
                 done |= d
         for f in [f for f in uploading if f.done()]:
             entry = uploading.pop(f)
-            f.result()  # an upload error ends the serve
+            try:
+                f.result()  # an upload error ends the serve
+            except Exception:
+                pool.append(entry[1])
+                raise
             staged.append(entry)
 
     ex = ThreadPoolExecutor(max_workers=params.admit_workers)
@@ -212,12 +216,17 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
             del staged[:max(room, 0)]
             if not active:
                 start_uploads(params.max_batch + params.prefetch - len(uploading))
-                if not uploading:
-                    time.sleep(max(0.0, pending[pi].arrival - (time.perf_counter() - t0)))
-                else:
+                if uploading:
                     collect(wait_for=params.first)
+                elif pending[pi].arrival <= now():
+                    raise SubfvError(2, f"request {pending[pi].rid} needs {device_bytes(pending[pi])} bytes, "
+                                        f"more than the HBM budget of {params.budget}")
+                else:
+                    time.sleep(max(0.0, pending[pi].arrival - (time.perf_counter() - t0)))
                 continue
             steps = min(params.chunk, min(a[2] for a in active))
+            if batch is None:
+                batch = Batch(device)
             batch.run([a[1] for a in active], steps, params.use_graph)
             start_uploads(params.max_batch - len(active) + params.prefetch - len(staged) - len(uploading))
             batch.wait()
@@ -237,7 +246,8 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
             active = still
     finally:
         ex.shutdown(wait=True)
-        batch.close()
+        if batch is not None:
+            batch.close()
         makespan = time.perf_counter() - t0
         for a in active + staged + list(uploading.values()):
             pool.append(a[1])

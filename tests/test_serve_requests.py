@@ -51,3 +51,13 @@ def test_requests_from_mesh_files(tmp_path):
         assert a.cells == b.cells
         for name in ("elem_vert", "sub_face_left_elem", "face_right", "vert_sub_face"):
             assert np.array_equal(a.host.geti(name), b.host.geti(name)), name
+
+
+def test_a_request_beyond_the_budget_is_refused():
+    """Admission control: a request that cannot fit the HBM budget even on
+    an idle server is an error, not a wait (no device call is made)."""
+    import pytest
+    reqs = S.make_requests(1, rate=10.0, cmin=8 ** 2, cmax=8 ** 2)
+    reqs[0].cells = 64
+    with pytest.raises(S.SubfvError, match="HBM budget"):
+        S.serve(reqs, S.Params(budget=1024), realtime=False, pool=[])
