@@ -153,7 +153,7 @@ class Params:
 
 
 def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep: bool = False,
-          realtime: bool = True, pool: list | None = None) -> tuple[list[Result], float]:
+          realtime: bool = True, pool: list | None = None, on_finish=None) -> tuple[list[Result], float]:
     """Continuous batching.  Returns the results (latency = finish - arrival)
     and the makespan in seconds.  Arrived requests are uploaded into idle
     contexts by a pool of host threads in the background, up to the free
@@ -161,7 +161,9 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
     next chunk boundary; an idle server waits for `params.first` uploads
     (or all that are under way) before it starts.  `pool`: idle contexts to
     admit into (a server keeps them between calls; they are returned to it);
-    without one, contexts are created as needed and closed at the end."""
+    without one, contexts are created as needed and closed at the end.
+    `on_finish(request, solver)`: called as each solve finishes, before its
+    context is reused (e.g. to hand a snapshot to output.AsyncWriter)."""
     batch = None    # created with the first chunk
     own = pool is None
     pool = [] if own else pool
@@ -239,6 +241,8 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
                     s.run(0, 0)  # raises the instance's error, if any
                     sol = s.get_sol() if keep else None
                     results.append(Result(r.rid, done_t - r.arrival, done_t, sol))
+                    if on_finish is not None:
+                        on_finish(r, s)
                     pool.append(s)
                     used -= device_bytes(r)
                 else:

@@ -75,3 +75,22 @@ def test_periodic_instances_in_a_batch_equal_solves_alone(requests):
     m = {r.rid: r.sol for r in mixed}
     for rid in a:
         assert np.array_equal(a[rid], m[rid]), rid
+
+
+def test_finished_solves_are_written_in_the_background(requests, tmp_path):
+    """on_finish hands each finished solve's snapshot to the asynchronous
+    writer; the written tables hold the solves' final states."""
+    print("[gpu] serve async output", flush=True)
+    from subfv_amd import output as OUT
+    wr = OUT.AsyncWriter(workers=2)
+    futs = {}
+
+    def on_finish(r, s):
+        futs[r.rid] = wr.dat(str(tmp_path / f"req{r.rid}.dat"), OUT.solver_snapshot(s))
+
+    res, _ = S.serve(requests, S.Params(chunk=6, max_batch=3), keep=True, realtime=False, on_finish=on_finish)
+    wr.close()
+    for r in res:
+        assert futs[r.rid].exception() is None
+        rows = np.loadtxt(tmp_path / f"req{r.rid}.dat").reshape(-1, 13)
+        assert np.array_equal(rows[:, 3:8], r.sol)
