@@ -4,13 +4,16 @@
 50-96): a blank first line, then one record per cell strictly inside the
 box -- centre, conservative state, primitive state.  `write_sol_vtu` writes
 an ASCII VTK unstructured grid with the reference's cell-data names for the
-state (write_sol_vtu, :403-884, whose other fields -- Hessians, nodal
-gradients, Ducros sensor -- are not written).  `AsyncWriter` runs writes on
+state (write_sol_vtu, :403-884: Energy per unit mass, Temperature from
+temp_u; its other fields -- Hessians, nodal gradients, Ducros sensor -- are
+not written), and `restart_from_vtu` reads the state back as the
+reference's restart does.  `AsyncWriter` runs writes on
 host threads: the caller hands over a snapshot (arrays it no longer
 changes) and goes on, so a served loop never waits on a file.
 """
 from __future__ import annotations
 
+import xml.etree.ElementTree as ET
 from concurrent.futures import Future, ThreadPoolExecutor
 
 import numpy as np
@@ -68,7 +71,8 @@ def write_sol_vtu(path: str, vert_coord: np.ndarray, elem_kind: np.ndarray, elem
     w = primitives(sol)
     a = np.sqrt(GAMMA * w[:, 4] / w[:, 0])
     speed = np.sqrt(w[:, 1] ** 2 + w[:, 2] ** 2 + w[:, 3] ** 2)
-    temp = (sol[:, 4] / sol[:, 0] - 0.5 * speed ** 2) / cv
+    tau = 1.0 / sol[:, 0]  # temp_u (ns_euler_primitives_module.F90:52-63)
+    temp = (sol[:, 4] * tau - 0.5 * (sol[:, 1] ** 2 + sol[:, 2] ** 2 + sol[:, 3] ** 2) * tau ** 2) / cv
     with open(path, "w") as fh:
         fh.write('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian">\n')
         fh.write(f'<UnstructuredGrid>\n<Piece NumberOfPoints="{X.shape[0]}" NumberOfCells="{kind.size}">\n')
@@ -85,12 +89,27 @@ def write_sol_vtu(path: str, vert_coord: np.ndarray, elem_kind: np.ndarray, elem
             _array(fh, "Tag", elem_tag, 1, "Int32", "%d")
         _array(fh, "Density", sol[:, 0], 1)
         _array(fh, "Momentum", sol[:, 1:4], 3)
-        _array(fh, "Energy", sol[:, 4], 1)
+        _array(fh, "Energy", sol[:, 4] / sol[:, 0], 1)  # per unit mass, as the reference writes it
         _array(fh, "Velocity", w[:, 1:4], 3)
         _array(fh, "Pressure", w[:, 4], 1)
         _array(fh, "Mach", speed / a, 1)
         _array(fh, "Temperature", temp, 1)
         fh.write("</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n")
+
+
+def restart_from_vtu(path: str) -> np.ndarray:
+    """The state of restart_from_vtu_file (ns_io_module.F90:1333-1382) from a
+    grid this module wrote: Density, Momentum and Energy (per unit mass, so
+    multiplied back by the density) of every cell, as sol (C, 5)."""
+    cells = {a.get("Name"): a for a in ET.parse(path).getroot().iter("DataArray")}
+    rho = np.array(cells["Density"].text.split(), np.float64)
+    mom = np.array(cells["Momentum"].text.split(), np.float64).reshape(-1, 3)
+    e = np.array(cells["Energy"].text.split(), np.float64)
+    sol = np.empty((rho.size, 5))
+    sol[:, 0] = rho
+    sol[:, 1:4] = mom
+    sol[:, 4] = e * rho
+    return sol
 
 
 def solver_snapshot(solver) -> dict:

@@ -58,6 +58,11 @@ def test_vtu_mixed_kinds(tmp_path):
         assert set(a["types"]) == {OUT.VTK_TYPE[int(mesh.kind[0])]}
         assert np.array_equal(a["Density"], sol[:, 0]) and np.array_equal(a["Momentum"].reshape(-1, 3), sol[:, 1:4])
         assert np.allclose(a["Pressure"], w[:, 4], rtol=1e-14)
+        assert np.allclose(a["Energy"], sol[:, 4] / sol[:, 0], rtol=1e-15)
+        # restart_from_vtu_file multiplies the specific energy back by rho
+        back = OUT.restart_from_vtu(path)
+        assert np.array_equal(back[:, :4], sol[:, :4])
+        assert np.allclose(back[:, 4], sol[:, 4], rtol=1e-15)
 
 
 def test_async_writer(tmp_path):
