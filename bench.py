@@ -104,6 +104,47 @@ def cpu_baseline(n=192, steps=1, viscous=True, first_order=False):
             "sample": f"tri {n}x{n} ({om.n_elems} prisms), " + what}
 
 
+def bcsr_bench(n=512, restart=30):
+    """The implicit low-Mach system (512^2 periodic quads, 4x4 blocks, Mach
+    0.01 vortex, backward Euler at acoustic CFL 5): bcsr_gmres_diag to
+    ||r|| <= 1e-10 ||b|| from x = 0, in FP64 and with the SpMV and
+    block-Jacobi of each cycle in FP32 (FP64 restarts).  Wall time of the
+    device-resident solve (one call: rhs in, x out), SpMV count, the SpMV's
+    algorithmic bytes (blocks, column ids, row offsets, x read once, y
+    written) and the solution difference against FP64."""
+    import numpy as np
+
+    from subfv_amd import lowmach as lm
+    from subfv_amd.api import Bcsr, Solver
+
+    sysm = lm.build_system(n)
+    m, nnzb = sysm["m"], len(sysm["col"])
+    rhs = sysm["rhs"]
+    tol = 1e-10 * float(np.linalg.norm(rhs))
+    s = Solver(0)
+    A = Bcsr(s, m, 4, sysm["row"], sysm["col"], sysm["v"])
+    out = {"system": f"periodic quad {n}x{n}, 4x4 blocks, m={m}, nnzb={nnzb}", "restart": restart,
+           "tol_rel": 1e-10}
+    xs = {}
+    for prec, name in ((0, "fp64"), (1, "fp32_inner")):
+        A.gmres_diag(rhs, tol, restart, prec)  # warm-up (factors, work arrays)
+        t0 = time.perf_counter()
+        x, conv, st = A.gmres_diag(rhs, tol, restart, prec)
+        el = time.perf_counter() - t0
+        xs[name] = x
+        vbytes = 4 if prec else 8
+        spmv_bytes = nnzb * (16 * vbytes + 4) + (m + 1) * 4 + 2 * m * 32
+        out[name] = {"seconds": el, "converged": conv, "restarts": st["restarts"], "spmv": st["spmv"],
+                     "rel_residual": st["rho"] / float(np.linalg.norm(rhs)),
+                     "spmv_bytes": spmv_bytes,
+                     "effective_spmv_GBs_incl_orthogonalisation": st["spmv"] * spmv_bytes / el / 1e9}
+    out["fp32_inner_rel_diff_vs_fp64"] = float(np.linalg.norm(xs["fp32_inner"] - xs["fp64"]) /
+                                               np.linalg.norm(xs["fp64"]))
+    A.close()
+    s.close()
+    return out
+
+
 def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cmax=512 ** 2):
     """Served mixed-size first-order solves (quad/tri 50/50, 32^2..512^2
     cells log-uniform, seeded discontinuities, `steps` steps each, Poisson
@@ -152,6 +193,10 @@ def main():
     ap.add_argument("--inviscid", action="store_true", help="drop the viscous terms (Euler only)")
     ap.add_argument("--served-requests", type=int, default=192,
                     help="requests in the served-solves measurement (0 skips it)")
+    ap.add_argument("--no-bcsr", dest="bcsr", action="store_false",
+                    help="skip the implicit low-Mach GMRES measurement")
+    ap.add_argument("--bcsr-only", action="store_true",
+                    help="only the implicit low-Mach GMRES measurement (one JSON object) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against known bytes) and exit")
@@ -160,6 +205,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path is single-GPU; multi-GPU sharding is design-only).", file=sys.stderr)
         sys.exit(2)
+    if args.bcsr_only:
+        print(json.dumps(bcsr_bench()), flush=True)
+        return
     if args.copy_probe:
         import torch
 
@@ -276,6 +324,8 @@ def main():
         # the same loop on small meshes (32^2 .. 96^2 cells), where one solve
         # alone cannot fill the GPU and launches dominate the plain path
         out["served_small"] = served_bench(2 * args.served_requests, rate=4000.0, calib_n=96, cmax=96 ** 2)
+    if args.bcsr and not args.first_order:
+        out["bcsr_gmres"] = bcsr_bench()
     if not args.no_cpu_baseline:
         # samples of about 10-15 s of single-core work
         if args.first_order:
