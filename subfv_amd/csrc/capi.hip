@@ -1381,7 +1381,7 @@ struct subfv_bcsr {
   // GMRES workspace
   int mr = 0;
   double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
-  double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr;
+  double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr, *part2 = nullptr;
   unsigned int *ticket = nullptr;
   // host copies (0-based) for add_block and the sweep schedules
   std::vector<int> h_row, h_col, h_diag;
@@ -1618,6 +1618,7 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   if (!r) r = balloc(A, &A->scal, 16);
   if (!r) r = balloc(A, &A->z, 256);
   if (!r) r = balloc(A, &A->part, kRedBlocks);
+  if (!r) r = balloc(A, &A->part2, kRedBlocks);
   if (!r) r = balloc(A, &A->ticket, 1);
   if (!r) r = balloc(A, &A->perr, 1);
   if (r) {
@@ -1730,11 +1731,18 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
           if ((r = spmv(A, fp32, A->tmp, Vc(j)))) return r;
         }
         nspmv++;
+        // modified Gram-Schmidt against V(1..j-1): each step sums the
+        // previous step's dot (h(i-1, j-1)) itself, so partial buffers alternate
         double *w = Vc(j);
-        mgs_step(A, w, nullptr, nullptr, Vc(1), Hd(1, j - 1));
-        for (int i = 2; i <= j - 1; i++) mgs_step(A, w, Vc(i - 1), Hd(i - 1, j - 1), Vc(i), Hd(i, j - 1));
-        mgs_step(A, w, Vc(j - 1), Hd(j - 1, j - 1), nullptr, A->scal + 1);
-        hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, w, w, A->scal + 1, 1, Hd(j, j - 1));
+        double *pa = A->part, *pb = A->part2;
+        hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)nullptr, (const double *)nullptr,
+                           (double *)nullptr, (const double *)Vc(1), pa);
+        for (int i = 2; i <= j; i++) {
+          hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)Vc(i - 1), (const double *)pa,
+                             Hd(i - 1, j - 1), (const double *)(i < j ? Vc(i) : nullptr), pb);
+          std::swap(pa, pb);
+        }
+        hipLaunchKernelGGL(k_scale_norm, vg, vb, 0, s, n, w, (const double *)pa, Hd(j, j - 1));
       }
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipMemcpyAsync(H.data(), A->H, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -291,6 +291,55 @@ __global__ __launch_bounds__(kVecThreads) void k_mgs_step(long n, double *w, con
   block_sum_store(acc, part);
 }
 
+// The sum of the previous dot's per-block partials, in the order of
+// k_sum_finalize, computed by every block of the kernel that needs it (the
+// kernel boundary orders the partials' writes before these reads); block 0
+// stores it to *h_out.
+SFV_DEV double block_total(const double *part, int nblk, double *h_out) {
+  __shared__ double s[kVecThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kVecThreads) acc = acc + part[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kVecThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double t = s[0];
+  if (h_out && blockIdx.x == 0 && threadIdx.x == 0) *h_out = t;
+  __syncthreads();  // s is reused by the caller's reduction
+  return t;
+}
+
+// One step of the Arnoldi MGS without a separate finalize: h = the sum of
+// part_prev (the dot of the previous step, stored to *h_out), w = w - h *
+// v_prev, then per-block partials of <v_next, w> (or <w, w> when v_next is
+// null) into part_out (a different buffer from part_prev).
+__global__ __launch_bounds__(kVecThreads) void k_mgs_chain(long n, double *w, const double *v_prev,
+                                                           const double *part_prev, double *h_out,
+                                                           const double *v_next, double *part_out) {
+  const double hh = v_prev ? block_total(part_prev, (int)gridDim.x, h_out) : 0.0;
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double x = w[i];
+    if (v_prev) {
+      x = x - hh * v_prev[i];
+      w[i] = x;
+    }
+    acc = acc + (v_next ? v_next[i] * x : x * x);
+  }
+  block_sum_store(acc, part_out);
+}
+
+// w = w / sqrt(sum of part) (the norm of the last MGS step; block 0 stores
+// it to *h_out)
+__global__ __launch_bounds__(kVecThreads) void k_scale_norm(long n, double *w, const double *part, double *h_out) {
+  const double d = sqrt(block_total(part, (int)gridDim.x, nullptr));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *h_out = d;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    w[i] = w[i] / d;
+}
+
 // out = a / *s   (normalisation by a device scalar, sqrt taken if take_sqrt)
 __global__ void k_scale_div(long n, const double *a, double *out, const double *s, int take_sqrt, double *s_out) {
   double d = take_sqrt ? sqrt(*s) : *s;
