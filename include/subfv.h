@@ -24,6 +24,7 @@
  *   subfv_add_timedisc_and_solve add_timedisc_and_solve (src/ns/ns_euler_module.F90:521-609)
  *   subfv_compute_rhs_ns        compute_rhs_ns (src/ns/ns_main.F90:130-179)
  *   subfv_set_diffusion         activate_diffusion and the viscous namelist entries
+ *   subfv_set_sutherland        use_sutherland, mu0, T0 (compute_mu)
  *                               (src/ns/ns_global_data_module.F90:8-12,177-213)
  *   subfv_run                   the time loop of src/ns/ns_main.F90:83-107
  *   subfv_bcsr_*                bcsr_matrix_type, bcsr_matmul, bcsr_diag, bcsr_gmres_diag, bcsr_gmres,
@@ -174,7 +175,8 @@ int subfv_compute_rhs_ns(subfv_ctx *ctx);
 /* Viscous terms of compute_rhs_ns (activate_diffusion, ns_main.F90:170-175):
  * build_momentum_diff_mat_into_global_mat_diag (ns_vectorial_diffusion_module.F90:200-320)
  * and build_diff_mat_into_global_mat_diag (ns_diffusion_module.F90:112-184) with a
- * constant viscosity mu_p (compute_mu without Sutherland, :322-347), Cv_p and the
+ * viscosity mu_p (compute_mu without Sutherland, :322-347; subfv_set_sutherland
+ * switches to the per-cell law), Cv_p and the
  * Prandtl number (a compile-time 0.71 in the reference).  Per BC index
  * i = 1..n_bc (arrays 0-based): bc_v_id / bc_t_id 1 = neumann, 2 = dirichlet
  * (bc_type_V / bc_type_T), bc_val_v(3, n_bc), bc_val_t(n_bc).  Applies to
@@ -183,6 +185,15 @@ int subfv_compute_rhs_ns(subfv_ctx *ctx);
 int subfv_set_diffusion(subfv_ctx *ctx, int32_t activate, double mu_p, double cv_p, double prandtl,
                         int32_t n_bc, const int32_t *bc_v_id, const double *bc_val_v, const int32_t *bc_t_id,
                         const double *bc_val_t);
+
+/* use_sutherland of compute_mu (ns_vectorial_diffusion_module.F90:322-359,
+ * namelist entries use_sutherland, mu0, T0 of ns_global_data_module.F90:13-20):
+ * with activate != 0 the viscous terms take each cell's viscosity
+ * mu0 (T/T0)^1.5 (T0 + 110.4) / (T + 110.4), T = temp_u of the cell state,
+ * and mu0corr = the minimum over the cells; mu_p of subfv_set_diffusion is
+ * then unused.  The nodal systems are re-formed before every residual.
+ * Stays set across subfv_set_diffusion calls. */
+int subfv_set_sutherland(subfv_ctx *ctx, int32_t activate, double mu0, double T0);
 
 /* The time loop on device-resident state without host synchronisation.
  * time_scheme 0: the reference loop (forward Euler, or its two-stage scheme

@@ -1,9 +1,10 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- the oracle (see om_internal.h).
  *
- * Viscous terms of compute_rhs_ns (src/ns/ns_main.F90:170-175) with a
- * constant viscosity (compute_mu with use_sutherland = .false.,
- * ns_vectorial_diffusion_module.F90:322-347: mu_arr = mu_p, mu0corr = mu_p):
+ * Viscous terms of compute_rhs_ns (src/ns/ns_main.F90:170-175) with the
+ * viscosity of compute_mu (ns_vectorial_diffusion_module.F90:322-359):
+ * mu_arr = mu_p and mu0corr = mu_p, or with use_sutherland the per-cell
+ * mu_sutherland(temp_u(sol)) and mu0corr = their minimum:
  *   build_momentum_diff_nodal_system        ns_vectorial_diffusion_module.F90:7-198
  *   build_momentum_diff_mat_into_global_mat_diag                          :200-320
  *   build_diff_nodal_system                 ns_diffusion_module.F90:7-110
@@ -173,9 +174,31 @@ static void dtheta_du(const om_config *cfg, const double *u, double *P) {
 
 static double kappa(const om_config *cfg, double mu) { return mu * cfg->Cv_p * GAMMA / cfg->prandtl; }
 
+/* mu_sutherland (ns_vectorial_diffusion_module.F90:349-359), C0 = 110.4
+ * (ns_global_data_module.F90:20); T**1.5 through pow */
+double om_mu_sutherland(const om_config *cfg, const double *u) {
+  const double T = temp_u(cfg, u);
+  return cfg->mu0 * pow(T / cfg->T0, 1.5) * (cfg->T0 + 110.4) / (T + 110.4);
+}
+
+/* compute_mu (:322-347): mu of every cell (mu[0..ne-1]) and the return value
+ * mu0corr, the minimum (from 1e12) with use_sutherland, else mu_p */
+double om_compute_mu(int ne, const om_config *cfg, const double *sol, double *mu) {
+  if (!cfg->use_sutherland) {
+    for (int c = 0; c < ne; c++) mu[c] = cfg->mu_p;
+    return cfg->mu_p;
+  }
+  double mn = 1e12;
+  for (int c = 0; c < ne; c++) {
+    mu[c] = om_mu_sutherland(cfg, sol + 5L * c);
+    mn = fmin(mn, mu[c]);
+  }
+  return mn;
+}
+
 /* -------------------------------------------------------------- heat */
-static void heat_nodal_system(const om_mesh *m, const om_config *cfg, int v, double *N, double *S, double *StT,
-                              double *B) {
+static void heat_nodal_system(const om_mesh *m, const om_config *cfg, const double *mu_arr, int v, double *N,
+                              double *S, double *StT, double *B) {
   const int nf = m->v_nsfn[v], ns = m->v_nsen[v];
   memset(N, 0, sizeof(double) * nf * nf);
   memset(S, 0, sizeof(double) * nf * ns);
@@ -201,8 +224,7 @@ static void heat_nodal_system(const om_mesh *m, const om_config *cfg, int v, dou
       }
       const int id_elem = side == 0 ? m->f_left[ifl] : m->f_right[ifl];
       const int so = m->se_loc[ise] - 1;
-      const double kap = kappa(cfg, cfg->mu_p);
-      (void)id_elem;
+      const double kap = kappa(cfg, mu_arr[id_elem]);
       for (int p = 1; p <= m->se_nsf[ise]; p++) {
         const int isfp = m->se_sf[ise][p];
         const int po = m->sf_loc[isfp] - 1;
@@ -217,7 +239,8 @@ static void heat_nodal_system(const om_mesh *m, const om_config *cfg, int v, dou
   }
 }
 
-static void heat_into_global(const om_mesh *m, const om_config *cfg, const double *sol, double *rhs, double *diag) {
+static void heat_into_global(const om_mesh *m, const om_config *cfg, const double *mu_arr, const double *sol,
+                             double *rhs, double *diag) {
   const int cap = m->nmax_sfn > 0 ? m->nmax_sfn : 1;
   double *N = (double *)malloc(sizeof(double) * cap * cap);
   double *S = (double *)malloc(sizeof(double) * cap * cap);
@@ -230,7 +253,7 @@ static void heat_into_global(const om_mesh *m, const om_config *cfg, const doubl
   double *b1 = (double *)malloc(sizeof(double) * cap);
   for (int v = 1; v <= m->nv; v++) {
     const int nf = m->v_nsfn[v], ns = m->v_nsen[v];
-    heat_nodal_system(m, cfg, v, N, S, StT, B);
+    heat_nodal_system(m, cfg, mu_arr, v, N, S, StT, B);
     om_inv(nf, N);
     matmul3(StT, N, S, SNS, ns, nf, nf, ns);
     matvec2(StT, N, B, SB, ns, nf, nf);
@@ -278,11 +301,10 @@ static void add3t(double *M, int ld, int i0, int j0, const double *b) {
     for (int i = 0; i < 3; i++) M[(i0 + i) + (long)ld * (j0 + j)] = M[(i0 + i) + (long)ld * (j0 + j)] + b[j + 3 * i];
 }
 
-static void mom_nodal_system(const om_mesh *m, const om_config *cfg, int v, double *N, double *S, double *StT,
-                             double *B) {
+static void mom_nodal_system(const om_mesh *m, const om_config *cfg, const double *mu_arr, double mu0, int v,
+                             double *N, double *S, double *StT, double *B) {
   const int nf = m->v_nsfn[v], ns = m->v_nsen[v];
   const int F = 3 * nf, E = 3 * ns;
-  const double mu = cfg->mu_p, mu0 = cfg->mu_p;
   memset(N, 0, sizeof(double) * F * F);
   memset(S, 0, sizeof(double) * F * E);
   memset(StT, 0, sizeof(double) * E * F);
@@ -315,7 +337,7 @@ static void mom_nodal_system(const om_mesh *m, const om_config *cfg, int v, doub
           const int ifp0 = 3 * (m->sf_loc[isfp] - 1);
           double np[3];
           for (int d = 0; d < 3; d++) np[d] = (m->sf_lse[isfp] == ise) ? SN(m, isfp, d) : -SN(m, isfp, d);
-          blc_mu(al, m->sf_area[isfp], m->se_vol[ise], mu, mu0, nl, np, blc);
+          blc_mu(al, m->sf_area[isfp], m->se_vol[ise], mu_arr[m->se_elem[ise]], mu0, nl, np, blc);
           add3t(StT, E, ise0, ifl0, blc);
           add3(N, F, ifl0, ifp0, blc);
           add3(S, F, ifl0, ise0, blc);
@@ -337,7 +359,7 @@ static void mom_nodal_system(const om_mesh *m, const om_config *cfg, int v, doub
           for (int i = 0; i < 3; i++) blc[i + 3 * j] = s * (dd * (i == j ? 1.0 : 0.0) + (np[j] * nl[i]) / 3.0);
         add3(N, F, ifl0, ifp0, blc);
         add3(S, F, ifl0, ise0, blc);
-        blc_mu(al, m->sf_area[isfp], m->se_vol[ise], mu, mu0, nl, np, blc);
+        blc_mu(al, m->sf_area[isfp], m->se_vol[ise], mu_arr[m->se_elem[ise]], mu0, nl, np, blc);
         add3t(StT, E, ise0, ifl0, blc);
       }
     } else {
@@ -349,7 +371,7 @@ static void mom_nodal_system(const om_mesh *m, const om_config *cfg, int v, doub
         const int isfp = m->se_sf[ise][p];
         double np[3];
         for (int d = 0; d < 3; d++) np[d] = (m->sf_lse[isfp] == ise) ? SN(m, isfp, d) : -SN(m, isfp, d);
-        blc_mu(al, m->sf_area[isfp], m->se_vol[ise], mu, mu0, nl, np, blc);
+        blc_mu(al, m->sf_area[isfp], m->se_vol[ise], mu_arr[m->se_elem[ise]], mu0, nl, np, blc);
         add3t(StT, E, ise0, ifl0, blc);
       }
       for (int d = 0; d < 3; d++) B[ifl0 + d] = al * cfg->bc_val_V[-id_elem_r][d];
@@ -362,7 +384,8 @@ static void mom_nodal_system(const om_mesh *m, const om_config *cfg, int v, doub
   }
 }
 
-static void mom_into_global(const om_mesh *m, const om_config *cfg, const double *sol, double *rhs, double *diag) {
+static void mom_into_global(const om_mesh *m, const om_config *cfg, const double *mu_arr, double mu0corr,
+                            const double *sol, double *rhs, double *diag) {
   const int capF = 3 * (m->nmax_sfn > 0 ? m->nmax_sfn : 1);
   double *N = (double *)malloc(sizeof(double) * capF * capF);
   double *S = (double *)malloc(sizeof(double) * capF * capF);
@@ -374,11 +397,10 @@ static void mom_into_global(const om_mesh *m, const om_config *cfg, const double
   double *Vf = (double *)malloc(sizeof(double) * capF);
   double *SNSVe = (double *)malloc(sizeof(double) * capF);
   double *blc3 = (double *)malloc(sizeof(double) * 9 * capF);
-  const double mu = cfg->mu_p;
   for (int v = 1; v <= m->nv; v++) {
     const int nf = m->v_nsfn[v], ns = m->v_nsen[v];
     const int F = 3 * nf, E = 3 * ns;
-    mom_nodal_system(m, cfg, v, N, S, StT, B);
+    mom_nodal_system(m, cfg, mu_arr, mu0corr, v, N, S, StT, B);
     for (int l = 0; l < ns; l++) {
       const int iel = m->se_elem[m->v_se[v][l + 1]];
       for (int k = 0; k < 3; k++) Ve[3 * l + k] = SOLV(sol, iel, k + 1) / SOLV(sol, iel, 0);
@@ -431,7 +453,7 @@ static void mom_into_global(const om_mesh *m, const om_config *cfg, const double
             Spc[i + 3 * j] = Spc[i + 3 * j] + m->sf_area[isfl] * (tp + tpt - (2.0 / 3.0) * dv * (i == j ? 1.0 : 0.0));
           }
       }
-      for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * mu / m->se_vol[isep];
+      for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * mu_arr[iep] / m->se_vol[isep];
       for (int l = 1; l <= m->se_nsf[isep]; l++) {
         const int isfl = m->se_sf[isep][l];
         const int lo = m->sf_loc[isfl] - 1;
@@ -449,8 +471,12 @@ static void mom_into_global(const om_mesh *m, const om_config *cfg, const double
 /* the diffusion block of compute_rhs_ns: momentum first, then heat */
 int om_add_diffusion(const om_mesh *m, const om_config *cfg, const double *sol, double *rhs, double *diag) {
   if (!cfg->activate_diffusion) return OM_OK;
-  mom_into_global(m, cfg, sol, rhs, diag);
-  heat_into_global(m, cfg, sol, rhs, diag);
+  double *mu_arr = (double *)malloc(sizeof(double) * ((size_t)m->ne + 1));
+  if (!mu_arr) return OM_ERR_BAD_INPUT;
+  const double mu0corr = om_compute_mu(m->ne, cfg, sol, mu_arr + 1);
+  mom_into_global(m, cfg, mu_arr, mu0corr, sol, rhs, diag);
+  heat_into_global(m, cfg, mu_arr, sol, rhs, diag);
+  free(mu_arr);
   return OM_OK;
 }
 

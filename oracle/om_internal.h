@@ -67,14 +67,17 @@ typedef struct {
   double cfl;
   double tmax;
   double t;                          /* physical time seen by time-dependent BCs */
-  /* viscous terms (compute_rhs_ns with activate_diffusion, ns_main.F90:170-175);
-   * constant viscosity (use_sutherland = .false.) */
+  /* viscous terms (compute_rhs_ns with activate_diffusion, ns_main.F90:170-175) */
   int activate_diffusion;
   double mu_p, Cv_p, prandtl;
   int bc_V_id[OM_MNBC + 1];          /* 1 neumann, 2 dirichlet (ns_global_data_module.F90:40-41) */
   int bc_T_id[OM_MNBC + 1];
   double bc_val_V[OM_MNBC + 1][3];
   double bc_val_T[OM_MNBC + 1];
+  /* compute_mu with use_sutherland (ns_vectorial_diffusion_module.F90:322-359):
+   * mu0 at T0 (ns_global_data_module.F90:17-20) */
+  int use_sutherland;
+  double mu0, T0;
 } om_config;
 
 typedef struct {

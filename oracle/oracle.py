@@ -64,15 +64,20 @@ class Config(C.Structure):
         ("bc_T_id", C.c_int * (MNBC + 1)),
         ("bc_val_V", (C.c_double * 3) * (MNBC + 1)),
         ("bc_val_T", C.c_double * (MNBC + 1)),
+        ("use_sutherland", C.c_int),
+        ("mu0", C.c_double),
+        ("T0", C.c_double),
     ]
 
 
 def make_config(scheme="multi_point", bc_style=2, boundary_2d=True, exclude_bound_vert=False,
                 local_time_step=False, second_order=False, method=4, bcs=(), cfl=0.4, tmax=1e30,
-                t=0.0, diffusion=False, mu=1e-3, cv=720.19471, prandtl=0.71, bcs_v=(), bcs_t=()) -> Config:
+                t=0.0, diffusion=False, mu=1e-3, cv=720.19471, prandtl=0.71, bcs_v=(), bcs_t=(),
+                sutherland=False, mu0=1.716e-5, T0=273.15) -> Config:
     """bcs: sequence of (bc_type, (rho,u,v,w,p)) in BC index order 1..n_bc;
     bcs_v / bcs_t: ("neumann"|"dirichlet", value) per BC index for the
-    viscous terms (velocity (3,) / temperature), default neumann."""
+    viscous terms (velocity (3,) / temperature), default neumann;
+    sutherland: per-cell viscosity mu0 (T/T0)^1.5 (T0 + 110.4) / (T + 110.4)."""
     c = Config()
     c.scheme_id = SCHEMES[scheme]
     c.bc_style = bc_style
@@ -93,6 +98,9 @@ def make_config(scheme="multi_point", bc_style=2, boundary_2d=True, exclude_boun
     c.mu_p = mu
     c.Cv_p = cv
     c.prandtl = prandtl
+    c.use_sutherland = int(sutherland)
+    c.mu0 = mu0
+    c.T0 = T0
     for i in range(1, MNBC + 1):
         c.bc_V_id[i] = 1
         c.bc_T_id[i] = 1
@@ -151,6 +159,8 @@ def lib():
                                          C.c_int, ip, ip, ip, dp]
         L.om_bcsr_pcgm.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, dp]
         L.om_add_diffusion.argtypes = [P, cp, dp, dp, dp]
+        L.om_compute_mu.restype = C.c_double
+        L.om_compute_mu.argtypes = [C.c_int, cp, dp, dp]
         L.om_test_inv.argtypes = [C.c_int, dp, dp]
         L.om_inverse_3_by_3.argtypes = [dp, dp]
         L.om_potential_flow.argtypes = [C.c_int, dp, dp]
@@ -446,3 +456,11 @@ def inv(A: np.ndarray) -> np.ndarray:
     out = np.zeros(n * n)
     lib().om_test_inv(n, np.ascontiguousarray(A.T).ravel(), out)
     return out.reshape(n, n).T
+
+
+def compute_mu(cfg: Config, sol: np.ndarray):
+    """compute_mu restatement: (mu of every cell, mu0corr) for sol (C, 5)."""
+    sol = np.ascontiguousarray(sol, np.float64).reshape(-1, 5)
+    mu = np.zeros(sol.shape[0])
+    mn = lib().om_compute_mu(sol.shape[0], C.byref(cfg), sol.ravel(), mu)
+    return mu, mn

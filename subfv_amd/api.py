@@ -238,6 +238,11 @@ class Solver:
         self._check(_lib.lib().subfv_set_diffusion(self.h, int(activate), mu, cv, prandtl, n, _lib.iptr(vid),
                                                    _lib.dptr(vv), _lib.iptr(tid), _lib.dptr(tv)))
 
+    def set_sutherland(self, activate: bool = True, mu0: float = 1.716e-5, T0: float = 273.15):
+        """use_sutherland of compute_mu: per-cell viscosity from the cell
+        temperature (ns_vectorial_diffusion_module.F90:322-359)."""
+        self._check(_lib.lib().subfv_set_sutherland(self.h, int(activate), mu0, T0))
+
     def compute_rhs_ns(self):
         self._check(_lib.lib().subfv_compute_rhs_ns(self.h))
         return self.get("rhs"), self.get("sum_lambda")

@@ -81,11 +81,13 @@ struct subfv_ctx {
   double *rhs = nullptr, *rhs2 = nullptr, *sl = nullptr, *sl2 = nullptr;
   double *NG = nullptr, *VMM = nullptr, *G = nullptr, *vp = nullptr, *diag = nullptr;
   double *blk = nullptr;
-  // viscous terms (constant viscosity): per-vertex precomputed blocks
+  // viscous terms: per-vertex precomputed blocks (once per mesh with a
+  // constant viscosity, every residual with Sutherland's law)
   bool diff_on = false, diff_ready = false;
   DiffParams dparams{};
   long *doff = nullptr;
   double *pre = nullptr, *PD = nullptr, *rdiff = nullptr;
+  double *mu_cell = nullptr, *mu_blk = nullptr;  // Sutherland: [C] viscosities; block minima + mu0corr
   long pre_n = 0;  // doubles in the packed viscous blocks
   int *derr = nullptr;  // [2]: error bits of the path, failed check_mesh cells
   StepState *st = nullptr;
@@ -184,6 +186,7 @@ void free_mesh(subfv_ctx *c) {
   c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = nullptr;
   c->doff = nullptr;
   c->pre = c->PD = c->rdiff = nullptr;
+  c->mu_cell = c->mu_blk = nullptr;
   c->diff_ready = false;
 }
 
@@ -584,6 +587,10 @@ int ensure_diffusion(subfv_ctx *c) {
   c->pre_n = off[h.nv];
   if ((r = dalloc(c, &c->PD, (size_t)kDiffSlot * h.nse))) return r;
   if ((r = dalloc(c, &c->rdiff, 4ull * h.ne))) return r;
+  if (c->dparams.suth) {
+    if ((r = dalloc(c, &c->mu_cell, (size_t)h.ne))) return r;
+    if ((r = dalloc(c, &c->mu_blk, (size_t)nblk(h.ne, kCellThreads) + 1))) return r;
+  }
   if (!c->diag && (r = dalloc(c, &c->diag, 25ull * h.ne))) return r;
   HIPCHK(c, hipMemsetAsync(c->diag, 0, sizeof(double) * 25 * h.ne, c->stream));
   HIPCHK(c, hipMemsetAsync(c->rdiff, 0, sizeof(double) * 4 * h.ne, c->stream));
@@ -596,10 +603,33 @@ int ensure_diffusion(subfv_ctx *c) {
   a.PD = c->PD;
   a.rdiff = c->rdiff;
   a.diag = c->diag;
-  hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)h.nv), dim3(64), 0, c->stream, a);
+  // with Sutherland's law the blocks depend on the state: formed per residual
+  if (!c->dparams.suth) hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)h.nv), dim3(64), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   c->diff_ready = true;
   return 0;
+}
+
+// compute_mu with use_sutherland (ns_vectorial_diffusion_module.F90:322-359):
+// mu_sutherland of every cell, C0 = 110.4 (ns_global_data_module.F90:20), and
+// the block minima of mu0corr
+__global__ __launch_bounds__(kCellThreads) void k_compute_mu(long C, const double *U, double cv, double mu0,
+                                                             double T0, double *mu, double *blk) {
+  const long c = (long)blockIdx.x * kCellThreads + threadIdx.x;
+  double v = 1e12;
+  if (c < C) {
+    double u[5];
+    for (int k = 0; k < 5; k++) u[k] = U[5 * c + k];
+    const double T = temp_u(u, cv);
+    v = mu0 * pow(T / T0, 1.5) * (T0 + 110.4) / (T + 110.4);
+    mu[c] = v;
+  }
+  block_min_store(v, blk);
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_mu_min(const double *blk, int nb, double *out) {
+  const double v = reduce_min_one_block(blk, nb);
+  if (threadIdx.x == 0) *out = v;
 }
 
 // rdiff and diag of the current state (build_momentum_diff_mat_into_global_mat_diag
@@ -617,6 +647,17 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
   a.rdiff = c->rdiff;
   a.diag = c->diag;
   Prof pf(c, "diffusion");
+  if (c->dparams.suth) {
+    Prof ps(c, "diff_setup");
+    const unsigned nb = nblk(c->dm.C, kCellThreads);
+    hipLaunchKernelGGL(k_compute_mu, dim3(nb), dim3(kCellThreads), 0, c->stream, (long)c->dm.C, U, c->dparams.cv,
+                       c->dparams.suth_mu0, c->dparams.suth_T0, c->mu_cell, c->mu_blk);
+    hipLaunchKernelGGL(k_mu_min, dim3(1), dim3(kCellThreads), 0, c->stream, (const double *)c->mu_blk, (int)nb,
+                       c->mu_blk + nb);
+    a.mu_cell = c->mu_cell;
+    a.mu_min = c->mu_blk + nb;
+    hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)c->dm.V), dim3(64), 0, c->stream, a);
+  }
   a.vblocks = (int)nblk(c->dm.V, kDiffWaves);
   a.vper_xcd = (a.vblocks + 7) / 8;
   a.cblocks = (int)nblk(c->dm.C, 256);
@@ -1312,6 +1353,10 @@ int subfv_set_diffusion(subfv_ctx *c, int32_t activate, double mu_p, double cv_p
   p.mu0 = mu_p;  // compute_mu without Sutherland: mu0corr = mu_p
   p.cv = cv_p;
   p.kap = mu_p * cv_p * kGamma / prandtl;  // kappa (ns_diffusion_module.F90:186-194)
+  p.pr = prandtl;
+  p.suth = c->dparams.suth;  // set by subfv_set_sutherland
+  p.suth_mu0 = c->dparams.suth_mu0;
+  p.suth_T0 = c->dparams.suth_T0;
   for (int i = 0; i < 11; i++) {
     p.bc_V_id[i] = 1;
     p.bc_T_id[i] = 1;
@@ -1326,6 +1371,20 @@ int subfv_set_diffusion(subfv_ctx *c, int32_t activate, double mu_p, double cv_p
   c->dparams = p;
   c->diff_on = activate != 0;
   if (changed) c->diff_ready = false;
+  return 0;
+}
+
+int subfv_set_sutherland(subfv_ctx *c, int32_t activate, double mu0, double T0) {
+  if (!c) return SUBFV_E_INVALID;
+  if (activate && !(mu0 > 0.0 && T0 > 0.0)) return fail(c, SUBFV_E_INVALID, "Sutherland's law needs mu0 and T0 > 0");
+  DiffParams p = c->dparams;
+  p.suth = activate != 0;
+  p.suth_mu0 = activate ? mu0 : 0.0;
+  p.suth_T0 = activate ? T0 : 0.0;
+  if (std::memcmp(&p, &c->dparams, sizeof(p)) != 0) {
+    c->dparams = p;
+    c->diff_ready = false;
+  }
   return 0;
 }
 

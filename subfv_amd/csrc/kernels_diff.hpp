@@ -5,7 +5,10 @@
 //
 // With mu constant in space and time (use_sutherland = .false.) the nodal
 // matrices N, S, S~^T and B depend on the mesh only, so the setup kernel
-// forms, once per mesh, every product the residual needs (one wavefront per
+// forms, once per mesh, every product the residual needs.  With Sutherland's
+// law (compute_mu, ns_vectorial_diffusion_module.F90:322-359) mu varies per
+// cell and step: the same setup kernel then runs before every residual, on
+// the cell viscosities and their minimum mu0corr (one wavefront per
 // vertex, matrices in LDS, the inverse in the order of LU + unblocked
 // dgetri, products summed over k in increasing order):
 //   momentum: SNS = (S~^T N^-1) S, SB = S~^T (N^-1 B), the sub-element sums of
@@ -30,6 +33,9 @@ constexpr int kDiffSlot = 18;               // partials per slot
 
 struct DiffParams {
   double mu, mu0, kap, cv;
+  double pr;                  // Prandtl number
+  double suth_mu0, suth_T0;   // Sutherland reference viscosity and temperature
+  int suth, pad_;             // use_sutherland: mu per cell from mu_cell, mu0corr from mu_min
   int bc_V_id[11], bc_T_id[11];  // 1 neumann, 2 dirichlet, by BC index (1-based)
   double bc_val_V[11][3], bc_val_T[11];
 };
@@ -59,6 +65,8 @@ struct DiffArgs {
   double *PD;         // [SE][18] node-sorted slot partials
   double *rdiff;      // [C][4]
   double *diag;       // [C][25]
+  const double *mu_cell;  // [C] viscosity of each cell (Sutherland), else nullptr
+  const double *mu_min;   // mu0corr, their minimum (Sutherland)
   int vper_xcd, vblocks;  // XCD slices of the vertex blocks (apply)
   int cper_xcd, cblocks;  // and of the cell blocks (gather)
 };
@@ -241,7 +249,8 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
           const int po = (int)(sp - sf0);
           double np[3];
           sf_normal(m, sp, m.sf_le[sp] == cell ? 1.0 : -1.0, np);
-          const double blc = P.kap * al * m.sfn[3 * m.SF + sp] * (1.0 / vol) * (np[0] * nl[0] + np[1] * nl[1] + np[2] * nl[2]);
+          const double kap = a.mu_cell ? a.mu_cell[cell] * P.cv * kGamma / P.pr : P.kap;  // kappa (:186-194)
+          const double blc = kap * al * m.sfn[3 * m.SF + sp] * (1.0 / vol) * (np[0] * nl[0] + np[1] * nl[1] + np[2] * nl[2]);
           L.N[l + nf * po] = L.N[l + nf * po] + blc;
           L.S[l + nf * so] = L.S[l + nf * so] + blc;
           L.T[so + ns * l] = L.T[so + ns * l] + blc;
@@ -274,7 +283,7 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   wsync_d();
   if (lane < 9) {
     const int i = lane % 3, j = lane / 3;
-    const double mu = P.mu, mu0 = P.mu0;
+    const double mu0 = a.mu_cell ? *a.mu_min : P.mu0;
     for (int l = 0; l < nf; l++) {
       const long s = sf0 + l;
       const int re = m.sf_re[s];
@@ -298,6 +307,7 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
         sf_normal(m, s, side == 0 ? 1.0 : -1.0, nl);
         const long se = m.v_seid[se0 + so];
         const int cell = m.se_cell[se0 + so];
+        const double mu = a.mu_cell ? a.mu_cell[cell] : P.mu;  // mu_arr of the side's cell
         const double vol = m.se_vol[se];
         for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
           const long sp = m.se_sf[pp];
@@ -520,7 +530,8 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
         for (int i = 0; i < 3; i++)
           Spc[i + 3 * j] = Spc[i + 3 * j] + ar * (vm[i] * nl[j] + vm[j] * nl[i] - (2.0 / 3.0) * dv * (i == j ? 1.0 : 0.0));
     }
-    for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * a.p.mu / L.vol[l];
+    const double mu = a.mu_cell ? a.mu_cell[cell] : a.p.mu;
+    for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * mu / L.vol[l];
     double work = 0.0;
     for (int e = 0; e < cnt; e++) {
       const int lo = L.ssf[l][1 + e];

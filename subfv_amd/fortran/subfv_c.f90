@@ -165,6 +165,12 @@ module subfv_c
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function subfv_set_sutherland(ctx, activate, mu0, t0) bind(C, name="subfv_set_sutherland")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: activate
+      real(c_double), value :: mu0, t0
+    end function
     integer(c_int) function subfv_set_diffusion(ctx, activate, mu_p, cv_p, prandtl, n_bc, bc_v_id, bc_val_v, &
         bc_t_id, bc_val_t) bind(C, name="subfv_set_diffusion")
       import :: c_ptr, c_int, c_int32_t, c_double

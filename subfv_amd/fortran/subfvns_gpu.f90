@@ -142,7 +142,7 @@ program subfvns_gpu
       bc_v_id(i) = merge(2, 1, trim(adjustl(bc_type_V(i))) == "dirichlet")
       bc_t_id(i) = merge(2, 1, trim(adjustl(bc_type_T(i))) == "dirichlet")
     end do
-    if (use_sutherland) error stop "use_sutherland is not supported on the device path"
+    call check(ctx, subfv_set_sutherland(ctx, merge(1, 0, use_sutherland), mu0, T0))  ! compute_mu
     call check(ctx, subfv_set_diffusion(ctx, 1, mu_p, Cv_p, 0.71_DP, int(n_bc, c_int32_t), bc_v_id, bc_val_V, &
       bc_t_id, bc_val_T))
   end if

@@ -104,3 +104,104 @@ def test_vertices_beyond_the_limit_are_rejected(solver):
     with pytest.raises(SubfvError):
         solver.compute_rhs_ns()
     solver.set_diffusion(False)
+
+
+def warm_state(xc, seed=3):
+    """air-like temperatures (250-350 K) so Sutherland's mu varies by ~30 %"""
+    rng = np.random.default_rng(seed)
+    x, y = xc[:, 0], xc[:, 1]
+    rho = 1.0 + 0.2 * np.sin(3 * x + 1) * np.cos(2 * y) + 0.02 * rng.random(x.size)
+    T = 300.0 + 50.0 * np.sin(2 * np.pi * x) * np.cos(np.pi * y) + rng.random(x.size)
+    w = np.stack([rho, 30 * np.sin(2 * y + 0.5), -20 * np.cos(3 * x), 5 * np.sin(4 * xc[:, 2] + x),
+                  rho * 0.4 * 720.19471 * T], 1)
+    return mg.primit_to_conserv(w)
+
+
+@pytest.mark.parametrize("second", [False, True])
+@pytest.mark.parametrize("name", list(CASES))
+def test_sutherland_rhs_and_diag_match_oracle(solver, name, second):
+    """compute_mu with use_sutherland: mu per cell, mu0corr their minimum;
+    T^1.5 goes through pow on both sides, so the per-cell viscosities may
+    differ in the last bit: the same 1e-11 bound as the constant case."""
+    print(f"[gpu] sutherland rhs {name} second={second}", flush=True)
+    mk, cfgkw, dkw = CASES[name]
+    mesh = mk()
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    sol = warm_state(om.elem_coord)
+    solver.set_config(make_config("multi_point", second_order=second, method=4, **cfgkw))
+    solver.set_diffusion(True, mu=1.0, **dkw)  # mu_p unused with Sutherland
+    solver.set_sutherland(True)
+    try:
+        solver.put_sol(sol)
+        solver.compute_rhs_ns()
+        rhs = solver.get("rhs")
+        diag = solver.get("diag")
+    finally:
+        solver.set_sutherland(False)
+        solver.set_diffusion(False)
+    ocfg = O.make_config("multi_point", second_order=second, method=4, diffusion=True, mu=1.0, sutherland=True,
+                         **cfgkw, **dkw)
+    mu, mn = O.compute_mu(ocfg, sol)
+    assert mu.max() > 1.2 * mn
+    r0, sl = om.compute_euler_flux(ocfg, sol)
+    if second:
+        g = om.compute_cell_grad(sol, 4)[0]
+        r2, _ = om.compute_euler_flux(ocfg, sol, grad=g, second_order=True)
+        r0 = om.flux_limiting(sol, r0, r2, sl, ocfg.cfl)
+    rd, dd = om.add_diffusion(ocfg, sol)
+    ref_rhs = r0 + rd
+    scale_r = np.abs(rd).max(axis=0) + 1e-300
+    assert np.all(np.abs(rhs - ref_rhs) <= 1e-11 * scale_r[None, :] + 1e-14 * np.abs(ref_rhs)), \
+        np.abs(rhs - ref_rhs).max(axis=0) / scale_r
+    assert np.abs(diag - dd).max() <= 1e-11 * np.abs(dd).max()
+
+
+def test_sutherland_time_loop_matches_oracle(solver):
+    print("[gpu] sutherland run", flush=True)
+    mesh = mg.tri_mesh(8)
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    sol = warm_state(om.elem_coord, 4)
+    solver.set_config(make_config("multi_point", second_order=True, method=4, cfl=0.3))
+    solver.set_diffusion(True, mu=1.0)
+    solver.set_sutherland(True, mu0=1.716e-5, T0=273.15)
+    solver.set_time(0.0)
+    try:
+        solver.put_sol(sol)
+        solver.run(10, 0)
+        got = solver.get_sol()
+    finally:
+        solver.set_sutherland(False)
+        solver.set_diffusion(False)
+    st, ref, info = om.run(O.make_config("multi_point", second_order=True, method=4, cfl=0.3, diffusion=True,
+                                         mu=1.0, sutherland=True), sol, 10, 0)
+    assert st == 0
+    assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-11
+    # the viscous terms moved the state: the same run without them differs
+    st, inv, _ = om.run(O.make_config("multi_point", second_order=True, method=4, cfl=0.3), sol, 10, 0)
+    assert np.abs(got - inv).max() / np.abs(ref).max() > 1e-9
+
+
+def test_sutherland_off_restores_constant_viscosity(solver):
+    """the blocks formed per residual under Sutherland are replaced by the
+    constant-viscosity ones when it is switched off: same bits as before"""
+    print("[gpu] sutherland toggle", flush=True)
+    mesh = mg.quad_mesh(9)
+    solver.build_mesh(mesh)
+    sol = warm_state(solver.elem_coord, 6)
+    solver.set_config(make_config("multi_point"))
+    solver.set_diffusion(True, mu=2e-5)
+    solver.put_sol(sol)
+    solver.compute_rhs_ns()
+    a = solver.get("rhs"), solver.get("diag")
+    solver.set_sutherland(True)
+    solver.compute_rhs_ns()
+    b = solver.get("rhs"), solver.get("diag")
+    solver.set_sutherland(False)
+    solver.set_diffusion(True, mu=2e-5)
+    solver.compute_rhs_ns()
+    c = solver.get("rhs"), solver.get("diag")
+    solver.set_diffusion(False)
+    assert not np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])

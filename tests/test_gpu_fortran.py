@@ -169,3 +169,32 @@ def test_fortran_driver_periodic_vortices(tmp_path, init):
     # against the state's largest magnitude (the z momentum is round-off)
     scale = np.abs(ref).max()
     assert np.abs(sol - ref).max() <= 1e-11 * scale, np.abs(sol - ref).max(axis=0) / scale
+
+
+def test_fortran_driver_sutherland_viscous(tmp_path):
+    """activate_diffusion with use_sutherland through the namelist: a
+    Riemann problem between walls at air-like temperatures (347 K and
+    278 K), second order; the viscous terms agree with the oracle to 1e-11
+    of the state (the device sums each vertex's terms at once)."""
+    print("[gpu] fortran driver sutherland", flush=True)
+    mesh = mg.quad_mesh(12)
+    mg.write_msh(mesh, str(tmp_path / "mesh.msh"))
+    text = NAMELIST.format(scheme="multi_point", second=".true.", iters=9, periodic=".false.", x1drp=0.5)
+    text = text.replace("sol_w_1drp_l = 1.0, 0.0, 0.0, 0.0, 1.0", "sol_w_1drp_l = 1.0, 0.0, 0.0, 0.0, 1.0e5")
+    text = text.replace("sol_w_1drp_r = 0.125, 0.0, 0.0, 0.0, 0.1", "sol_w_1drp_r = 0.5, 0.0, 0.0, 0.0, 4.0e4")
+    text = text.replace("/\n", "activate_diffusion = .true.\nuse_sutherland = .true.\nmu0 = 1.716e-5\nT0 = 273.15\n/\n")
+    (tmp_path / "input_data.f").write_text(text)
+    out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    raw = np.fromfile(tmp_path / "sol_final.bin", dtype=np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    sol = raw[4:].view(np.float64).reshape(n, 5)
+    om = O.OracleMesh(mesh)
+    init = mg.primit_to_conserv(np.where((om.elem_coord[:, 0] < 0.5)[:, None],
+                                         np.array([1.0, 0, 0, 0, 1e5]), np.array([0.5, 0, 0, 0, 4e4])))
+    kw = dict(bc_style=2, boundary_2d=True, second_order=True, method=4, cfl=0.4, tmax=1e30)
+    st, ref, _ = om.run(O.make_config("multi_point", diffusion=True, sutherland=True, **kw), init, 10, 0)
+    assert st == 0
+    assert np.abs(sol - ref).max() <= 1e-11 * np.abs(ref).max(), np.abs(sol - ref).max() / np.abs(ref).max()
+    st, inv, _ = om.run(O.make_config("multi_point", **kw), init, 10, 0)
+    assert np.abs(sol - inv).max() > 1e3 * np.abs(sol - ref).max()
