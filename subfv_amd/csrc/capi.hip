@@ -88,6 +88,7 @@ struct subfv_ctx {
   long *doff = nullptr;
   double *pre = nullptr, *PD = nullptr, *rdiff = nullptr;
   double *mu_cell = nullptr, *mu_blk = nullptr;  // Sutherland: [C] viscosities; block minima + mu0corr
+  int diff_fm = 0, diff_em = 0;  // setup LDS extents: 3 * max sub-faces, 3 * max sub-elements of a vertex
   long pre_n = 0;  // doubles in the packed viscous blocks
   int *derr = nullptr;  // [2]: error bits of the path, failed check_mesh cells
   StepState *st = nullptr;
@@ -568,13 +569,21 @@ int ensure_diffusion(subfv_ctx *c) {
   const subfv::HostMesh &h = *c->hmp;
   if (h.periodic) return fail(c, SUBFV_E_INVALID, "viscous terms are not available on periodic meshes");
   std::vector<long> off(h.nv + 1, 0);
+  int mf = 1, me = 1;
   for (int v = 0; v < h.nv; v++) {
     const int nf = h.sf_off[v + 1] - h.sf_off[v], ns = h.v_seoff[v + 1] - h.v_seoff[v];
     if (nf > kDiffMaxSub || ns > kDiffMaxSub)
       return fail(c, SUBFV_E_INVALID, "viscous terms: vertex " + std::to_string(v + 1) + " has more than " +
                                           std::to_string(kDiffMaxSub) + " sub-faces or sub-elements");
     off[v + 1] = off[v] + diff_block_size(nf, ns);
+    mf = std::max(mf, nf);
+    me = std::max(me, ns);
   }
+  c->diff_fm = 3 * mf;
+  c->diff_em = 3 * me;
+  const size_t lds = diff_setup_lds_bytes(c->diff_fm, c->diff_em);
+  if (lds > (64u << 10))
+    HIPCHK(c, hipFuncSetAttribute((const void *)k_diff_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   size_t free_b = 0, total_b = 0;
   HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
   const size_t need = sizeof(double) * ((size_t)off[h.nv] + (size_t)kDiffSlot * h.nse + 29ull * h.ne);
@@ -604,7 +613,9 @@ int ensure_diffusion(subfv_ctx *c) {
   a.rdiff = c->rdiff;
   a.diag = c->diag;
   // with Sutherland's law the blocks depend on the state: formed per residual
-  if (!c->dparams.suth) hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)h.nv), dim3(64), 0, c->stream, a);
+  a.set_fm = c->diff_fm;
+  a.set_em = c->diff_em;
+  if (!c->dparams.suth) hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)h.nv), dim3(64), lds, c->stream, a);
   HIPCHK(c, hipGetLastError());
   c->diff_ready = true;
   return 0;
@@ -656,7 +667,10 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
                        c->mu_blk + nb);
     a.mu_cell = c->mu_cell;
     a.mu_min = c->mu_blk + nb;
-    hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)c->dm.V), dim3(64), 0, c->stream, a);
+    a.set_fm = c->diff_fm;
+    a.set_em = c->diff_em;
+    hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)c->dm.V), dim3(64), diff_setup_lds_bytes(a.set_fm, a.set_em),
+                       c->stream, a);
   }
   a.vblocks = (int)nblk(c->dm.V, kDiffWaves);
   a.vper_xcd = (a.vblocks + 7) / 8;

@@ -46,14 +46,25 @@ SFV_DEV __host__ long diff_block_size(int nf, int ns) {
   return E * E + E + 9L * ns + F * E + F + (long)ns * ns + 2L * ns;
 }
 
+// The setup's four matrices live in dynamic LDS sized for the mesh's largest
+// vertex (fm = 3 * max sub-faces, em = 3 * max sub-elements): N fm x fm, then
+// S (F x E), S~^T (E x F) and the products W (E x F), fm * em each, all stored
+// with the vertex's own leading dimensions.  Prism meshes need 26 KB instead
+// of 74 KB for the 16-sub-face limit, so several vertices share a CU.
+SFV_DEV __host__ inline size_t diff_setup_lds_bytes(int fm, int em) {
+  return sizeof(double) * ((size_t)fm * fm + 3ull * fm * em);
+}
+
 struct DiffSetupLds {
-  double N[kDiffMaxF * kDiffMaxF];
-  double S[kDiffMaxF * kDiffMaxF];
-  double T[kDiffMaxF * kDiffMaxF];  // S~^T
-  double W[kDiffMaxF * kDiffMaxF];  // products
   double B[kDiffMaxF], x[kDiffMaxF], work[kDiffMaxF];
   int piv[kDiffMaxF];
   int lse[kDiffMaxSub], rse[kDiffMaxSub];  // local slot of each sub-face's left/right cell (-1 none)
+  // the vertex's mesh data, staged once so the assembly loops read LDS only
+  double g[kDiffMaxSub][4];                // unit normal and area of each sub-face
+  int le[kDiffMaxSub], re[kDiffMaxSub];    // its left and right neighbour codes
+  double sevol[kDiffMaxSub], semu[kDiffMaxSub];  // volume and cell viscosity of each sub-element
+  int secell[kDiffMaxSub];
+  int ssf[kDiffMaxSub][kSlotSrc + 1];      // local sub-faces of each sub-element, count first
 };
 
 struct DiffArgs {
@@ -66,6 +77,7 @@ struct DiffArgs {
   double *rdiff;      // [C][4]
   double *diag;       // [C][25]
   const double *mu_cell;  // [C] viscosity of each cell (Sutherland), else nullptr
+  int set_fm, set_em;     // setup: largest 3 * sub-faces and 3 * sub-elements of a vertex
   const double *mu_min;   // mu0corr, their minimum (Sutherland)
   int vper_xcd, vblocks;  // XCD slices of the vertex blocks (apply)
   int cper_xcd, cblocks;  // and of the cell blocks (gather)
@@ -118,21 +130,25 @@ SFV_DEV void wave_inverse(double *A, int n, int *piv, double *work, int lane) {
     }
     wsync_d();
   }
-  // dtrti2 (upper, non-unit)
+  // dtrti2 (upper, non-unit).  Its dtrmv reads x(jj) = A(jj, j) at step jj
+  // before any later step changes it (step jj' only updates rows above jj'),
+  // so every row of the column is one independent sum in dtrmv's order:
+  // x(i) A(i, i) (when x(i) != 0), then + x(jj) A(i, jj) for jj = i+1..j-1
+  // with x(jj) != 0; one lane per row, one synchronisation per column.
   for (int j = 0; j < n; j++) {
+    for (int i = lane; i < j; i += 64) work[i] = AA(i, j);
     if (lane == 0) AA(j, j) = 1.0 / AA(j, j);
     wsync_d();
     const double ajj = -AA(j, j);
-    for (int jj = 0; jj < j; jj++) {
-      const double temp = AA(jj, j);
-      if (temp != 0.0) {
-        for (int i = lane; i < jj; i += 64) AA(i, j) = AA(i, j) + temp * AA(i, jj);
-        wsync_d();
-        if (lane == 0) AA(jj, j) = AA(jj, j) * AA(jj, jj);
+    for (int i = lane; i < j; i += 64) {
+      const double xi = work[i];
+      double acc = xi != 0.0 ? xi * AA(i, i) : xi;
+      for (int jj = i + 1; jj < j; jj++) {
+        const double t = work[jj];
+        if (t != 0.0) acc = acc + t * AA(i, jj);
       }
-      wsync_d();
+      AA(i, j) = ajj * acc;
     }
-    for (int i = lane; i < j; i += 64) AA(i, j) = ajj * AA(i, j);
     wsync_d();
   }
   // inv(A) L = inv(U), columns right to left
@@ -196,6 +212,9 @@ SFV_DEV double blc_mu_ij(double s, double mu, double mu0, const double *nl, cons
 // One 64-thread workgroup per vertex.
 __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   __shared__ DiffSetupLds L;
+  extern __shared__ double dyn_lds[];
+  const int fm = a.set_fm, em = a.set_em;
+  double *const Nm = dyn_lds, *const Sm = Nm + fm * fm, *const Tm = Sm + fm * em, *const Wm = Tm + fm * em;
   const DevMesh &m = a.m;
   const DiffParams &P = a.p;
   const long v = blockIdx.x;
@@ -205,89 +224,103 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   const int se0 = m.se_off[v], ns = m.se_off[v + 1] - se0;
   const int F = 3 * nf, E = 3 * ns;
   double *out = a.pre + a.doff[v];
-  // local slots of the left / right cells of each sub-face
   if (lane < nf) {
     const long s = sf0 + lane;
-    const int le = m.sf_le[s], re = m.sf_re[s];
+    for (int d = 0; d < 4; d++) L.g[lane][d] = m.sfn[d * m.SF + s];
+    L.le[lane] = m.sf_le[s];
+    L.re[lane] = m.sf_re[s];
+  } else if (lane >= 32 && lane - 32 < ns) {
+    const int q = lane - 32;
+    const long se = m.v_seid[se0 + q];
+    const int cell = m.se_cell[se0 + q];
+    L.secell[q] = cell;
+    L.sevol[q] = m.se_vol[se];
+    L.semu[q] = a.mu_cell ? a.mu_cell[cell] : P.mu;
+    const int pb = m.se_sfoff[se], pe = m.se_sfoff[se + 1];
+    L.ssf[q][0] = pe - pb;  // at most kSlotSrc (checked when the groups are built)
+    for (int pp = pb; pp < pe; pp++) L.ssf[q][1 + pp - pb] = (int)(m.se_sf[pp] - sf0);
+  }
+  wsync_d();
+  // local slots of the left / right cells of each sub-face
+  if (lane < nf) {
+    const int le = L.le[lane], re = L.re[lane];
     int ls = -1, rs = -1;
     for (int q = 0; q < ns; q++) {
-      const int c = m.se_cell[se0 + q];
+      const int c = L.secell[q];
       if (c == le) ls = q;
       if (re >= 0 && c == re) rs = q;
     }
     L.lse[lane] = ls;
     L.rse[lane] = rs;
   }
-  for (int q = lane; q < kDiffMaxF * kDiffMaxF; q += 64) L.N[q] = L.S[q] = L.T[q] = 0.0;
+  for (int q = lane; q < fm * fm; q += 64) Nm[q] = 0.0;
+  for (int q = lane; q < fm * em; q += 64) Sm[q] = Tm[q] = 0.0;
   for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
   wsync_d();
 
   // ---------------- heat (scalar entries, one lane, reference loop order)
   if (lane == 0) {
     for (int l = 0; l < nf; l++) {
-      const long s = sf0 + l;
-      const double al = m.sfn[3 * m.SF + s];
+      const double al = L.g[l][3];
       for (int side = 0; side < 2; side++) {
         const int so = side == 0 ? L.lse[l] : L.rse[l];
         if (side == 1 && so < 0) {
-          const int re = m.sf_re[s];
+          const int re = L.re[l];
           if (re <= -2 && P.bc_T_id[-re - 1] == 2) {
             L.B[l] = al * P.bc_val_T[-re - 1];
-            for (int c = 0; c < ns; c++) L.S[l + nf * c] = 0.0;
-            for (int c = 0; c < nf; c++) L.N[l + nf * c] = 0.0;
-            L.N[l + nf * l] = al;
+            for (int c = 0; c < ns; c++) Sm[l + nf * c] = 0.0;
+            for (int c = 0; c < nf; c++) Nm[l + nf * c] = 0.0;
+            Nm[l + nf * l] = al;
           }
           continue;
         }
-        double nl[3];
-        sf_normal(m, s, side == 0 ? 1.0 : -1.0, nl);
-        const long se = m.v_seid[se0 + so];
-        const int cell = m.se_cell[se0 + so];
-        const double vol = m.se_vol[se];
-        for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
-          const long sp = m.se_sf[pp];
-          const int po = (int)(sp - sf0);
-          double np[3];
-          sf_normal(m, sp, m.sf_le[sp] == cell ? 1.0 : -1.0, np);
-          const double kap = a.mu_cell ? a.mu_cell[cell] * P.cv * kGamma / P.pr : P.kap;  // kappa (:186-194)
-          const double blc = kap * al * m.sfn[3 * m.SF + sp] * (1.0 / vol) * (np[0] * nl[0] + np[1] * nl[1] + np[2] * nl[2]);
-          L.N[l + nf * po] = L.N[l + nf * po] + blc;
-          L.S[l + nf * so] = L.S[l + nf * so] + blc;
-          L.T[so + ns * l] = L.T[so + ns * l] + blc;
+        const double sgl = side == 0 ? 1.0 : -1.0;
+        const double nl[3] = {sgl * L.g[l][0], sgl * L.g[l][1], sgl * L.g[l][2]};
+        const int cell = L.secell[so];
+        const double vol = L.sevol[so];
+        const double kap = a.mu_cell ? L.semu[so] * P.cv * kGamma / P.pr : P.kap;  // kappa (:186-194)
+        for (int e = 0; e < L.ssf[so][0]; e++) {
+          const int po = L.ssf[so][1 + e];
+          const double sgp = L.le[po] == cell ? 1.0 : -1.0;
+          const double np[3] = {sgp * L.g[po][0], sgp * L.g[po][1], sgp * L.g[po][2]};
+          const double blc = kap * al * L.g[po][3] * (1.0 / vol) * (np[0] * nl[0] + np[1] * nl[1] + np[2] * nl[2]);
+          Nm[l + nf * po] = Nm[l + nf * po] + blc;
+          Sm[l + nf * so] = Sm[l + nf * so] + blc;
+          Tm[so + ns * l] = Tm[so + ns * l] + blc;
         }
       }
     }
   }
   wsync_d();
-  wave_inverse(L.N, nf, L.piv, L.work, lane);
+  wave_inverse(Nm, nf, L.piv, L.work, lane);
   // SNS_h = (T N^-1) S ; SB_h = T (N^-1 B) ; b1 = row sums of T
   double *o_snsh = out + (long)E * E + E + 9L * ns + (long)F * E + F;
   double *o_sbh = o_snsh + (long)ns * ns;
   double *o_b1 = o_sbh + ns;
-  wave_gemm(ns, nf, nf, L.T, L.N, L.W, lane);
+  wave_gemm(ns, nf, nf, Tm, Nm, Wm, lane);
   wsync_d();
-  wave_gemm(ns, ns, nf, L.W, L.S, o_snsh, lane);
-  wave_gemv(nf, nf, L.N, L.B, L.x, lane);
+  wave_gemm(ns, ns, nf, Wm, Sm, o_snsh, lane);
+  wave_gemv(nf, nf, Nm, L.B, L.x, lane);
   wsync_d();
-  wave_gemv(ns, nf, L.T, L.x, o_sbh, lane);
+  wave_gemv(ns, nf, Tm, L.x, o_sbh, lane);
   for (int l = lane; l < ns; l += 64) {
     double acc = 0.0;
-    for (int c = 0; c < nf; c++) acc = acc + L.T[l + ns * c];
+    for (int c = 0; c < nf; c++) acc = acc + Tm[l + ns * c];
     o_b1[l] = acc;
   }
   wsync_d();
 
   // ---------------- momentum (3x3 blocks, lane (i, j) < 9 owns entry (i, j))
-  for (int q = lane; q < kDiffMaxF * kDiffMaxF; q += 64) L.N[q] = L.S[q] = L.T[q] = 0.0;
+  for (int q = lane; q < fm * fm; q += 64) Nm[q] = 0.0;
+  for (int q = lane; q < fm * em; q += 64) Sm[q] = Tm[q] = 0.0;
   for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
   wsync_d();
   if (lane < 9) {
     const int i = lane % 3, j = lane / 3;
     const double mu0 = a.mu_cell ? *a.mu_min : P.mu0;
     for (int l = 0; l < nf; l++) {
-      const long s = sf0 + l;
-      const int re = m.sf_re[s];
-      const double al = m.sfn[3 * m.SF + s];
+      const int re = L.re[l];
+      const double al = L.g[l][3];
       bool neumann, dirichlet;
       if (re >= 0) {
         neumann = dirichlet = false;
@@ -303,32 +336,31 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
       for (int side = 0; side < nsides; side++) {
         const int so = side == 0 ? L.lse[l] : L.rse[l];
         const int se0l = 3 * so;
-        double nl[3];
-        sf_normal(m, s, side == 0 ? 1.0 : -1.0, nl);
-        const long se = m.v_seid[se0 + so];
-        const int cell = m.se_cell[se0 + so];
-        const double mu = a.mu_cell ? a.mu_cell[cell] : P.mu;  // mu_arr of the side's cell
-        const double vol = m.se_vol[se];
-        for (int pp = m.se_sfoff[se]; pp < m.se_sfoff[se + 1]; pp++) {
-          const long sp = m.se_sf[pp];
-          const int fp0 = 3 * (int)(sp - sf0);
-          double np[3];
-          sf_normal(m, sp, m.sf_le[sp] == cell ? 1.0 : -1.0, np);
-          const double sc = al * m.sfn[3 * m.SF + sp] * (1.0 / vol);
+        const double sgl = side == 0 ? 1.0 : -1.0;
+        const double nl[3] = {sgl * L.g[l][0], sgl * L.g[l][1], sgl * L.g[l][2]};
+        const int cell = L.secell[so];
+        const double mu = L.semu[so];  // mu_arr of the side's cell
+        const double vol = L.sevol[so];
+        for (int e = 0; e < L.ssf[so][0]; e++) {
+          const int po = L.ssf[so][1 + e];
+          const int fp0 = 3 * po;
+          const double sgp = L.le[po] == cell ? 1.0 : -1.0;
+          const double np[3] = {sgp * L.g[po][0], sgp * L.g[po][1], sgp * L.g[po][2]};
+          const double sc = al * L.g[po][3] * (1.0 / vol);
           const double bmu = blc_mu_ij(sc, mu, mu0, nl, np, i, j);
           const double bmu_t = blc_mu_ij(sc, mu, mu0, nl, np, j, i);
           if (!neumann && !dirichlet) {
-            L.T[(se0l + i) + E * (fl0 + j)] = L.T[(se0l + i) + E * (fl0 + j)] + bmu_t;
-            L.N[(fl0 + i) + F * (fp0 + j)] = L.N[(fl0 + i) + F * (fp0 + j)] + bmu;
-            L.S[(fl0 + i) + F * (se0l + j)] = L.S[(fl0 + i) + F * (se0l + j)] + bmu;
+            Tm[(se0l + i) + E * (fl0 + j)] = Tm[(se0l + i) + E * (fl0 + j)] + bmu_t;
+            Nm[(fl0 + i) + F * (fp0 + j)] = Nm[(fl0 + i) + F * (fp0 + j)] + bmu;
+            Sm[(fl0 + i) + F * (se0l + j)] = Sm[(fl0 + i) + F * (se0l + j)] + bmu;
           } else if (neumann) {
             const double dd = nl[0] * np[0] + nl[1] * np[1] + nl[2] * np[2];
             const double bn = sc * (dd * (i == j ? 1.0 : 0.0) + (np[j] * nl[i]) / 3.0);
-            L.N[(fl0 + i) + F * (fp0 + j)] = L.N[(fl0 + i) + F * (fp0 + j)] + bn;
-            L.S[(fl0 + i) + F * (se0l + j)] = L.S[(fl0 + i) + F * (se0l + j)] + bn;
-            L.T[(se0l + i) + E * (fl0 + j)] = L.T[(se0l + i) + E * (fl0 + j)] + bmu_t;
+            Nm[(fl0 + i) + F * (fp0 + j)] = Nm[(fl0 + i) + F * (fp0 + j)] + bn;
+            Sm[(fl0 + i) + F * (se0l + j)] = Sm[(fl0 + i) + F * (se0l + j)] + bn;
+            Tm[(se0l + i) + E * (fl0 + j)] = Tm[(se0l + i) + E * (fl0 + j)] + bmu_t;
           } else {
-            L.T[(se0l + i) + E * (fl0 + j)] = L.T[(se0l + i) + E * (fl0 + j)] + bmu_t;
+            Tm[(se0l + i) + E * (fl0 + j)] = Tm[(se0l + i) + E * (fl0 + j)] + bmu_t;
           }
         }
       }
@@ -338,17 +370,16 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   // Dirichlet rows (each sub-face's rows only receive its own terms above)
   if (lane == 0) {
     for (int l = 0; l < nf; l++) {
-      const long s = sf0 + l;
-      const int re = m.sf_re[s];
+      const int re = L.re[l];
       if (re <= -2 && P.bc_V_id[-re - 1] == 2) {
-        const double al = m.sfn[3 * m.SF + s];
+        const double al = L.g[l][3];
         for (int d = 0; d < 3; d++) {
           const int r = 3 * l + d;
           L.B[r] = al * P.bc_val_V[-re - 1][d];
-          for (int c = 0; c < E; c++) L.S[r + F * c] = 0.0;
-          for (int c = 0; c < F; c++) L.N[r + F * c] = 0.0;
+          for (int c = 0; c < E; c++) Sm[r + F * c] = 0.0;
+          for (int c = 0; c < F; c++) Nm[r + F * c] = 0.0;
         }
-        for (int d = 0; d < 3; d++) L.N[(3 * l + d) + F * (3 * l + d)] = al * 1.0;
+        for (int d = 0; d < 3; d++) Nm[(3 * l + d) + F * (3 * l + d)] = al * 1.0;
       }
     }
   }
@@ -362,18 +393,18 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   for (int q = lane; q < 9 * ns; q += 64) {
     const int p = q / 9, ij = q % 9, i = ij % 3, j = ij / 3;
     double acc = 0.0;
-    for (int l = 0; l < nf; l++) acc = acc + L.T[(3 * p + i) + E * (3 * l + j)];
+    for (int l = 0; l < nf; l++) acc = acc + Tm[(3 * p + i) + E * (3 * l + j)];
     o_blcs[q] = acc;
   }
-  wave_inverse(L.N, F, L.piv, L.work, lane);
-  wave_gemm(E, F, F, L.T, L.N, L.W, lane);
+  wave_inverse(Nm, F, L.piv, L.work, lane);
+  wave_gemm(E, F, F, Tm, Nm, Wm, lane);
   wsync_d();
-  wave_gemm(E, E, F, L.W, L.S, o_sns, lane);
-  wave_gemv(F, F, L.N, L.B, L.x, lane);
+  wave_gemm(E, E, F, Wm, Sm, o_sns, lane);
+  wave_gemv(F, F, Nm, L.B, L.x, lane);
   wsync_d();
-  wave_gemv(E, F, L.T, L.x, o_sb, lane);
+  wave_gemv(E, F, Tm, L.x, o_sb, lane);
   for (int q = lane; q < F; q += 64) o_ninvb[q] = L.x[q];
-  wave_gemm(F, E, F, L.N, L.S, o_ninvs, lane);
+  wave_gemm(F, E, F, Nm, Sm, o_ninvs, lane);
 }
 
 // temp_u and dtheta_dU (ns_euler_primitives_module.F90:52-63, ns_diffusion_module.F90:196-206)
