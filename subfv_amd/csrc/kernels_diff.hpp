@@ -258,9 +258,12 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
   wsync_d();
 
-  // ---------------- heat (scalar entries, one lane, reference loop order)
-  if (lane == 0) {
-    for (int l = 0; l < nf; l++) {
+  // ---------------- heat (scalar entries, reference loop order).  Sub-face l
+  // only writes row l of N and S, column l of S~^T and B[l], so one lane per
+  // sub-face adds every entry's terms in the serial loop's order.
+  if (lane < nf) {
+    {
+      const int l = lane;
       const double al = L.g[l][3];
       for (int side = 0; side < 2; side++) {
         const int so = side == 0 ? L.lse[l] : L.rse[l];
@@ -310,15 +313,17 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   }
   wsync_d();
 
-  // ---------------- momentum (3x3 blocks, lane (i, j) < 9 owns entry (i, j))
+  // ---------------- momentum (3x3 blocks, lane (i, j) owns entry (i, j) of a block)
   for (int q = lane; q < fm * fm; q += 64) Nm[q] = 0.0;
   for (int q = lane; q < fm * em; q += 64) Sm[q] = Tm[q] = 0.0;
   for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
   wsync_d();
-  if (lane < 9) {
-    const int i = lane % 3, j = lane / 3;
+  // sub-face l only writes its rows of N and S and its columns of S~^T:
+  // lanes (l mod 7, i, j), l = lane / 9, lane / 9 + 7, ...
+  if (lane < 63) {
+    const int i = (lane % 9) % 3, j = (lane % 9) / 3;
     const double mu0 = a.mu_cell ? *a.mu_min : P.mu0;
-    for (int l = 0; l < nf; l++) {
+    for (int l = lane / 9; l < nf; l += 7) {
       const int re = L.re[l];
       const double al = L.g[l][3];
       bool neumann, dirichlet;
