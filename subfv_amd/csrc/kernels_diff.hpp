@@ -94,18 +94,28 @@ SFV_DEV void wsync_d() {
 SFV_DEV void wave_inverse(double *A, int n, int *piv, double *work, int lane) {
 #define AA(i, j) A[(i) + n * (j)]
   for (int k = 0; k < n; k++) {
-    if (lane == 0) {
-      int p = k;
-      double amax = fabs(AA(k, k));
-      for (int i = k + 1; i < n; i++)
-        if (fabs(AA(i, k)) > amax) {
-          amax = fabs(AA(i, k));
-          p = i;
+    // pivot: the first row of the largest |A(i, k)|, i >= k, as the serial
+    // scan with a strict > finds it (NaN never replaces the running maximum,
+    // so a NaN diagonal keeps p = k): a butterfly over the wave, ties to the
+    // lower row (n <= 48 <= 64 lanes)
+    int p;
+    {
+      const int i = k + lane;
+      double v = i < n ? fabs(AA(i, k)) : -1.0;
+      if (!(v == v)) v = -1.0;
+      int idx = i;
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
         }
-      piv[k] = p;
+      }
+      const double d0 = fabs(AA(k, k));
+      p = (d0 == d0) ? idx : k;
     }
-    wsync_d();
-    const int p = piv[k];
+    if (lane == 0) piv[k] = p;
     if (AA(p, k) != 0.0) {
       if (p != k)
         for (int j = lane; j < n; j += 64) {
@@ -158,9 +168,10 @@ SFV_DEV void wave_inverse(double *A, int n, int *piv, double *work, int lane) {
       AA(i, j) = 0.0;
     }
     wsync_d();
-    for (int jj = j + 1; jj < n; jj++) {
-      const double temp = -1.0 * work[jj];
-      for (int i = lane; i < n; i += 64) AA(i, j) = AA(i, j) + temp * AA(i, jj);
+    for (int i = lane; i < n; i += 64) {
+      double acc = AA(i, j);
+      for (int jj = j + 1; jj < n; jj++) acc = acc + (-1.0 * work[jj]) * AA(i, jj);
+      AA(i, j) = acc;
     }
     wsync_d();
   }
