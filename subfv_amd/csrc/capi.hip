@@ -46,7 +46,8 @@ struct subfv_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  std::vector<subfv_ctx *> members;
+  // members with the generation each had when the tables were built
+  std::vector<std::pair<subfv_ctx *, uint64_t>> members;
   int scheme = 0, total_g = 0, total_c = 0;
   sfv::InstDesc *d_desc = nullptr;
   int *d_gtab = nullptr, *d_ctab = nullptr;
@@ -59,6 +60,12 @@ struct subfv_batch {
 struct subfv_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // recorded on the stream when a batch takes the context, so the batch's
+  // stream waits for the context's pending uploads, state puts and inits
+  hipEvent_t ev = nullptr;
+  // bumped whenever the mesh, configuration, time or state changes; a batch
+  // rebuilds its tables and its captured graph when a member's differs
+  uint64_t gen = 0;
   std::string err;
   bool has_mesh = false;
   bool b2d = false;
@@ -439,6 +446,7 @@ int apply_bc(subfv_ctx *c) {
 
 // upload of a built host connectivity, device geometry and BC data
 int finish_mesh(subfv_ctx *c, int32_t boundary_2d) {
+  c->gen++;
   c->b2d = boundary_2d != 0;
   int r = upload_mesh(c);
   if (r) return r;
@@ -822,6 +830,7 @@ int subfv_create(subfv_ctx **out, int device) {
   subfv_ctx *c = new subfv_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void **)&c->derr, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&c->st, sizeof(StepState)) != hipSuccess) {
     delete c;
@@ -858,6 +867,7 @@ void subfv_destroy(subfv_ctx *c) {
   c->pins.clear();
   if (c->derr) (void)hipFree(c->derr);
   if (c->st) (void)hipFree(c->st);
+  if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1009,6 +1019,7 @@ int subfv_set_config(subfv_ctx *c, const subfv_config *cfg) {
     if (cfg->bc_euler_id[i] < 1 || cfg->bc_euler_id[i] > 9)
       return fail(c, SUBFV_E_INVALID, "ERROR: bc_type unrecognized");
   c->cfg = *cfg;
+  c->gen++;
   BCParams &b = c->bc;
   b = BCParams{};
   b.scheme = cfg->scheme_id;
@@ -1029,11 +1040,13 @@ int subfv_set_config(subfv_ctx *c, const subfv_config *cfg) {
 int subfv_set_time(subfv_ctx *c, double t) {
   if (!c) return SUBFV_E_INVALID;
   c->host_t = t;
+  c->gen++;
   return push_state_scalars(c, t);
 }
 
 int subfv_state_put(subfv_ctx *c, const double *sol) {
   if (!c || !c->has_mesh || !sol) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
+  c->gen++;
   return host_to_dev_soa(c, sol, 1, 5 * c->dm.C, c->U[c->cur]);
 }
 
@@ -1041,6 +1054,7 @@ int subfv_state_put(subfv_ctx *c, const double *sol) {
 // cell takes the left or right primitive state by its centroid's x
 int subfv_init_1drp(subfv_ctx *c, double x1drp, const double *w_left, const double *w_right) {
   if (!c || !c->has_mesh || !w_left || !w_right) return fail(c, SUBFV_E_INVALID, "no mesh or null state");
+  c->gen++;
   Init1drp a;
   a.x1drp = x1drp;
   for (int k = 0; k < 5; k++) {
@@ -1085,6 +1099,7 @@ int64_t subfv_put_f64(subfv_ctx *c, const char *name, const double *in, int64_t 
   struct Item { const char *nm; double *p; int nc; long n; };
   Item items[] = {{"sol", c->U[c->cur], 1, 5 * C}, {"rhs", c->rhs, 5, C}, {"rhs_o2", c->rhs2, 5, C},
                   {"sum_lambda", c->sl, 1, C}, {"cell_grad", c->G, 1, 15 * C}};
+  c->gen++;
   for (auto &it : items)
     if (n == it.nm) {
       if ((int64_t)it.nc * it.n != nval) return -1;
@@ -2239,18 +2254,31 @@ int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps
     if (c->dm.ngroups > kTabMask || (long)nblk(c->dm.C, kCellThreads) > kTabMask)
       return bfail(b, SUBFV_E_INVALID, "instance too large for a batch");
   }
-  std::vector<subfv_ctx *> mem(ctxs, ctxs + n);
+  std::vector<std::pair<subfv_ctx *, uint64_t>> mem(n);
+  for (int i = 0; i < n; i++) mem[i] = {ctxs[i], ctxs[i]->gen};
+  // the batch's stream waits for whatever each member has queued on its own
+  // stream (uploads, state puts, device inits, time pushes)
+  for (int i = 0; i < n; i++) {
+    BHIP(b, hipEventRecord(ctxs[i]->ev, ctxs[i]->stream));
+    BHIP(b, hipStreamWaitEvent(b->stream, ctxs[i]->ev, 0));
+  }
   if (mem != b->members || scheme != b->scheme) {
-    // pending uploads and state puts of the members complete first
-    for (auto *c : mem) BHIP(b, hipStreamSynchronize(c->stream));
+    // a member's mesh may have changed under the same context pointer (a
+    // pooled context re-used for the next request): rebuild the tables and
+    // drop the graph whenever a (context, generation) pair differs
     b->members = mem;
     b->scheme = scheme;
     std::vector<int> gt, ct;
+    long sum_g = 0, sum_c = 0;
     for (int i = 0; i < n; i++) {
       for (int g = 0; g < ctxs[i]->dm.ngroups; g++) gt.push_back((i << kTabShift) | g);
       const int nc = (int)nblk(ctxs[i]->dm.C, kCellThreads);
       for (int q = 0; q < nc; q++) ct.push_back((i << kTabShift) | q);
+      sum_g += ctxs[i]->dm.ngroups;
+      sum_c += nc;
     }
+    if ((long)gt.size() != sum_g || (long)ct.size() != sum_c)
+      return bfail(b, SUBFV_E_INVALID, "batch tables do not match the members' meshes");
     int r;
     if ((r = grow(b, &b->d_gtab, &b->cap_gtab, gt.size()))) return r;
     if ((r = grow(b, &b->d_ctab, &b->cap_ctab, ct.size()))) return r;
@@ -2279,7 +2307,7 @@ int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps
   }
   for (; s < n_steps; s++) launch_phase_any(b, n, s % 3);
   BHIP(b, hipGetLastError());
-  for (auto *c : mem) c->cur = (c->cur + n_steps) % 3;
+  for (int i = 0; i < n; i++) ctxs[i]->cur = (ctxs[i]->cur + n_steps) % 3;
   return 0;
 }
 

@@ -42,6 +42,8 @@ __global__ __launch_bounds__(kGrpThreads, 3) void k_node_flux_multi(const InstDe
   // constant address space lets the compiler keep them across the sweep's
   // barriers instead of re-reading them
   const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
+  // a table entry past the instance's groups would read foreign headers
+  if ((e & kTabMask) >= D.ngroups) return;
   node_flux_group<SCHEME>(D.fa, D.U[(D.cur0 + phase) % 3], e & kTabMask, L);
 }
 
@@ -49,6 +51,7 @@ __global__ __launch_bounds__(kCellThreads) void k_gather_multi(const InstDesc *d
   if ((int)blockIdx.x >= total) return;
   const int e = ctab[blockIdx.x];
   const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
+  if ((e & kTabMask) >= D.ncblk) return;
   gather_block(D.ga, e & kTabMask);
 }
 
@@ -62,6 +65,7 @@ __global__ __launch_bounds__(kCellThreads) void k_update_multi(const InstDesc *d
   if ((int)blockIdx.x >= total) return;
   const int e = ctab[blockIdx.x];
   const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
+  if ((e & kTabMask) >= D.ncblk) return;
   update_block<false>(D.ua, D.U[(D.cur0 + phase) % 3], D.U[(D.cur0 + phase + 1) % 3], e & kTabMask);
 }
 
@@ -75,6 +79,7 @@ __global__ __launch_bounds__(kCellThreads) void k_update_fixup_multi(const InstD
   if ((int)blockIdx.x >= total) return;
   const int e = ctab[blockIdx.x];
   const InstDesc &D = *(const InstDesc *)((const CInstDesc *)d + (e >> kTabShift));
+  if ((e & kTabMask) >= D.ncblk) return;
   update_fixup_block<false>(D.ua, D.U[(D.cur0 + phase) % 3], D.U[(D.cur0 + phase + 1) % 3], e & kTabMask);
 }
 

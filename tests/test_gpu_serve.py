@@ -94,3 +94,19 @@ def test_finished_solves_are_written_in_the_background(requests, tmp_path):
         assert futs[r.rid].exception() is None
         rows = np.loadtxt(tmp_path / f"req{r.rid}.dat").reshape(-1, 13)
         assert np.array_equal(rows[:, 3:8], r.sol)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_pooled_context_reused_for_a_different_mesh(requests, graph):
+    """One context serves every request in turn (capacity 1, no prefetch):
+    each new request uploads a mesh of another size into the context the
+    batch used before, so the batch must rebuild its tables and graph."""
+    print(f"[gpu] serve pooled context reuse graph={graph}", flush=True)
+    alone, _ = S.serve_plain(requests, keep=True, realtime=False)
+    one, _ = S.serve(requests, S.Params(chunk=6, max_batch=1, prefetch=0, first=1, use_graph=graph), keep=True,
+                     realtime=False)
+    a = {r.rid: r.sol for r in alone}
+    m = {r.rid: r.sol for r in one}
+    assert set(a) == set(m)
+    for rid in a:
+        assert np.array_equal(a[rid], m[rid]), rid
