@@ -22,12 +22,6 @@ module subfv_ns_gpu
 
   integer, parameter :: DP = c_double
 
-  ! gmsh element kind -> number of vertices (mesh_reading_module.F90:1006-1009)
-  integer, parameter :: n_vert_kind(31) = (/ &
-    2, 3, 4, 4, 8, 6, 5, 3, 6, 9, 10, &
-    27, 18, 14, 1, 8, 20, 15, 13, 9, 10, 12, &
-    15, 15, 21, 4, 5, 6, 20, 35, 56/)
-
   type :: raw_mesh
     integer(c_int32_t) :: n_vert = 0, n_elems = 0, n_bfaces = 0
     real(c_double), allocatable :: coord(:, :)
@@ -36,6 +30,35 @@ module subfv_ns_gpu
   end type raw_mesh
 
 contains
+
+  ! vertices of a gmsh element kind (gmsh format 2.2 element types 1-31:
+  ! lines, triangles, quadrangles, tetrahedra, hexahedra, prisms, pyramids and
+  ! their higher-order forms, points)
+  pure integer function verts_of_kind(kind) result(n)
+    integer, intent(in) :: kind
+    select case (kind)
+    case (15); n = 1
+    case (1); n = 2
+    case (2, 8); n = 3
+    case (3, 4, 26); n = 4
+    case (7, 27); n = 5
+    case (6, 9, 28); n = 6
+    case (5, 16); n = 8
+    case (10, 20); n = 9
+    case (11, 21); n = 10
+    case (22); n = 12
+    case (19); n = 13
+    case (14); n = 14
+    case (18, 23, 24); n = 15
+    case (13); n = 18
+    case (17, 29); n = 20
+    case (25); n = 21
+    case (12); n = 27
+    case (30); n = 35
+    case (31); n = 56
+    case default; n = 0
+    end select
+  end function
 
   subroutine check(ctx, st)
     type(c_ptr), intent(in) :: ctx
@@ -46,104 +69,101 @@ contains
     end if
   end subroutine
 
-  ! msh 2.2 reader with the reference's semantics: boundary faces (gmsh kinds
-  ! 2 and 3) get the index j of the bc_name whose physical tag they carry, 0
-  ! (default wall) otherwise; volume elements are kinds 4-7 in file order.
+  ! msh 2.2 reader (gmsh ASCII format 2.2: sections "$Name" ... "$EndName";
+  ! $PhysicalNames lines "dim tag name", $Nodes lines "id x y z", $Elements
+  ! lines "id kind ntags tag(1:ntags) vertex(1:n)"), with the reference's
+  ! semantics (mesh_reading_module.F90:989-1324): nodes in file order (their
+  ! ids are not used), volume elements of kinds 4-7 in file order, boundary
+  ! faces of kinds 2 and 3 tagged with the bc index whose name carries the
+  ! face's first (physical) tag, the last such index when several match, 0
+  ! (default wall) when none does.  With n_bc > 0 the file must name its
+  ! physical groups before its nodes.
   subroutine read_mesh_msh_2(mesh, meshfile_path, meshfile, n_bc, bc_name)
     type(raw_mesh), intent(out) :: mesh
     character(len=*), intent(in) :: meshfile_path, meshfile
     integer, intent(in) :: n_bc
     character(len=*), dimension(:), intent(in) :: bc_name
-    integer :: funit, i, j, n_items, dummy, kind, ntag, nv, bc_tag(max(n_bc, 1)), n_bc_tmp, tag
-    integer :: iv, ib, ie, nbv
-    integer :: tags(16), verts(64)
-    character(len=255) :: text
-    integer, allocatable :: kinds(:), nvs(:), ptag(:), vlist(:, :)
+    character(len=512) :: line
+    character(len=255) :: gname
+    integer :: u, ios, count, k, m, gdim, gtag, head(3), nv, ntag
+    integer :: ne, nb, lv, lb
+    integer :: fld(96)
+    integer, allocatable :: phys_of_bc(:)
+    logical :: have_names
 
-    bc_tag = -huge(1)
-    open (newunit=funit, file=trim(adjustl(meshfile_path))//trim(adjustl(meshfile)), status="old")
-    if (n_bc /= 0) then
-      read (funit, '(a)') text
-      do while ("$PhysicalNames" /= text(1:14))
-        read (funit, '(a)') text
-        if ("$Nodes" == text(1:6)) then
+    allocate (phys_of_bc(max(n_bc, 1)))
+    phys_of_bc = -huge(1)
+    have_names = .false.
+    ne = 0; nb = 0; lv = 0; lb = 0
+    open (newunit=u, file=trim(adjustl(meshfile_path))//trim(adjustl(meshfile)), status="old", action="read")
+    do
+      read (u, '(a)', iostat=ios) line
+      if (ios /= 0) exit
+      select case (trim(adjustl(line)))
+      case ("$PhysicalNames")
+        have_names = .true.
+        read (u, *) count
+        do k = 1, count
+          read (u, *) gdim, gtag, gname
+          do m = 1, n_bc
+            if (trim(adjustl(bc_name(m))) == trim(adjustl(gname))) phys_of_bc(m) = gtag
+          end do
+        end do
+      case ("$Nodes")
+        if (n_bc /= 0 .and. .not. have_names) then
           print *, "No Physical names inside mesh!!!"
           error stop
         end if
-      end do
-      read (funit, *) n_bc_tmp
-      do i = 1, n_bc_tmp
-        read (funit, *) dummy, tag, text
-        do j = 1, n_bc
-          if (trim(adjustl(bc_name(j))) == trim(adjustl(text))) bc_tag(j) = tag
+        read (u, *) mesh%n_vert
+        allocate (mesh%coord(3, mesh%n_vert))
+        do k = 1, mesh%n_vert
+          read (u, *) gtag, mesh%coord(:, k)
         end do
-      end do
-    end if
-    read (funit, '(a)') text
-    do while ("$Nodes" /= text(1:6))
-      read (funit, '(a)') text
+      case ("$Elements")
+        read (u, *) count
+        ! first pass over the section: sizes only
+        allocate (mesh%kind(count), mesh%elem_vert(8 * count))
+        allocate (mesh%bf_nvert(count), mesh%bf_vert(4 * count), mesh%bf_tag(count))
+        do k = 1, count
+          read (u, '(a)') line
+          read (line, *) head
+          ntag = head(3)
+          nv = verts_of_kind(head(2))
+          if (3 + ntag + nv > size(fld)) error stop "msh element line too long"
+          read (line, *) fld(1:3 + ntag + nv)
+          select case (head(2))
+          case (4:7)
+            ne = ne + 1
+            mesh%kind(ne) = head(2)
+            mesh%elem_vert(lv + 1:lv + nv) = fld(4 + ntag:3 + ntag + nv)
+            lv = lv + nv
+          case (2:3)
+            nb = nb + 1
+            mesh%bf_nvert(nb) = nv
+            mesh%bf_vert(lb + 1:lb + nv) = fld(4 + ntag:3 + ntag + nv)
+            lb = lb + nv
+            mesh%bf_tag(nb) = 0
+            if (ntag >= 1) then
+              do m = 1, n_bc
+                if (phys_of_bc(m) == fld(4)) mesh%bf_tag(nb) = m
+              end do
+            end if
+          end select
+        end do
+      end select
     end do
-    read (funit, *) mesh%n_vert
-    allocate (mesh%coord(3, mesh%n_vert))
-    do i = 1, mesh%n_vert
-      read (funit, *) dummy, mesh%coord(1, i), mesh%coord(2, i), mesh%coord(3, i)
-    end do
-    read (funit, '(a)') text
-    do while ("$Elements" /= text(1:9))
-      read (funit, '(a)') text
-    end do
-    read (funit, *) n_items
-    allocate (kinds(n_items), nvs(n_items), ptag(n_items), vlist(8, n_items))
-    mesh%n_elems = 0
-    mesh%n_bfaces = 0
-    nbv = 0
-    iv = 0
-    do i = 1, n_items
-      read (funit, '(a)') text
-      read (text, *) dummy, kind, ntag
-      nv = n_vert_kind(kind)
-      read (text, *) dummy, kind, ntag, tags(1:ntag), verts(1:nv)
-      kinds(i) = kind
-      nvs(i) = nv
-      ptag(i) = tags(1)
-      vlist(:, i) = 0
-      if (nv <= 8) vlist(1:nv, i) = verts(1:nv)
-      if (kind == 2 .or. kind == 3) then
-        mesh%n_bfaces = mesh%n_bfaces + 1
-        nbv = nbv + nv
-      else if (kind >= 4 .and. kind <= 7) then
-        mesh%n_elems = mesh%n_elems + 1
-        iv = iv + nv
-      end if
-    end do
-    close (funit)
-    if (mesh%n_elems < 1) then
+    close (u)
+    if (ne < 1) then
       print *, "[-] No 3D elements in mesh !"
       error stop
     end if
-    allocate (mesh%kind(mesh%n_elems), mesh%elem_vert(iv))
-    allocate (mesh%bf_nvert(max(mesh%n_bfaces, 1)), mesh%bf_vert(max(nbv, 1)), mesh%bf_tag(max(mesh%n_bfaces, 1)))
-    ie = 0
-    ib = 0
-    iv = 0
-    nbv = 0
-    do i = 1, n_items
-      if (kinds(i) == 2 .or. kinds(i) == 3) then
-        ib = ib + 1
-        mesh%bf_nvert(ib) = nvs(i)
-        mesh%bf_vert(nbv + 1:nbv + nvs(i)) = vlist(1:nvs(i), i)
-        nbv = nbv + nvs(i)
-        mesh%bf_tag(ib) = 0
-        do j = 1, n_bc
-          if (bc_tag(j) == ptag(i)) mesh%bf_tag(ib) = j
-        end do
-      else if (kinds(i) >= 4 .and. kinds(i) <= 7) then
-        ie = ie + 1
-        mesh%kind(ie) = kinds(i)
-        mesh%elem_vert(iv + 1:iv + nvs(i)) = vlist(1:nvs(i), i)
-        iv = iv + nvs(i)
-      end if
-    end do
+    mesh%n_elems = ne
+    mesh%n_bfaces = nb
+    mesh%kind = mesh%kind(1:ne)
+    mesh%elem_vert = mesh%elem_vert(1:lv)
+    mesh%bf_nvert = mesh%bf_nvert(1:max(nb, 1))
+    mesh%bf_vert = mesh%bf_vert(1:max(lb, 1))
+    mesh%bf_tag = mesh%bf_tag(1:max(nb, 1))
   end subroutine
 
   ! build_mesh + compute_geometry_mesh on the device (one rank, sub-entities);
