@@ -244,6 +244,16 @@ int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs);
  * stats[2] = SpMV count. Returns SUBFV_E_NOT_CONVERGED (soft) if not conv. */
 int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                           int32_t precision, int32_t *conv, double *stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): subfv_bcsr_gmres_diag with its Krylov basis V and
+ * Hessenberg matrix H in a file-scope device array of 4 Mi doubles (32 MiB per
+ * GPU, held until the process ends) instead of memory the library allocates.
+ * Requires m*b*(restart + 1) + (restart + 1)*restart <= 4194304 (4 Mi), else
+ * SUBFV_E_INVALID with a message; results equal subfv_bcsr_gmres_diag's.
+ * Calls that overlap on one GPU share the array and race. */
+int subfv_bcsr_gmres_diag_noalloc(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
+                                  int32_t precision, int32_t *conv, double *stats);
 /* The other GMRES loops on the same interface: precond 0 = bcsr_gmres
  * (:194-290, no preconditioner, 20 restarts), 1 = bcsr_gmres_diag (as above),
  * 2 = bcsr_gmres_lusgs (:447-548, bcsr_lusgs inside every Arnoldi step and on

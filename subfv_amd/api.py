@@ -349,6 +349,20 @@ class Bcsr:
             self.solver._check(st)
         return x, bool(conv.value), {"restarts": int(stats[0]), "rho": float(stats[1]), "spmv": int(stats[2])}
 
+    def gmres_diag_noalloc(self, rhs, tol: float, restart: int = 30, precision: int = 0):
+        """gmres_diag with its Krylov basis in the library's fixed device
+        scratch array (subfv_bcsr_gmres_diag_noalloc: no device allocation;
+        n * (restart + 1) + (restart + 1) * restart <= 4 Mi doubles)."""
+        rhs = self._vec(rhs)
+        x = np.zeros(self.n)
+        conv = C.c_int32(0)
+        stats = np.zeros(3)
+        st = _lib.lib().subfv_bcsr_gmres_diag_noalloc(self.h, _lib.dptr(x), _lib.dptr(rhs), tol, restart, precision,
+                                                      C.byref(conv), _lib.dptr(stats))
+        if st not in (0, _lib.E_NOT_CONVERGED):
+            self.solver._check(st)
+        return x, bool(conv.value), {"restarts": int(stats[0]), "rho": float(stats[1]), "spmv": int(stats[2])}
+
     PRECONDITIONERS = {"none": 0, "diag": 1, "lusgs": 2, "parallel_lusgs": 3}
 
     def gmres(self, rhs, tol: float, restart: int = 30, precond: str = "none", precision: int = 0):

@@ -22,6 +22,39 @@
 
 using namespace sfv;
 
+#include <mutex>
+
+// Workspace of subfv_bcsr_gmres_diag_noalloc: the Krylov basis V and the
+// Hessenberg matrix H of one GMRES call live in this file-scope device array
+// instead of memory the library allocates.  Each GPU holds its own copy for
+// the life of the process; host code must use the per-device address that
+// hipGetSymbolAddress returns (noalloc_ptr), never the array's name.
+constexpr size_t kNoAllocDoubles = (size_t)4 << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+double *noalloc_ptr[kMaxDevices] = {};
+
+// once per device, with that device current
+hipError_t noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (noalloc_ptr[device]) return hipSuccess;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) noalloc_ptr[device] = (double *)p;
+  return e;
+}
+
+double *noalloc_get(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  return noalloc_ptr[device];
+}
+}  // namespace
+
 // Device-only tables derived from a host mesh (right-neighbour codes, slot
 // cells, the cells' slots in vertex order, vertex groups and their slot
 // sources), built once with the host mesh so an upload only copies.
@@ -834,6 +867,10 @@ int subfv_create(subfv_ctx **out, int device) {
       hipMalloc((void **)&c->derr, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&c->st, sizeof(StepState)) != hipSuccess) {
     delete c;
+    return SUBFV_E_DEVICE;
+  }
+  if (noalloc_setup(device) != hipSuccess) {
+    subfv_destroy(c);
     return SUBFV_E_DEVICE;
   }
   (void)hipMemset(c->derr, 0, 2 * sizeof(int));
@@ -1778,7 +1815,7 @@ namespace {
 // on the device, the Givens rotations and back substitution on the host in
 // the reference order.
 int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double tol, int32_t mr,
-               int32_t precision, int32_t *conv_out, double *stats) {
+               int32_t precision, int32_t *conv_out, double *stats, double *ws = nullptr) {
   if (!A || !x_out || !rhs || mr < 1 || mr > 200 || kind < 0 || kind > 3) return SUBFV_E_INVALID;
   subfv_ctx *c = A->ctx;
   hipStream_t s = c->stream;
@@ -1786,12 +1823,19 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
   if (fp32 && kind >= 2) return fail(c, SUBFV_E_INVALID, "the LU-SGS variants run in FP64 only");
   if (kind == 3 && A->nsets <= 0) return fail(c, SUBFV_E_INVALID, "no colour sets: call subfv_bcsr_set_colors first");
   const int maxit = kind == 1 ? 40 : 20;
-  int r = ensure_workspace(A, mr);
-  if (r) return r;
   const long n = A->n;
   const int ldh = mr + 1;
-  auto Vc = [&](int j) { return A->V + (long)(j - 1) * n; };  // 1-based column
-  auto Hd = [&](int i, int j) { return A->H + (i - 1) + (long)ldh * (j - 1); };
+  // the Krylov basis V (n x (mr + 1)) and H ((mr + 1) x mr): the handle's
+  // own workspace, or the caller-given ws that holds both
+  int r = 0;
+  double *Vb = ws, *Hb = ws ? ws + (size_t)n * (mr + 1) : nullptr;
+  if (!ws) {
+    if ((r = ensure_workspace(A, mr))) return r;
+    Vb = A->V;
+    Hb = A->H;
+  }
+  auto Vc = [&](int j) { return Vb + (long)(j - 1) * n; };  // 1-based column
+  auto Hd = [&](int i, int j) { return Hb + (i - 1) + (long)ldh * (j - 1); };
   dim3 vg(kRedBlocks), vb(kVecThreads);
   long nspmv = 0;
   HIPCHK(c, hipMemcpyAsync(A->rhs, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1833,7 +1877,7 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
         hipLaunchKernelGGL(k_scale_norm, vg, vb, 0, s, n, w, (const double *)pa, Hd(j, j - 1));
       }
       HIPCHK(c, hipGetLastError());
-      HIPCHK(c, hipMemcpyAsync(H.data(), A->H, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(c, hipMemcpyAsync(H.data(), Hb, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
       auto h = [&](int i, int j) -> double & { return H[(i - 1) + (size_t)ldh * (j - 1)]; };
       int nr = mr, i, exited = 0;
@@ -1870,9 +1914,9 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       }
       HIPCHK(c, hipMemcpyAsync(A->z, z.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
       if (kind == 0) {
-        hipLaunchKernelGGL(k_add_combination, vg, vb, 0, s, n, nr, (const double *)A->V, n, (const double *)A->z, A->x);
+        hipLaunchKernelGGL(k_add_combination, vg, vb, 0, s, n, nr, (const double *)Vb, n, (const double *)A->z, A->x);
       } else {
-        hipLaunchKernelGGL(k_combine, vg, vb, 0, s, n, nr, A->V, n, A->z, A->r);
+        hipLaunchKernelGGL(k_combine, vg, vb, 0, s, n, nr, Vb, n, A->z, A->r);
         if ((r = precond(A, kind, fp32, A->r, A->zp))) return r;
         hipLaunchKernelGGL(k_add, vg, vb, 0, s, n, A->x, A->zp);
       }
@@ -1904,6 +1948,21 @@ extern "C" {
 int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                           int32_t precision, int32_t *conv, double *stats) {
   return gmres_impl(A, 1, x, rhs, tol, restart, precision, conv, stats);
+}
+
+// bcsr_gmres_diag with V and H in the file-scope scratch array: no device
+// allocation.  Calls that overlap on one GPU share the array and race.
+int subfv_bcsr_gmres_diag_noalloc(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
+                                  int32_t precision, int32_t *conv, double *stats) {
+  if (!A || !x || !rhs || restart < 1 || restart > 200) return SUBFV_E_INVALID;
+  const size_t need = (size_t)A->n * (restart + 1) + (size_t)(restart + 1) * restart;
+  if (need > kNoAllocDoubles)
+    return fail(A->ctx, SUBFV_E_INVALID,
+                "subfv_bcsr_gmres_diag_noalloc: n * (restart + 1) + (restart + 1) * restart = " + std::to_string(need) +
+                    " doubles exceeds the " + std::to_string(kNoAllocDoubles) + "-double scratch array");
+  double *ws = noalloc_get(A->ctx->device);
+  if (!ws) return fail(A->ctx, SUBFV_E_DEVICE, "scratch array address not set up for this device");
+  return gmres_impl(A, 1, x, rhs, tol, restart, precision, conv, stats, ws);
 }
 
 int subfv_bcsr_gmres(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart, int32_t precond_kind,
