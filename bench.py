@@ -152,8 +152,9 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cm
     against the same requests solved one at a time.  Mesh connectivity is
     prepared on the client before the arrivals (outside the timed region);
     upload, geometry, initial state, the steps and the read-back are timed."""
-    from subfv_amd import serve as S
+    import numpy as np
 
+    from subfv_amd import serve as S
     from subfv_amd.api import Solver
 
     reqs = S.prepare(S.make_requests(n_req, rate=rate, steps=steps, cmin=cmin, cmax=cmax))
@@ -161,12 +162,17 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cm
     best, trials = S.autotune(calib)
     # contexts are created before the timed runs, as a server holds them
     one = Solver(0)
-    plain = S.summarize(*S.serve_plain(reqs, solver=one))
+    plain_res, plain_ms = S.serve_plain(reqs, solver=one, keep=True)
+    plain = S.summarize(plain_res, plain_ms)
     one.close()
     pool = [Solver(0) for _ in range(best.max_batch + best.prefetch)]
-    served = S.summarize(*S.serve(reqs, best, pool=pool))
+    served_res, served_ms = S.serve(reqs, best, pool=pool, keep=True)
+    served = S.summarize(served_res, served_ms)
     for s in pool:
         s.close()
+    # every served solve must equal the same request solved alone, bit for bit
+    ref = {r.rid: r.sol for r in plain_res}
+    mismatches = sum(int(not np.array_equal(r.sol, ref[r.rid])) for r in served_res)
     cells = sum(r.cells for r in reqs)
     return {
         "requests": n_req, "steps_per_request": steps, "arrival_rate_per_s": rate,
@@ -174,6 +180,7 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cm
         "mean_cells": cells / n_req, "cell_steps": cells * steps,
         "served": served, "plain": plain,
         "throughput_vs_plain": served["solves_per_s"] / plain["solves_per_s"],
+        "served_vs_alone_mismatches": mismatches,
         "autotuned": {"chunk": best.chunk, "max_batch": best.max_batch, "use_graph": best.use_graph},
         "autotune_trials": trials,
         "note": "subsample of the 4096-request stream; same requests and arrival times for both",
@@ -235,7 +242,9 @@ def main():
     if viscous:
         s.set_diffusion(True, mu=1e-3, prandtl=0.72)
     s.set_time(0.0)
-    s.put_sol(mg.vortex_state(s.elem_coord))
+    sol0 = mg.vortex_state(s.elem_coord)
+    s.put_sol(sol0)
+    vol = s.elem_volume
     setup_s = time.perf_counter() - t0
     print(f"# mesh {C} cells, {V} vertices, {SF} sub-faces, {SE} sub-elements; setup {setup_s:.1f}s",
           file=sys.stderr, flush=True)
@@ -254,6 +263,16 @@ def main():
         if cnt:
             phases[ph] = {"ms_total": ms, "launches": cnt, "us_avg": 1e3 * ms / cnt}
     s.profile(False)
+    # sanity of the timed run: finite states and mass conserved to round-off
+    # (all-wall mesh: sum of volume * rho is invariant)
+    import numpy as np
+
+    sol1 = s.get_sol()
+    mass0 = float(np.dot(vol, sol0[:, 0]))
+    mass1 = float(np.dot(vol, sol1[:, 0]))
+    checks = {"finite": bool(np.isfinite(sol1).all()), "mass_drift_rel": abs(mass1 - mass0) / mass0,
+              "mass_drift_bound": 1e-13}
+    checks["ok"] = checks["finite"] and checks["mass_drift_rel"] <= checks["mass_drift_bound"]
 
     value = C * stages * args.steps / elapsed
     # node sweep k_node_flux, launched once per stage for the first-order
@@ -317,6 +336,7 @@ def main():
             "phases_us_avg": {k: v["us_avg"] for k, v in phases.items()},
         },
         "t_final": tf,
+        "checks": checks,
     }
     s.close()
     if args.served_requests > 0 and not args.first_order:
@@ -333,6 +353,9 @@ def main():
         else:
             out["cpu_baseline"] = cpu_baseline(args.cpu_n, steps=2, viscous=viscous)
     print(json.dumps(out), flush=True)
+    if not checks["ok"]:
+        print(f"bench: timed run failed its checks: {checks}", file=sys.stderr, flush=True)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

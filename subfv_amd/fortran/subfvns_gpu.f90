@@ -4,8 +4,8 @@
 ! explicit path and its viscous terms use are honoured) and its msh 2.2 mesh,
 ! builds the mesh on
 ! the device, sets the initial state (init_uniform, init_1drp,
-! init_isentropic_vortex or init_gresho,
-! ns_euler_module.F90:20-70), runs the time loop of ns_main.F90:83-107 with
+! init_isentropic_vortex, init_gresho, init_sedov or init_double_mach,
+! ns_euler_module.F90:20-138), runs the time loop of ns_main.F90:83-107 with
 ! the state resident in HBM, prints the final time and the wall time of the
 ! loop (as ns_main.F90:113-126 does) and writes the final sol(5, n_elems) to
 ! sol_final.bin (stream: int32 n_elems, then the doubles).
@@ -77,7 +77,8 @@ program subfvns_gpu
   type(raw_mesh) :: mesh
   type(subfv_config) :: cfg
   real(DP), allocatable :: sol(:, :), xc(:, :), vol(:)
-  real(DP) :: t, dt, w(5), wall_time
+  real(DP) :: t, dt, w(5), wall_time, rmin, r
+  integer :: imin
   integer(8) :: start_count, end_count, count_rate
   integer(c_int) :: st
   integer(c_int32_t) :: bc_v_id(mnbc) = 1, bc_t_id(mnbc) = 1
@@ -153,8 +154,34 @@ program subfvns_gpu
   if (init_1drp .and. .not. init_uniform) then
     ! init_sol's init_1drp branch on the device (ns_euler_module.F90:36-43)
     call check(ctx, subfv_init_1drp(ctx, x1drp, sol_w_1drp_l, sol_w_1drp_r))
+  else if (init_sedov .and. .not. (init_uniform .or. init_isentropic_vortex .or. init_gresho)) then
+    ! init_sedov (ns_euler_module.F90:64-111, one rank): rho = 1 at rest with
+    ! E = 1e-16 / (gamma - 1), and the blast energy in the first cell of
+    ! smallest centroid norm (the x-y norm with boundary_2d)
+    rmin = huge(1.0_DP)
+    imin = -1
+    do i = 1, n_elems
+      if (boundary_2d) then
+        r = sqrt(xc(1, i)**2 + xc(2, i)**2)
+      else
+        r = sqrt(xc(1, i)**2 + xc(2, i)**2 + xc(3, i)**2)
+      end if
+      if (r < rmin) then
+        rmin = r
+        imin = i
+      end if
+    end do
+    sol(1, :) = 1.0_DP
+    sol(2:4, :) = 0.0_DP
+    sol(5, :) = 1e-16_DP / (gamma - 1.0_DP)
+    if (boundary_2d) then
+      sol(5, imin) = 0.984042_DP / vol(imin)
+    else
+      sol(5, imin) = 0.851072_DP / vol(imin)
+    end if
+    call check(ctx, subfv_state_put(ctx, sol))
   else
-    ! the other init_sol branches in the reference's order (ns_euler_module.F90:32-53)
+    ! the other init_sol branches in the reference's order (ns_euler_module.F90:32-138)
     do i = 1, n_elems
       if (init_uniform) then
         w = sol_uniform
@@ -162,13 +189,28 @@ program subfvns_gpu
         w = isentropic_vortex(xc(:, i))
       else if (init_gresho) then
         w = gresho_state(xc(:, i), 1.0e-5_DP)
+      else if (init_double_mach) then
+        ! the shock of the double Mach reflection through (0.1667, 0) at 60
+        ! degrees; tmax is set to 0.2 as init_sol does (:127-138)
+        if (xc(2, i) < 1.732_DP * (xc(1, i) - 0.1667_DP)) then
+          w = (/1.4_DP, 0.0_DP, 0.0_DP, 0.0_DP, 1.0_DP/)
+        else
+          w = (/8.0_DP, 7.145_DP, -4.125_DP, 0.0_DP, 116.5_DP/)
+        end if
       else
-        error stop "only init_uniform, init_1drp, init_isentropic_vortex and init_gresho are supported"
+        error stop "supported initial states: init_uniform, init_1drp, init_isentropic_vortex, init_gresho, " // &
+          "init_sedov, init_double_mach"
       end if
       sol(1, i) = w(1)
       sol(2:4, i) = w(1) * w(2:4)
       sol(5, i) = w(5) / (gamma - 1.0_DP) + 0.5_DP * w(1) * (w(2)**2 + w(3)**2 + w(4)**2)
     end do
+    if (init_double_mach .and. .not. (init_uniform .or. init_isentropic_vortex .or. init_gresho)) then
+      tmax = 0.2_DP
+      cfg%tmax = tmax
+      print *, "T_max =", tmax
+      call check(ctx, subfv_set_config(ctx, cfg))
+    end if
     call check(ctx, subfv_state_put(ctx, sol))
   end if
   call check(ctx, subfv_set_time(ctx, 0.0_DP))

@@ -94,7 +94,8 @@ def quad_mesh(n: int, jitter: float = 0.2, seed: int = DEFAULT_SEED, lx: float =
     """n x n jittered quads extruded one layer in z (gmsh hexahedra).
 
     ``tags='lr'`` tags the x = x0 side faces 1 and the x = x0 + lx side faces 2,
-    everything else untagged (default wall)."""
+    everything else untagged (default wall); ``tags='lrbt'`` also tags the
+    y = y0 side faces 3 and the y = y0 + ly side faces 4."""
     h = lx / n
     if dz is None:
         dz = h
@@ -116,15 +117,23 @@ def quad_mesh(n: int, jitter: float = 0.2, seed: int = DEFAULT_SEED, lx: float =
     ev = np.stack([a, b, c, d, a + n2, b + n2, c + n2, d + n2], axis=1) + 1
     kind = np.full(n * n, 5, np.int32)
     mesh = Mesh(coord, kind, ev.astype(np.int32).ravel(), boundary_2d=True, name=f"quad{n}")
-    if tags == "lr":
+    if tags in ("lr", "lrbt"):
         left = np.arange(n) * (n + 1)
         right = left + n
         fl = np.stack([left, left + n2, left + n + 1 + n2, left + n + 1], 1) + 1
         fr = np.stack([right, right + n + 1, right + n + 1 + n2, right + n2], 1) + 1
-        faces = np.concatenate([fl, fr]).astype(np.int32)
-        mesh.bf_nvert = np.full(2 * n, 4, np.int32)
+        groups = [fl, fr]
+        if tags == "lrbt":
+            bot = np.arange(n)
+            top = bot + n * (n + 1)
+            groups.append(np.stack([bot, bot + 1, bot + 1 + n2, bot + n2], 1) + 1)
+            groups.append(np.stack([top, top + n2, top + 1 + n2, top + 1], 1) + 1)
+        faces = np.concatenate(groups).astype(np.int32)
+        mesh.bf_nvert = np.full(faces.shape[0], 4, np.int32)
         mesh.bf_vert = faces.ravel()
-        mesh.bf_tag = np.concatenate([np.full(n, 1), np.full(n, 2)]).astype(np.int32)
+        mesh.bf_tag = np.concatenate([np.full(n, k + 1) for k in range(len(groups))]).astype(np.int32)
+    elif tags is not None:
+        raise ValueError(f"unknown tags {tags!r}")
     return mesh
 
 
@@ -272,6 +281,31 @@ def blast_state(centroids: np.ndarray, volumes: np.ndarray, h: float, seed: int 
     vdep = volumes[sel].sum()
     u[sel, 4] += energy / vdep
     return u
+
+
+def sedov_state(centroids: np.ndarray, volumes: np.ndarray, boundary_2d: bool) -> np.ndarray:
+    """init_sedov of init_sol (ns_euler_module.F90:64-111, one rank): rho = 1,
+    u = 0, E = 1e-16 / (gamma - 1) everywhere, and in the first cell of
+    smallest centroid norm (|(x, y)| with boundary_2d) E = 0.984042 / vol in
+    2-D, 0.851072 / vol in 3-D."""
+    xc = centroids[:, :2] if boundary_2d else centroids
+    r = np.sqrt((xc * xc).sum(1))
+    imin = int(np.argmin(r))  # first index of the minimum, as the strict < scan
+    u = np.zeros((centroids.shape[0], 5))
+    u[:, 0] = 1.0
+    u[:, 4] = 1e-16 / (GAMMA - 1.0)
+    u[imin, 4] = (0.984042 if boundary_2d else 0.851072) / volumes[imin]
+    return u
+
+
+def double_mach_state(centroids: np.ndarray) -> np.ndarray:
+    """init_double_mach of init_sol (ns_euler_module.F90:127-138): the post-shock
+    state (8, 7.145, -4.125, 0, 116.5) where y >= 1.732 (x - 0.1667), the
+    pre-shock state (1.4, 0, 0, 0, 1) below (primitive rho, u, v, w, p)."""
+    below = centroids[:, 1] < 1.732 * (centroids[:, 0] - 0.1667)
+    w = np.where(below[:, None], np.array([1.4, 0.0, 0.0, 0.0, 1.0])[None, :],
+                 np.array([8.0, 7.145, -4.125, 0.0, 116.5])[None, :])
+    return primit_to_conserv(w)
 
 
 def write_msh(mesh: Mesh, path: str, bc_names=()) -> None:

@@ -198,3 +198,84 @@ def test_fortran_driver_sutherland_viscous(tmp_path):
     assert np.abs(sol - ref).max() <= 1e-11 * np.abs(ref).max(), np.abs(sol - ref).max() / np.abs(ref).max()
     st, inv, _ = om.run(O.make_config("multi_point", **kw), init, 10, 0)
     assert np.abs(sol - inv).max() > 1e3 * np.abs(sol - ref).max()
+
+
+BLAST = """&INPUT_PARAM
+meshfile_path = "./"
+meshfile = "mesh.msh"
+n_bc = {n_bc}
+{bcs}
+scheme = "multi_point"
+second_order = {second}
+method = {method}
+bc_style = 2
+boundary_2d = {b2d}
+local_time_step = .false.
+{init} = .true.
+cfl = {cfl}
+tmax = 1e30
+final_res = -1.0
+n_max_iter = {iters}
+/
+"""
+
+
+def _driver_state(tmp_path):
+    raw = np.fromfile(tmp_path / "sol_final.bin", dtype=np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    return raw[4:].view(np.float64).reshape(n, 5)
+
+
+@pytest.mark.parametrize("kind,second", [("tet", ".false."), ("tet", ".true."), ("tri", ".true.")])
+def test_fortran_driver_init_sedov(tmp_path, kind, second):
+    """init_sedov (ns_euler_module.F90:64-111) through the namelist, the
+    reference's Sedov inputs (test/sedov_tri, test/sedov_tet: cfl 0.8,
+    method 0) on a tet mesh around the origin and a prism mesh of unit
+    depth; 20 steps, bit-exact with the oracle from the same state."""
+    print(f"[gpu] fortran driver init_sedov {kind} second={second}", flush=True)
+    if kind == "tet":
+        mesh = mg.tet_mesh(5)
+        mesh.coord[:] -= 0.5
+    else:
+        mesh = mg.tri_mesh(12, lx=2.4, ly=2.4, x0=-1.2, y0=-1.2, dz=1.0)
+    mg.write_msh(mesh, str(tmp_path / "mesh.msh"))
+    b2d = mesh.boundary_2d
+    (tmp_path / "input_data.f").write_text(BLAST.format(n_bc=0, bcs="", second=second, method=0,
+                                                        b2d=".true." if b2d else ".false.",
+                                                        init="init_sedov", cfl=0.8, iters=19))
+    out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    sol = _driver_state(tmp_path)
+    om = O.OracleMesh(mesh)
+    init = mg.sedov_state(om.elem_coord, om.getd("elem_volume"), b2d)
+    cfg = O.make_config("multi_point", bc_style=2, boundary_2d=b2d, second_order=second == ".true.", method=0,
+                        cfl=0.8, tmax=1e30)
+    st, ref, _ = om.run(cfg, init, 20, 0)
+    assert st == 0
+    assert np.array_equal(sol, ref), np.abs(sol - ref).max()
+
+
+def test_fortran_driver_double_mach(tmp_path):
+    """The reference's double Mach reflection inputs (test/double_mach_quad:
+    init_double_mach, inout_double_mach on the left and top, outflow on the
+    right, double_mach_bottom below) on a jittered quad mesh of [0, 3.2] x
+    [0, 1], second order; bit-exact with the oracle."""
+    print("[gpu] fortran driver double mach", flush=True)
+    mesh = mg.quad_mesh(16, lx=3.2, ly=1.0, tags="lrbt")
+    names = ("left", "right", "bottom", "top")
+    mg.write_msh(mesh, str(tmp_path / "mesh.msh"), names)
+    kinds = ("inout_double_mach", "outflowsupersonic", "double_mach_bottom", "inout_double_mach")
+    bcs = "\n".join([f'bc_name({i + 1}) = "{nm}"\nbc_type({i + 1}) = "{k}"' for i, (nm, k) in
+                     enumerate(zip(names, kinds))])
+    (tmp_path / "input_data.f").write_text(BLAST.format(n_bc=4, bcs=bcs, second=".true.", method=4, b2d=".true.",
+                                                        init="init_double_mach", cfl=0.4, iters=14))
+    out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    sol = _driver_state(tmp_path)
+    om = O.OracleMesh(mesh)
+    init = mg.double_mach_state(om.elem_coord)
+    cfg = O.make_config("multi_point", bc_style=2, boundary_2d=True, second_order=True, method=4, cfl=0.4, tmax=0.2,
+                        bcs=[(k, (0, 0, 0, 0, 0)) for k in kinds])
+    st, ref, _ = om.run(cfg, init, 15, 0)
+    assert st == 0
+    assert np.array_equal(sol, ref), np.abs(sol - ref).max()
