@@ -72,11 +72,10 @@ def test_blast_tet_two_stage_50_steps_bit_exact(solver, n, method):
     got, ref, t, info, om = _run_both(solver, mesh, sol0, 50, 0, second_order=True, method=method, cfl=0.4)
     assert np.array_equal(got, ref), np.abs(got - ref).max()
     assert t == info["t"]
+    # mass only: multi_point's wall nodes (bc_style 2) exchange energy with
+    # the walls, so total energy is not an invariant of the reference scheme
     m0, m1 = _mass(om, sol0), _mass(om, got)
     assert abs(m1 - m0) <= 1e-13 * m0, (m1 - m0) / m0
-    e0 = float((om.getd("elem_volume") * sol0[:, 4]).sum())
-    e1 = float((om.getd("elem_volume") * got[:, 4]).sum())
-    assert abs(e1 - e0) <= 1e-12 * e0, (e1 - e0) / e0
 
 
 @pytest.mark.parametrize("kind,o2", [("tet", False), ("tet", True), ("tri", True)])
