@@ -120,6 +120,8 @@ struct subfv_ctx {
   double *P = nullptr, *P2 = nullptr;
   double *rhs = nullptr, *rhs2 = nullptr, *sl = nullptr, *sl2 = nullptr;
   double *NG = nullptr, *VMM = nullptr, *G = nullptr, *vp = nullptr, *diag = nullptr;
+  double *NGP = nullptr;  // [V][9] cached nodal-gradient pseudo-inverses (geometry only)
+  bool ngp_ready = false;
   double *blk = nullptr;
   // viscous terms: per-vertex precomputed blocks (once per mesh with a
   // constant viscosity, every residual with Sutherland's law)
@@ -224,7 +226,8 @@ void free_mesh(subfv_ctx *c) {
   c->has_mesh = false;
   for (auto &u : c->U) u = nullptr;
   c->P = c->P2 = c->rhs = c->rhs2 = c->sl = c->sl2 = nullptr;
-  c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = nullptr;
+  c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = c->NGP = nullptr;
+  c->ngp_ready = false;
   c->doff = nullptr;
   c->pre = c->PD = c->rdiff = nullptr;
   c->mu_cell = c->mu_blk = nullptr;
@@ -558,8 +561,20 @@ int launch_grad(subfv_ctx *c, const double *U, int method) {
   const int nb = c->dm.ngroups;
   g.per_xcd = (nb + 7) / 8;
   g.nblocks = nb;
+  // the geometric pseudo-inverses are formed by the first gradient of a
+  // mesh and read back by every later one
+  if (!c->NGP) {
+    int r = dalloc(c, &c->NGP, 9 * (size_t)c->dm.V);
+    if (r) return r;
+  }
+  g.NGP = c->NGP;
   Prof pf(c, "grad");
-  hipLaunchKernelGGL(k_nodal_grad, dim3(8 * g.per_xcd), dim3(kGrpThreads), 0, c->stream, g);
+  if (c->ngp_ready) {
+    hipLaunchKernelGGL(k_nodal_grad<kNGCached>, dim3(8 * g.per_xcd), dim3(kGrpThreads), 0, c->stream, g);
+  } else {
+    hipLaunchKernelGGL(k_nodal_grad<kNGStore>, dim3(8 * g.per_xcd), dim3(kGrpThreads), 0, c->stream, g);
+    c->ngp_ready = true;
+  }
   g.cblocks = (int)nblk(c->dm.C, kCellThreads);
   g.cper_xcd = (g.cblocks + 7) / 8;
   hipLaunchKernelGGL(k_cell_grad, dim3(8 * g.cper_xcd), dim3(kCellThreads), 0, c->stream, g);

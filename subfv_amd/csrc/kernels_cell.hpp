@@ -291,6 +291,7 @@ struct GradArgs {
   double *NG;        // [V][15] nodal gradients, 3*k + d fastest
   double *VMM;       // [V][4] rho_min, rho_max, p_min, p_max over the vertex's cells
   double *G;         // [C][15]
+  double *NGP;       // [V][9] pinv of sum a n (x) (x_R - x_L): geometry only, cached per mesh
   int method;
   int per_xcd, nblocks;    // vertex groups (k_nodal_grad)
   int cper_xcd, cblocks;   // cell blocks (k_cell_grad)
@@ -310,6 +311,13 @@ struct GradLds {
 // non-periodic mesh; min and max do not depend on the order).  Sums run in
 // sub-face order (seg_sum); the 3x3 pseudo-inverses of the group run in the
 // first wavefront.
+// MODE: kNGFull forms and pseudo-inverts the geometric matrix of every
+// vertex; kNGStore does the same and keeps the pseudo-inverse in a.NGP;
+// kNGCached reads it from a.NGP instead (the matrix depends on the mesh only,
+// so the three give the same bits).
+enum { kNGFull = 0, kNGStore = 1, kNGCached = 2 };
+
+template <int MODE>
 __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   __shared__ GradLds L;
   const DevMesh &m = a.m;
@@ -334,21 +342,23 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   to_prim(ur, wr);
   const double n[3] = {m.sfn[s], m.sfn[SF + s], m.sfn[2 * SF + s]};
   const double ar = m.sfn[3 * SF + s];
-  // pass 1: sum a n (x) (x_R - x_L) and the bounds
-  for (int b = 0; b < 3; b++) {
-    double dx = inner ? m.ccoord[3L * re + b] - m.ccoord[3L * le + b] : 0.0;
-    for (int d = 0; d < 3; d++) L.red[d + 3 * b][tid] = inner ? ar * (n[d] * dx) : 0.0;
-  }
+  // pass 1: sum a n (x) (x_R - x_L) (unless cached) and the bounds
+  if (MODE != kNGCached)
+    for (int b = 0; b < 3; b++) {
+      double dx = inner ? m.ccoord[3L * re + b] - m.ccoord[3L * le + b] : 0.0;
+      for (int d = 0; d < 3; d++) L.red[d + 3 * b][tid] = inner ? ar * (n[d] * dx) : 0.0;
+    }
   // bounds: the left cell always, the right cell when it is a cell
   L.red[9][tid] = active ? dmin(wl[0], inner ? wr[0] : 1e100) : 1e100;
   L.red[10][tid] = active ? dmax(wl[0], inner ? wr[0] : 0.0) : 0.0;
   L.red[11][tid] = active ? dmin(wl[4], inner ? wr[4] : 1e100) : 1e100;
   L.red[12][tid] = active ? dmax(wl[4], inner ? wr[4] : 0.0) : 0.0;
   __syncthreads();
-  for (int p = tid; p < nv * 9; p += kGrpThreads) {
-    const int vv = p / 9, c = p - vv * 9;
-    L.sum[vv][15 + c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
-  }
+  if (MODE != kNGCached)
+    for (int p = tid; p < nv * 9; p += kGrpThreads) {
+      const int vv = p / 9, c = p - vv * 9;
+      L.sum[vv][15 + c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
+    }
   for (int p = tid; p < nv * 4; p += kGrpThreads) {
     const int vv = p >> 2, c = p & 3;
     double mm;
@@ -390,10 +400,16 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   __syncthreads();
   if (tid < nv) {  // first wavefront only (nv <= 64)
     const double *S = L.sum[tid];
-    double M[9];
-    for (int q = 0; q < 9; q++) M[q] = S[15 + q];
-    pinv3(M);
     const bool bnd = L.vbound[tid];
+    double M[9];
+    if (MODE == kNGCached) {
+      for (int q = 0; q < 9; q++) M[q] = bnd ? 0.0 : a.NGP[9L * (vb + tid) + q];
+    } else {
+      for (int q = 0; q < 9; q++) M[q] = S[15 + q];
+      pinv3(M);
+      if (MODE == kNGStore)
+        for (int q = 0; q < 9; q++) a.NGP[9L * (vb + tid) + q] = M[q];
+    }
     double *o = a.NG + 15L * (vb + tid);
     for (int k = 0; k < 5; k++)
       for (int d = 0; d < 3; d++) {
