@@ -452,6 +452,101 @@ struct DiffApplyLds {
   double pc[kDiffStage];       // the slot terms' block data, staged while the row pass streams
 };
 
+// (c) of k_diff_apply: the slot rows of one vertex from the row pass's Vf and
+// SVe.  STAGED: sb, blcs, snsh, sbh, b1 and the diagonal 3x3 blocks of
+// S~^T N^-1 S were staged in L.pc; otherwise they are read from the block.
+// The viscous work is spread over (slot, tensor entry) and (slot, sub-face)
+// lanes; every value is formed with the same operations in the same order
+// as one lane per slot would.
+template <bool STAGED>
+SFV_DEV void diff_slot_terms(const DiffArgs &a, DiffApplyLds &L, int lane, int ns, int se0, int E, const double *sns,
+                             const double *sb, const double *blcs, const double *snsh) {
+  const int o_tail = 12 * ns, o_diag = 12 * ns + ns * ns + 2 * ns;
+  const double *c_sb = STAGED ? L.pc : sb, *c_blcs = STAGED ? L.pc + 3 * ns : blcs;
+  const double *c_snsh = STAGED ? L.pc + o_tail : snsh;
+  const double *c_sbh = c_snsh + (long)ns * ns, *c_b1 = c_sbh + ns;
+  // diagonal block entry (i, j) of slot l
+  auto sdiag = [&](int l, int i, int j) -> double {
+    return STAGED ? L.pc[o_diag + 9 * l + i + 3 * j] : sns[(3 * l + i) + (long)E * (3 * l + j)];
+  };
+  double *PD = a.PD + (long)kDiffSlot * se0;
+  // diagonal block of the momentum rows, 9 entries per slot
+  for (int t = lane; t < 9 * ns; t += 64) {
+    const int l = t / 9, ij = t - 9 * l, i = ij % 3, j = ij / 3;
+    const double rho = L.rho[l];
+    PD[(long)kDiffSlot * l + 4 + ij] = -sdiag(l, i, j) / rho + c_blcs[9 * l + j + 3 * i] / rho;
+  }
+  // momentum rhs, 3 per slot
+  for (int t = lane; t < 3 * ns; t += 64) {
+    const int l = t / 3, i = t - 3 * l;
+    double mv = 0.0;
+    for (int j = 0; j < 3; j++) mv = mv + sdiag(l, i, j) * L.Ve[3 * l + j];
+    PD[(long)kDiffSlot * l + i] = (L.SVe[3 * l + i] - mv) + c_sb[3 * l + i];
+  }
+  // heat row of the diagonal, 5 per slot
+  for (int t = lane; t < 5 * ns; t += 64) {
+    const int l = t / 5, k = t - 5 * l;
+    const double sll = c_snsh[l + (long)ns * l];
+    PD[(long)kDiffSlot * l + 13 + k] = (c_b1[l] - sll) * L.P[l][k];
+  }
+  wsync_d();
+  // the viscous stress of each slot's sub-element, entry q = i + 3 j per
+  // lane: sum over its sub-faces of area (vm_i nl_j + vm_j nl_i - 2/3 dv
+  // delta_ij), times mu / vol.  The staged sb / blcs and diagonal blocks
+  // are dead now: spc takes the first, the work terms the second region
+  // (or the free staging area when nothing was staged).
+  double *spc = L.pc;
+  double *wt = L.pc + (STAGED ? o_diag : 9 * ns);
+  for (int t = lane; t < 9 * ns; t += 64) {
+    const int l = t / 9, q = t - 9 * l, i = q % 3, j = q / 3;
+    const int cell = L.cell[l], cnt = L.ssf[l][0];
+    double sp = 0.0;
+    for (int e = 0; e < cnt; e++) {
+      const int lo = L.ssf[l][1 + e];
+      const double sg = L.le[lo] == cell ? 1.0 : -1.0;
+      double nl[3], vm[3];
+      for (int d = 0; d < 3; d++) nl[d] = sg * L.nrm[lo][d];
+      for (int d = 0; d < 3; d++) vm[d] = L.Vf[3 * lo + d] - L.Ve[3 * l + d];
+      const double dv = vm[0] * nl[0] + vm[1] * nl[1] + vm[2] * nl[2];
+      const double ar = L.nrm[lo][3];
+      const double vmi = L.Vf[3 * lo + i] - L.Ve[3 * l + i], vmj = L.Vf[3 * lo + j] - L.Ve[3 * l + j];
+      const double nli = sg * L.nrm[lo][i], nlj = sg * L.nrm[lo][j];
+      sp = sp + ar * (vmi * nlj + vmj * nli - (2.0 / 3.0) * dv * (i == j ? 1.0 : 0.0));
+    }
+    const double mu = a.mu_cell ? a.mu_cell[cell] : a.p.mu;
+    spc[t] = sp * mu / L.vol[l];
+  }
+  wsync_d();
+  // area * (Spc Vf) . nl of each (slot, sub-face)
+  for (int t = lane; t < kSlotSrc * ns; t += 64) {
+    const int l = t / kSlotSrc, e = t - kSlotSrc * l;
+    if (e < L.ssf[l][0]) {
+      const int lo = L.ssf[l][1 + e];
+      const double sg = L.le[lo] == L.cell[l] ? 1.0 : -1.0;
+      const double *S = spc + 9 * l;
+      double nl[3];
+      for (int d = 0; d < 3; d++) nl[d] = sg * L.nrm[lo][d];
+      double sv[3];
+      for (int i = 0; i < 3; i++)
+        sv[i] = S[i] * L.Vf[3 * lo] + S[i + 3] * L.Vf[3 * lo + 1] + S[i + 6] * L.Vf[3 * lo + 2];
+      wt[t] = L.nrm[lo][3] * (sv[0] * nl[0] + sv[1] * nl[1] + sv[2] * nl[2]);
+    }
+  }
+  wsync_d();
+  // viscous work (its terms in sub-face order) and heat rhs, one lane per slot
+  if (lane < ns) {
+    const int l = lane, cnt = L.ssf[l][0];
+    double work = 0.0;
+    for (int e = 0; e < cnt; e++) work = work + wt[kSlotSrc * l + e];
+    double h = 0.0;
+    for (int p = 0; p < ns; p++) h = h + c_snsh[l + (long)ns * p] * L.th[p];
+    const double sll = c_snsh[l + (long)ns * l];
+    const double pu = L.pu[l];
+    h = h - sll * pu + (c_b1[l] - sll) * pu - c_b1[l] * L.th[l] + c_sbh[l];
+    PD[(long)kDiffSlot * l + 3] = work + h;
+  }
+}
+
 // One wavefront per vertex: the scatter terms of
 // build_momentum_diff_mat_into_global_mat_diag (:231-317) and
 // build_diff_mat_into_global_mat_diag (:139-182) per sub-element slot.
@@ -519,13 +614,6 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
       L.pc[o_diag + t] = sns[(3 * l + i) + (long)E * (3 * l + j)];
     }
   }
-  const double *c_sb = staged ? L.pc : sb, *c_blcs = staged ? L.pc + 3 * ns : blcs;
-  const double *c_snsh = staged ? L.pc + o_tail : snsh;
-  const double *c_sbh = c_snsh + (long)ns * ns, *c_b1 = c_sbh + ns;
-  // diagonal block entry (i, j) of slot l
-  auto sdiag = [&](int l, int i, int j) -> double {
-    return staged ? L.pc[o_diag + 9 * l + i + 3 * j] : sns[(3 * l + i) + (long)E * (3 * l + j)];
-  };
   wsync_d();
   // ---- (b)
   for (int r = lane; r < F + E; r += 64) {
@@ -541,62 +629,11 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
       L.SVe[r - F] = acc;
   }
   wsync_d();
-  // ---- (c) diagonal block of the momentum rows, 9 entries per slot
-  for (int t = lane; t < 9 * ns; t += 64) {
-    const int l = t / 9, ij = t - 9 * l, i = ij % 3, j = ij / 3;
-    const double rho = L.rho[l];
-    a.PD[(long)kDiffSlot * (se0 + l) + 4 + ij] =
-        -sdiag(l, i, j) / rho + c_blcs[9 * l + j + 3 * i] / rho;
-  }
-  // momentum rhs, 3 per slot
-  for (int t = lane; t < 3 * ns; t += 64) {
-    const int l = t / 3, i = t - 3 * l;
-    double mv = 0.0;
-    for (int j = 0; j < 3; j++) mv = mv + sdiag(l, i, j) * L.Ve[3 * l + j];
-    a.PD[(long)kDiffSlot * (se0 + l) + i] = (L.SVe[3 * l + i] - mv) + c_sb[3 * l + i];
-  }
-  // heat row of the diagonal, 5 per slot
-  for (int t = lane; t < 5 * ns; t += 64) {
-    const int l = t / 5, k = t - 5 * l;
-    const double sll = c_snsh[l + (long)ns * l];
-    a.PD[(long)kDiffSlot * (se0 + l) + 13 + k] = (c_b1[l] - sll) * L.P[l][k];
-  }
-  // viscous work and heat rhs, one lane per slot
-  if (lane < ns) {
-    const int l = lane, cell = L.cell[l], cnt = L.ssf[l][0];
-    double Spc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int e = 0; e < cnt; e++) {
-      const int lo = L.ssf[l][1 + e];
-      const double sg = L.le[lo] == cell ? 1.0 : -1.0;
-      double nl[3], vm[3];
-      for (int d = 0; d < 3; d++) nl[d] = sg * L.nrm[lo][d];
-      for (int d = 0; d < 3; d++) vm[d] = L.Vf[3 * lo + d] - L.Ve[3 * l + d];
-      const double dv = vm[0] * nl[0] + vm[1] * nl[1] + vm[2] * nl[2];
-      const double ar = L.nrm[lo][3];
-      for (int j = 0; j < 3; j++)
-        for (int i = 0; i < 3; i++)
-          Spc[i + 3 * j] = Spc[i + 3 * j] + ar * (vm[i] * nl[j] + vm[j] * nl[i] - (2.0 / 3.0) * dv * (i == j ? 1.0 : 0.0));
-    }
-    const double mu = a.mu_cell ? a.mu_cell[cell] : a.p.mu;
-    for (int q = 0; q < 9; q++) Spc[q] = Spc[q] * mu / L.vol[l];
-    double work = 0.0;
-    for (int e = 0; e < cnt; e++) {
-      const int lo = L.ssf[l][1 + e];
-      const double sg = L.le[lo] == cell ? 1.0 : -1.0;
-      double nl[3];
-      for (int d = 0; d < 3; d++) nl[d] = sg * L.nrm[lo][d];
-      double sv[3];
-      for (int i = 0; i < 3; i++)
-        sv[i] = Spc[i] * L.Vf[3 * lo] + Spc[i + 3] * L.Vf[3 * lo + 1] + Spc[i + 6] * L.Vf[3 * lo + 2];
-      work = work + L.nrm[lo][3] * (sv[0] * nl[0] + sv[1] * nl[1] + sv[2] * nl[2]);
-    }
-    double h = 0.0;
-    for (int p = 0; p < ns; p++) h = h + c_snsh[l + (long)ns * p] * L.th[p];
-    const double sll = c_snsh[l + (long)ns * l];
-    const double pu = L.pu[l];
-    h = h - sll * pu + (c_b1[l] - sll) * pu - c_b1[l] * L.th[l] + c_sbh[l];
-    a.PD[(long)kDiffSlot * (se0 + l) + 3] = work + h;
-  }
+  // ---- (c), with the slot data in LDS or read from the block in HBM
+  if (staged)
+    diff_slot_terms<true>(a, L, lane, ns, se0, E, sns, sb, blcs, snsh);
+  else
+    diff_slot_terms<false>(a, L, lane, ns, se0, E, sns, sb, blcs, snsh);
 }
 
 // rdiff and diag of each cell: its slots' rows in increasing vertex id

@@ -214,6 +214,27 @@ def tet_mesh(n: int, jitter: float = 0.1, seed: int = DEFAULT_SEED, length: floa
     return Mesh(coord, kind, (tets + 1).astype(np.int32).ravel(), boundary_2d=False, name=f"tet{n}")
 
 
+def hex_mesh(n: int, jitter: float = 0.1, seed: int = DEFAULT_SEED, length: float = 1.0) -> Mesh:
+    """n^3 jittered hexahedra on [0, length]^3 (gmsh hexahedra: the z = k
+    face counter-clockwise seen from above, then the z = k + 1 face)."""
+    h = length / n
+    m1 = n + 1
+    k, j, i = np.meshgrid(np.arange(m1), np.arange(m1), np.arange(m1), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    gid = (k * m1 * m1 + j * m1 + i).astype(np.uint64)
+    coord = np.stack([i * h, j * h, k * h], 1).astype(np.float64)
+    interior = (i > 0) & (i < n) & (j > 0) & (j < n) & (k > 0) & (k < n)
+    for ax in range(3):
+        dd = (2.0 * uniform(seed, 7 + ax, gid) - 1.0) * jitter * h
+        coord[:, ax] = np.where(interior, coord[:, ax] + dd, coord[:, ax])
+    ck, cj, ci = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    base = (ck * m1 * m1 + cj * m1 + ci).ravel()
+    a, b, c, d = base, base + 1, base + m1 + 1, base + m1
+    ev = np.stack([a, b, c, d, a + m1 * m1, b + m1 * m1, c + m1 * m1, d + m1 * m1], 1) + 1
+    kind = np.full(n ** 3, 5, np.int32)
+    return Mesh(coord, kind, ev.astype(np.int32).ravel(), boundary_2d=False, name=f"hex{n}")
+
+
 # ----------------------------------------------------------------- states
 def primit_to_conserv(w: np.ndarray) -> np.ndarray:
     """Primitive (rho,u,v,w,p) rows to conserved rows, in the reference's

@@ -111,7 +111,7 @@ def test_viscous_rk3_step_tri256_vortex(solver):
     assert st == 0
     rel = np.abs(got - ref).max() / np.abs(ref).max()
     print(f"  viscous rk3 step tri256: max rel diff {rel:.3e}", flush=True)
-    assert rel <= 1e-11, rel
+    assert rel <= 1e-12, rel
     assert abs(t - info["t"]) <= 1e-14 * info["t"]
 
 

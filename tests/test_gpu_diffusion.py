@@ -1,8 +1,10 @@
 """Viscous terms on the device (kernels_diff.hpp) against the oracle
 (om_diffusion.c).  The device forms the nodal products once per mesh and
 adds each cell's terms as one sum per vertex, where the reference adds them
-one by one; results agree to round-off (relative 1e-11 of the terms' scale,
-stated per test), not bit for bit."""
+one by one; results agree to round-off, not bit for bit: the residual within
+1e-12 of the viscous terms' scale (measured: 2e-14 on tri 8^2, 4e-17 for the
+point-implicit diagonal) and the state after 100 viscous steps within 1e-12
+relative (measured: 1e-15 on tri, quad, 3-D hex and a 10-sub-element fan)."""
 import numpy as np
 import pytest
 
@@ -65,10 +67,10 @@ def test_rhs_and_diag_match_oracle(solver, name, second):
     ref_rhs = r0 + rd
     scale_r = np.abs(rd).max(axis=0) + 1e-300
     scale_d = np.abs(dd).max()
-    # Euler part bit-exact; the viscous part to 1e-11 of its scale
-    assert np.all(np.abs(rhs - ref_rhs) <= 1e-11 * scale_r[None, :] + 1e-14 * np.abs(ref_rhs)), \
+    # Euler part bit-exact; the viscous part to 1e-12 of its scale
+    assert np.all(np.abs(rhs - ref_rhs) <= 1e-12 * scale_r[None, :] + 1e-14 * np.abs(ref_rhs)), \
         np.abs(rhs - ref_rhs).max(axis=0) / scale_r
-    assert np.abs(diag - dd).max() <= 1e-11 * scale_d
+    assert np.abs(diag - dd).max() <= 1e-12 * scale_d
     solver.set_diffusion(False)
 
 
@@ -91,8 +93,58 @@ def test_viscous_time_loop_matches_oracle(solver, scheme, second, ts):
                            sol, 10, ts)
     assert st == 0
     rel = np.abs(got - ref).max() / np.abs(ref).max()
-    assert rel <= 1e-11, rel
+    assert rel <= 1e-12, rel
     solver.set_diffusion(False)
+
+
+def fan_prism_mesh(k=5, layers=2):
+    """k prisms around a vertical axis, extruded in `layers` layers: the axis
+    vertices between layers have 2k sub-elements and 3k sub-faces (10 and
+    15 for k = 5), more than the viscous apply stages in LDS."""
+    ang = 2 * np.pi * np.arange(k) / k + 0.1
+    pts = np.concatenate([[[0.0, 0.0]], np.stack([np.cos(ang), np.sin(ang)], 1)])
+    coord = np.concatenate([np.column_stack([pts, np.full(len(pts), 0.4 * z)]) for z in range(layers + 1)])
+    ev = []
+    for lay in range(layers):
+        o, o2 = lay * len(pts), (lay + 1) * len(pts)
+        for q in range(k):
+            a, b, c = 0, 1 + q, 1 + (q + 1) % k
+            ev.append([o + a, o + b, o + c, o2 + a, o2 + b, o2 + c])
+    ev = np.array(ev) + 1
+    return mg.Mesh(coord, np.full(len(ev), 6, np.int32), ev.astype(np.int32).ravel(), boundary_2d=False, name="fan")
+
+
+LONG_CASES = {
+    "tri8": lambda: mg.tri_mesh(8),
+    "hex3": lambda: mg.hex_mesh(3),  # 3-D interior vertices: 12 sub-faces, 8 sub-elements
+    "fan5": fan_prism_mesh,          # 10 sub-elements: slot terms read from HBM
+}
+
+
+@pytest.mark.parametrize("ts", [0, 1])
+@pytest.mark.parametrize("name", list(LONG_CASES))
+def test_viscous_100_steps_within_1e12(solver, name, ts):
+    """100 second-order viscous steps (the reference's two-stage loop and
+    SSP-RK3): the conserved states stay within 1e-12 relative of the
+    oracle's, on a prism mesh, a 3-D hex mesh and a mesh whose vertices
+    exceed the LDS staging of the viscous apply."""
+    print(f"[gpu] viscous 100 steps {name} ts={ts}", flush=True)
+    mesh = LONG_CASES[name]()
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    sol = smooth_state(om.elem_coord, 9)
+    kw = dict(boundary_2d=mesh.boundary_2d, second_order=True, method=4, cfl=0.3)
+    solver.set_config(make_config("multi_point", **kw))
+    solver.set_diffusion(True, mu=5e-3)
+    solver.set_time(0.0)
+    solver.put_sol(sol)
+    solver.run(100, ts)
+    got = solver.get_sol()
+    solver.set_diffusion(False)
+    st, ref, info = om.run(O.make_config("multi_point", diffusion=True, mu=5e-3, **kw), sol, 100, ts)
+    assert st == 0
+    rel = np.abs(got - ref).max() / np.abs(ref).max()
+    assert rel <= 1e-12, rel
 
 
 def test_vertices_beyond_the_limit_are_rejected(solver):
@@ -122,7 +174,7 @@ def warm_state(xc, seed=3):
 def test_sutherland_rhs_and_diag_match_oracle(solver, name, second):
     """compute_mu with use_sutherland: mu per cell, mu0corr their minimum;
     T^1.5 goes through pow on both sides, so the per-cell viscosities may
-    differ in the last bit: the same 1e-11 bound as the constant case."""
+    differ in the last bit: the same 1e-12 bound as the constant case."""
     print(f"[gpu] sutherland rhs {name} second={second}", flush=True)
     mk, cfgkw, dkw = CASES[name]
     mesh = mk()
@@ -152,9 +204,9 @@ def test_sutherland_rhs_and_diag_match_oracle(solver, name, second):
     rd, dd = om.add_diffusion(ocfg, sol)
     ref_rhs = r0 + rd
     scale_r = np.abs(rd).max(axis=0) + 1e-300
-    assert np.all(np.abs(rhs - ref_rhs) <= 1e-11 * scale_r[None, :] + 1e-14 * np.abs(ref_rhs)), \
+    assert np.all(np.abs(rhs - ref_rhs) <= 1e-12 * scale_r[None, :] + 1e-14 * np.abs(ref_rhs)), \
         np.abs(rhs - ref_rhs).max(axis=0) / scale_r
-    assert np.abs(diag - dd).max() <= 1e-11 * np.abs(dd).max()
+    assert np.abs(diag - dd).max() <= 1e-12 * np.abs(dd).max()
 
 
 def test_sutherland_time_loop_matches_oracle(solver):
@@ -177,7 +229,7 @@ def test_sutherland_time_loop_matches_oracle(solver):
     st, ref, info = om.run(O.make_config("multi_point", second_order=True, method=4, cfl=0.3, diffusion=True,
                                          mu=1.0, sutherland=True), sol, 10, 0)
     assert st == 0
-    assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-11
+    assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-12
     # the viscous terms moved the state: the same run without them differs
     st, inv, _ = om.run(O.make_config("multi_point", second_order=True, method=4, cfl=0.3), sol, 10, 0)
     assert np.abs(got - inv).max() / np.abs(ref).max() > 1e-9

@@ -151,7 +151,7 @@ def test_fortran_driver_periodic_vortices(tmp_path, init):
     init_gresho) on a periodic quad mesh, and the isentropic vortex; the
     initial states come from the driver's Fortran and the test's numpy
     (log, exp and norm2 may differ in the last bit), so the final states are
-    held to 1e-11 of the state's largest magnitude."""
+    held to 1e-12 of the state's largest magnitude."""
     print(f"[gpu] fortran driver {init}", flush=True)
     mesh = mg.quad_mesh(16, x0=-0.5, y0=-0.5)
     mg.write_msh(mesh, str(tmp_path / "mesh.msh"))
@@ -168,13 +168,13 @@ def test_fortran_driver_periodic_vortices(tmp_path, init):
     assert st == 0
     # against the state's largest magnitude (the z momentum is round-off)
     scale = np.abs(ref).max()
-    assert np.abs(sol - ref).max() <= 1e-11 * scale, np.abs(sol - ref).max(axis=0) / scale
+    assert np.abs(sol - ref).max() <= 1e-12 * scale, np.abs(sol - ref).max(axis=0) / scale
 
 
 def test_fortran_driver_sutherland_viscous(tmp_path):
     """activate_diffusion with use_sutherland through the namelist: a
     Riemann problem between walls at air-like temperatures (347 K and
-    278 K), second order; the viscous terms agree with the oracle to 1e-11
+    278 K), second order; the viscous terms agree with the oracle to 1e-12
     of the state (the device sums each vertex's terms at once)."""
     print("[gpu] fortran driver sutherland", flush=True)
     mesh = mg.quad_mesh(12)
@@ -195,7 +195,7 @@ def test_fortran_driver_sutherland_viscous(tmp_path):
     kw = dict(bc_style=2, boundary_2d=True, second_order=True, method=4, cfl=0.4, tmax=1e30)
     st, ref, _ = om.run(O.make_config("multi_point", diffusion=True, sutherland=True, **kw), init, 10, 0)
     assert st == 0
-    assert np.abs(sol - ref).max() <= 1e-11 * np.abs(ref).max(), np.abs(sol - ref).max() / np.abs(ref).max()
+    assert np.abs(sol - ref).max() <= 1e-12 * np.abs(ref).max(), np.abs(sol - ref).max() / np.abs(ref).max()
     st, inv, _ = om.run(O.make_config("multi_point", **kw), init, 10, 0)
     assert np.abs(sol - inv).max() > 1e3 * np.abs(sol - ref).max()
 
