@@ -148,7 +148,9 @@ int subfv_state_put(subfv_ctx *ctx, const double *sol);
 int subfv_init_1drp(subfv_ctx *ctx, double x1drp, const double *w_left, const double *w_right);
 int subfv_state_get(subfv_ctx *ctx, double *sol);
 
-/* Device arrays in reference layout, by name: "sol", "rhs", "rhs_o2",
+/* Device arrays in reference layout, by name (scalars: "dt_solve", the dt of
+ * the last subfv_add_timedisc_and_solve solve before its positivity shrink;
+ * "local_time_step", 1 or 0 as configured): "sol", "rhs", "rhs_o2",
  * "sum_lambda", "sum_lambda_o2", "cell_grad", "nodal_grad", "vp", and with
  * the viscous terms "diag" (5, 5, n_elems) and "rhs_diff" (4, n_elems: the
  * viscous rhs of rho u, rho v, rho w, E). */
@@ -282,6 +284,10 @@ int subfv_bcsr_parallel_lusgs(subfv_bcsr *A, double *x, const double *rhs);
  * SUBFV_E_NOT_CONVERGED where the reference stops with "did not converge",
  * SUBFV_E_INVALID where lu_solve stops on a singular block. */
 int subfv_bcsr_pcgm(subfv_bcsr *A, double *x, const double *rhs, double tol, double *stats);
+/* New values v(b,b,nnzb) for the same pattern (the reference refills
+ * bcsr_matrix_type%v in place between solves): device copy, FP32 copy and
+ * block-Jacobi factors refreshed; no allocation. */
+int subfv_bcsr_set_values(subfv_bcsr *A, const double *v);
 /* add_block_to_bcsr (:56-71): v(:,:,k) += M (b x b column-major) for the first
  * stored block (i, j), 1-based; nothing if the block is not stored. */
 int subfv_bcsr_add_block(subfv_bcsr *A, int32_t i, int32_t j, const double *M);

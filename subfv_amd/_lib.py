@@ -134,6 +134,7 @@ SIGNATURES = {
     "subfv_bcsr_destroy": (None, [_P]),
     "subfv_bcsr_matmul": (C.c_int, [_P, _dp, _dp]),
     "subfv_bcsr_diag": (C.c_int, [_P, _dp, _dp]),
+    "subfv_bcsr_set_values": (C.c_int, [_P, _dp]),
     "subfv_bcsr_gmres_diag": (C.c_int, [_P, _dp, _dp, C.c_double, _i32, _i32, _ip, _dp]),
     "subfv_bcsr_gmres_diag_noalloc": (C.c_int, [_P, _dp, _dp, C.c_double, _i32, _i32, _ip, _dp]),
     "subfv_bcsr_gmres": (C.c_int, [_P, _dp, _dp, C.c_double, _i32, _i32, _i32, _ip, _dp]),

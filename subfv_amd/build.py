@@ -31,8 +31,8 @@ FBUILD = os.path.join(FORTRAN, "build")
 # the Fortran host: iso_c_binding module, the reference-named procedures and
 # the subfvns driver; FP contraction off as for the kernels
 F_FLAGS = ["-O2", "-ffp-contract=off"]
-F_MODULES = ["subfv_c.f90", "subfv_ns_gpu.f90"]
-F_PROGRAMS = ["subfvns_gpu.f90", "mesh_check.f90"]
+F_MODULES = ["subfv_c.f90", "subfv_ns_gpu.f90", "mesh_module_counts.f90", "subfv_reference_api.f90"]
+F_PROGRAMS = ["subfvns_gpu.f90", "mesh_check.f90", "reference_api_check.f90"]
 
 
 def find_flang():

@@ -1127,6 +1127,18 @@ int subfv_state_get(subfv_ctx *c, double *sol) {
 int64_t subfv_get_f64(subfv_ctx *c, const char *name, double *out, int64_t cap) {
   if (!c || !c->has_mesh || !name || !out) return -1;
   std::string n(name);
+  if (n == "dt_solve") {  // the dt of the last stage update's solve (before a positivity shrink)
+    if (cap < 1) return -1;
+    StepState hs;
+    if (sync_state_scalars(c, &hs)) return -1;
+    *out = hs.dt;
+    return 1;
+  }
+  if (n == "local_time_step") {  // 1.0 when the configuration steps in local time
+    if (cap < 1) return -1;
+    *out = c->cfg.local_time_step ? 1.0 : 0.0;
+    return 1;
+  }
   const long C = c->dm.C, V = c->dm.V;
   struct Item { const char *nm; double *p; int nc; long n; };
   Item items[] = {{"sol", c->U[c->cur], 1, 5 * C}, {"rhs", c->rhs, 5, C},      {"rhs_o2", c->rhs2, 5, C},
@@ -1522,6 +1534,7 @@ struct subfv_bcsr {
   int mr = 0;
   double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
   double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr, *part2 = nullptr;
+  double *gp1 = nullptr, *gp2 = nullptr, *gp3 = nullptr;  // CGS2 partials (kGsMax x kGsBlocks)
   unsigned int *ticket = nullptr;
   // host copies (0-based) for add_block and the sweep schedules
   std::vector<int> h_row, h_col, h_diag;
@@ -1759,6 +1772,9 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   if (!r) r = balloc(A, &A->z, 256);
   if (!r) r = balloc(A, &A->part, kRedBlocks);
   if (!r) r = balloc(A, &A->part2, kRedBlocks);
+  if (!r) r = balloc(A, &A->gp1, (size_t)kGsMax * kGsBlocks);
+  if (!r) r = balloc(A, &A->gp2, (size_t)kGsMax * kGsBlocks);
+  if (!r) r = balloc(A, &A->gp3, kGsBlocks);
   if (!r) r = balloc(A, &A->ticket, 1);
   if (!r) r = balloc(A, &A->perr, 1);
   if (r) {
@@ -1823,6 +1839,31 @@ int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs) {
 
 namespace {
 
+// CGS2 passes of one Arnoldi step against the k columns of V (k <= kGsMax):
+// hcol(1:k) = h1 + h2 and w orthogonalised; the partials of <w, w> are left
+// in A->gp3 for k_scale_norm.
+template <int KM>
+void gs_project_k(subfv_bcsr *A, int k, const double *V, long ldv, double *w, double *hcol) {
+  hipStream_t s = A->ctx->stream;
+  dim3 g(kGsBlocks), b(kVecThreads);
+  hipLaunchKernelGGL((k_gs_dot<KM>), g, b, 0, s, A->n, k, V, ldv, (const double *)w, A->gp1);
+  hipLaunchKernelGGL((k_gs_update<KM>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp1, hcol, 0, 1, A->gp2);
+  hipLaunchKernelGGL((k_gs_update<KM>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp2, hcol, 1, 0, A->gp3);
+}
+
+void gs_project(subfv_bcsr *A, int k, const double *V, long ldv, double *w, double *hcol) {
+  switch ((k + 3) / 4) {
+    case 1: gs_project_k<4>(A, k, V, ldv, w, hcol); break;
+    case 2: gs_project_k<8>(A, k, V, ldv, w, hcol); break;
+    case 3: gs_project_k<12>(A, k, V, ldv, w, hcol); break;
+    case 4: gs_project_k<16>(A, k, V, ldv, w, hcol); break;
+    case 5: gs_project_k<20>(A, k, V, ldv, w, hcol); break;
+    case 6: gs_project_k<24>(A, k, V, ldv, w, hcol); break;
+    case 7: gs_project_k<28>(A, k, V, ldv, w, hcol); break;
+    default: gs_project_k<kGsMax>(A, k, V, ldv, w, hcol);
+  }
+}
+
 // The restarted GMRES loops of bcsr_gmres (kind 0, :194-290, modifiedArnoldi
 // :319-340), bcsr_gmres_diag (1, :1592-1691, modifiedArnoldi_diag
 // :1568-1590), bcsr_gmres_lusgs (2, :447-548, modifiedArnoldi_lusgs :642-665)
@@ -1878,18 +1919,25 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
           if ((r = spmv(A, fp32, A->tmp, Vc(j)))) return r;
         }
         nspmv++;
-        // modified Gram-Schmidt against V(1..j-1): each step sums the
-        // previous step's dot (h(i-1, j-1)) itself, so partial buffers alternate
         double *w = Vc(j);
-        double *pa = A->part, *pb = A->part2;
-        hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)nullptr, (const double *)nullptr,
-                           (double *)nullptr, (const double *)Vc(1), pa);
-        for (int i = 2; i <= j; i++) {
-          hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)Vc(i - 1), (const double *)pa,
-                             Hd(i - 1, j - 1), (const double *)(i < j ? Vc(i) : nullptr), pb);
-          std::swap(pa, pb);
+        if (mr <= kGsMax) {
+          // classical Gram-Schmidt twice against V(1..j-1): h(1:j-1, j-1) =
+          // h1 + h2, then h(j, j-1) = ||w|| and V(:, j) = w / h(j, j-1)
+          gs_project(A, j - 1, Vb, n, w, Hd(1, j - 1));
+          hipLaunchKernelGGL(k_scale_norm, dim3(kGsBlocks), vb, 0, s, n, w, (const double *)A->gp3, Hd(j, j - 1));
+        } else {
+          // modified Gram-Schmidt against V(1..j-1): each step sums the
+          // previous step's dot (h(i-1, j-1)) itself, so partial buffers alternate
+          double *pa = A->part, *pb = A->part2;
+          hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)nullptr, (const double *)nullptr,
+                             (double *)nullptr, (const double *)Vc(1), pa);
+          for (int i = 2; i <= j; i++) {
+            hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)Vc(i - 1), (const double *)pa,
+                               Hd(i - 1, j - 1), (const double *)(i < j ? Vc(i) : nullptr), pb);
+            std::swap(pa, pb);
+          }
+          hipLaunchKernelGGL(k_scale_norm, vg, vb, 0, s, n, w, (const double *)pa, Hd(j, j - 1));
         }
-        hipLaunchKernelGGL(k_scale_norm, vg, vb, 0, s, n, w, (const double *)pa, Hd(j, j - 1));
       }
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipMemcpyAsync(H.data(), Hb, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2133,6 +2181,34 @@ int subfv_bcsr_pcgm(subfv_bcsr *A, double *x_io, const double *rhs, double tol, 
   }
   if (herr) return fail(c, SUBFV_E_INVALID, "PCG: a diagonal block is singular (lu_solve stop)");
   return conv ? 0 : fail(c, SUBFV_E_NOT_CONVERGED, "PCG did not converge");
+}
+
+// New block values v(b, b, nnzb) for the matrix's pattern: device copy, the
+// FP32 copy and the block-Jacobi factors refreshed (no allocation).
+int subfv_bcsr_set_values(subfv_bcsr *A, const double *v) {
+  if (!A || !v) return SUBFV_E_INVALID;
+  subfv_ctx *c = A->ctx;
+  hipStream_t s = c->stream;
+  const long nv = (long)A->nnzb * A->b * A->b;
+  HIPCHK(c, hipMemcpyAsync(A->v, v, nv * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_to_f32, dim3(nblk(nv, 256)), dim3(256), 0, s, nv, (const double *)A->v, A->v32);
+  dim3 g(nblk(A->m, 256)), bl(256);
+  switch (A->b) {
+    case 3:
+      hipLaunchKernelGGL((k_bcsr_factor<double, 3>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F64);
+      hipLaunchKernelGGL((k_bcsr_factor<float, 3>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F32);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_bcsr_factor<double, 4>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F64);
+      hipLaunchKernelGGL((k_bcsr_factor<float, 4>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F32);
+      break;
+    default:
+      hipLaunchKernelGGL((k_bcsr_factor<double, 5>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F64);
+      hipLaunchKernelGGL((k_bcsr_factor<float, 5>), g, bl, 0, s, A->m, A->diag_idx, A->v, A->F32);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(s));
+  return 0;
 }
 
 // add_block_to_bcsr (:56-71): v(:,:,k) += M for the first k of row i with

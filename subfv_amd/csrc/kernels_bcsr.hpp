@@ -236,6 +236,12 @@ __global__ void k_add_combination(long n, int k, const double *V, long ldv, cons
   }
 }
 
+// the FP32 copy of the block values (FP32-inner GMRES)
+__global__ void k_to_f32(long n, const double *v, float *v32) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    v32[i] = (float)v[i];
+}
+
 // v(:,:,k) += M (add_block_to_bcsr), FP64 values and their FP32 copy
 __global__ void k_add_block(int bb, long k, const double *M, double *v, float *v32) {
   const int q = threadIdx.x;
@@ -329,6 +335,116 @@ __global__ __launch_bounds__(kVecThreads) void k_mgs_chain(long n, double *w, co
     acc = acc + (v_next ? v_next[i] * x : x * x);
   }
   block_sum_store(acc, part_out);
+}
+
+// ------------------------------------------------ classical Gram-Schmidt, twice
+// The Arnoldi orthogonalisation of GMRES as CGS2: h1 = V^T w, w -= V h1,
+// h2 = V^T w, w -= V h2, h = h1 + h2, then ||w||.  Three passes over V per
+// step (the second fuses the first update with the second projection)
+// instead of one pass per basis vector.  Partial sums are per block (fixed
+// grid kGsBlocks, fixed reduction trees), so results are deterministic.
+constexpr int kGsBlocks = 512;
+constexpr int kGsMax = 32;  // basis vectors the CGS2 passes handle (restart <= 32)
+
+// per-block sums of acc[c], c < k, into part[c * gridDim.x + blockIdx.x]
+template <int KM>
+SFV_DEV void gs_block_store(const double *acc, int k, double *part) {
+  __shared__ double s[kVecThreads / 64][KM];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < KM; c++) {
+    if (c < k) {
+      double v = acc[c];
+      for (int o = 32; o > 0; o >>= 1) v = v + __shfl_down(v, o);
+      if (lane == 0) s[wv][c] = v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    double t = s[0][threadIdx.x];
+    for (int q = 1; q < kVecThreads / 64; q++) t = t + s[q][threadIdx.x];
+    part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// h[c] = sum over the blocks of part[c * nb + b], c < k (eight threads per
+// column, each a fixed run of blocks, then the eight runs in order)
+template <int KM>
+SFV_DEV void gs_coeffs(const double *part, int k, int nb, double *h) {
+  __shared__ double s2[KM][8];
+  const int c = threadIdx.x >> 3, r = threadIdx.x & 7;
+  if (c < k && c < KM) {
+    const int per = (nb + 7) / 8, b0 = r * per, b1 = min(nb, b0 + per);
+    double t = 0.0;
+    for (int b = b0; b < b1; b++) t = t + part[(long)c * nb + b];
+    s2[c][r] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    double t = s2[threadIdx.x][0];
+    for (int q = 1; q < 8; q++) t = t + s2[threadIdx.x][q];
+    h[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// partials of <V[:, c], w>, c < k, into part[c * gridDim.x + blockIdx.x]
+template <int KM>
+__global__ __launch_bounds__(kVecThreads) void k_gs_dot(long n, int k, const double *V, long ldv, const double *w,
+                                                        double *part) {
+  double acc[KM];
+#pragma unroll
+  for (int c = 0; c < KM; c++) acc[c] = 0.0;
+  // columns c >= k (KM - k < 4 of them) re-read column k - 1, so every load
+  // of an element is issued without a branch; their sums are not stored
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double vv[KM];
+#pragma unroll
+    for (int c = 0; c < KM; c++) vv[c] = V[(long)(c < k ? c : k - 1) * ldv + i];
+    const double wi = w[i];
+#pragma unroll
+    for (int c = 0; c < KM; c++) acc[c] = acc[c] + vv[c] * wi;
+  }
+  gs_block_store<KM>(acc, k, part);
+}
+
+// One update pass of CGS2.  h = the sums of part_in's k columns; w = w -
+// sum_c h[c] V[:, c]; block 0 stores h to hcol (or adds it, accumulate).
+// project: then partials of <V[:, c], w> into part_out (layout of k_gs_dot);
+// otherwise partials of <w, w>, one per block.
+template <int KM>
+__global__ __launch_bounds__(kVecThreads) void k_gs_update(long n, int k, const double *V, long ldv, double *w,
+                                                           const double *part_in, double *hcol, int accumulate,
+                                                           int project, double *part_out) {
+  __shared__ double h[KM];
+  gs_coeffs<KM>(part_in, k, (int)gridDim.x, h);
+  if (blockIdx.x == 0 && (int)threadIdx.x < k)
+    hcol[threadIdx.x] = accumulate ? hcol[threadIdx.x] + h[threadIdx.x] : h[threadIdx.x];
+  double acc[KM];
+#pragma unroll
+  for (int c = 0; c < KM; c++) acc[c] = 0.0;
+  double nrm = 0.0;
+  // as in k_gs_dot, columns c >= k re-read column k - 1 (branch-free loads)
+  // and are skipped in the update
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double vv[KM];
+#pragma unroll
+    for (int c = 0; c < KM; c++) vv[c] = V[(long)(c < k ? c : k - 1) * ldv + i];
+    double x = w[i];
+#pragma unroll
+    for (int c = 0; c < KM; c++) x = (c < k) ? x - h[c] * vv[c] : x;
+    w[i] = x;
+    if (project) {
+#pragma unroll
+      for (int c = 0; c < KM; c++) acc[c] = acc[c] + vv[c] * x;
+    } else {
+      nrm = nrm + x * x;
+    }
+  }
+  if (project)
+    gs_block_store<KM>(acc, k, part_out);
+  else
+    block_sum_store(nrm, part_out);
 }
 
 // w = w / sqrt(sum of part) (the norm of the last MGS step; block 0 stores

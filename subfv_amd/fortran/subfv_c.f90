@@ -230,6 +230,11 @@ module subfv_c
       import :: c_ptr
       type(c_ptr), value :: A
     end subroutine
+    integer(c_int) function subfv_bcsr_set_values(A, v) bind(C, name="subfv_bcsr_set_values")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: A
+      real(c_double), intent(in) :: v(*)
+    end function
     integer(c_int) function subfv_bcsr_matmul(A, x, y) bind(C, name="subfv_bcsr_matmul")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: A
