@@ -839,8 +839,12 @@ static void cell_grad_method(const om_mesh *m, const double *ang, int e, int met
     }
     if (method == 6) {
       for (int q = 0; q < 15; q++) cg[q] = cg[q] / cws;
+      /* OI(central_grad(:, :)) at :312,315: the 3 x 5 actual is sequence-
+         associated with OI's dimension(3) dummy, which sees the first
+         variable's gradient only; one weight serves all five variables */
+      const double den1 = eps + oi3(&cg[0]);
       for (int k = 0; k < 5; k++) {
-        double den = eps + oi3(&cg[3 * k]);
+        double den = den1;
         den = den * den;
         for (int d = 0; d < 3; d++) g[3 * k + d] = g[3 * k + d] + cws / den * cg[3 * k + d];
         ws[k] = ws[k] + cws / den;

@@ -481,8 +481,12 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
     }
     if (a.method == 6) {
       for (int q = 0; q < 15; q++) cg[q] = cg[q] / cws;
+      // the reference passes the whole 3 x 5 central gradient to OI, whose
+      // dummy is dimension(3): sequence association hands it the first
+      // variable's gradient, so one weight serves all five variables
+      const double den1 = eps + oi3(&cg[0]);
       for (int k = 0; k < 5; k++) {
-        double den = eps + oi3(&cg[3 * k]);
+        double den = den1;
         den = den * den;
         for (int d = 0; d < 3; d++) g[3 * k + d] = g[3 * k + d] + cws / den * cg[3 * k + d];
         ws[k] = ws[k] + cws / den;
