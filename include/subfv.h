@@ -112,7 +112,8 @@ int subfv_mesh_build(subfv_ctx *ctx, int32_t n_vert, const double *coord, int32_
  * (subfv_host_mesh_build): upload, device geometry and BC data only. */
 int subfv_mesh_upload(subfv_ctx *ctx, const subfv_host_mesh *hm, int32_t boundary_2d);
 /* "n_vert", "n_elems", "n_faces", "n_sub_faces", "n_sub_elems", "n_max_sub_faces", "n_groups",
- * "diff_block_doubles" (size of the packed viscous blocks, 0 before the first viscous residual) */
+ * "diff_block_doubles" (size of the packed viscous blocks, 0 before the first viscous residual),
+ * "device_bytes_used" (device memory the mesh and its work arrays hold in the context) */
 int64_t subfv_mesh_count(const subfv_ctx *ctx, const char *what);
 /* Connectivity in the reference numbering (same names as the mesh_type
  * components, e.g. "vert_sub_face", "sub_face_left_elem"); returns the number

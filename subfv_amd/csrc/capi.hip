@@ -121,6 +121,7 @@ struct subfv_ctx {
   double *rhs = nullptr, *rhs2 = nullptr, *sl = nullptr, *sl2 = nullptr;
   double *NG = nullptr, *VMM = nullptr, *G = nullptr, *vp = nullptr, *diag = nullptr;
   double *NGP = nullptr;  // [V][9] cached nodal-gradient pseudo-inverses (geometry only)
+  double *W = nullptr;    // [C][6] primitives and sound speed of the state a residual works on
   bool ngp_ready = false;
   double *blk = nullptr;
   // viscous terms: per-vertex precomputed blocks (once per mesh with a
@@ -226,7 +227,7 @@ void free_mesh(subfv_ctx *c) {
   c->has_mesh = false;
   for (auto &u : c->U) u = nullptr;
   c->P = c->P2 = c->rhs = c->rhs2 = c->sl = c->sl2 = nullptr;
-  c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = c->NGP = nullptr;
+  c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = c->NGP = c->W = nullptr;
   c->ngp_ready = false;
   c->doff = nullptr;
   c->pre = c->PD = c->rdiff = nullptr;
@@ -497,12 +498,26 @@ int finish_mesh(subfv_ctx *c, int32_t boundary_2d) {
 }
 
 // --------------------------------------------------------------- launches
-int launch_flux(subfv_ctx *c, const double *U, const double *G, double *P, bool want_vp) {
+// c->W = the primitives and sound speeds of U, for the sweeps and gradients
+// of one residual
+int launch_prim(subfv_ctx *c, const double *U) {
+  if (!c->W) {
+    int r = dalloc(c, &c->W, 6 * (size_t)c->dm.C);
+    if (r) return r;
+  }
+  hipLaunchKernelGGL(k_prim, dim3(nblk(c->dm.C, 256)), dim3(256), 0, c->stream, (long)c->dm.C, U, c->W);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// the node sweep on U; W: its primitives from launch_prim, or null
+int launch_flux(subfv_ctx *c, const double *U, const double *G, double *P, bool want_vp, const double *W = nullptr) {
   FluxArgs a;
   a.m = c->dm;
   a.bc = c->bc;
   a.U = U;
   a.G = G;
+  a.W = W;
   a.P = P;
   a.vp = want_vp ? c->vp : nullptr;
   a.err = c->derr;
@@ -550,10 +565,11 @@ int launch_gather(subfv_ctx *c, const double *P, const double *P2, double *rhs, 
   return 0;
 }
 
-int launch_grad(subfv_ctx *c, const double *U, int method) {
+int launch_grad(subfv_ctx *c, const double *U, int method, const double *W = nullptr) {
   GradArgs g;
   g.m = c->dm;
   g.U = U;
+  g.W = W;
   g.NG = c->NG;
   g.VMM = c->VMM;
   g.G = c->G;
@@ -640,19 +656,23 @@ int ensure_diffusion(subfv_ctx *c) {
   const size_t lds = diff_setup_lds_bytes(c->diff_fm, c->diff_em);
   if (lds > (64u << 10))
     HIPCHK(c, hipFuncSetAttribute((const void *)k_diff_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  size_t free_b = 0, total_b = 0;
-  HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
-  const size_t need = sizeof(double) * ((size_t)off[h.nv] + (size_t)kDiffSlot * h.nse + 29ull * h.ne);
-  if (need + ((size_t)1 << 30) > free_b)
-    return fail(c, SUBFV_E_DEVICE, "viscous terms: " + std::to_string(need >> 20) + " MiB needed, " +
-                                       std::to_string(free_b >> 20) + " MiB free");
   int r;
-  if ((r = upload(c, &c->doff, off))) return r;
-  if ((r = dalloc(c, &c->pre, (size_t)off[h.nv]))) return r;
-  c->pre_n = off[h.nv];
-  if ((r = dalloc(c, &c->PD, (size_t)kDiffSlot * h.nse))) return r;
-  if ((r = dalloc(c, &c->rdiff, 4ull * h.ne))) return r;
-  if (c->dparams.suth) {
+  // the buffers' sizes depend on the mesh only: allocated once per mesh and
+  // kept when the viscous parameters change (a new mesh resets them)
+  if (!c->doff) {
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+    const size_t need = sizeof(double) * ((size_t)off[h.nv] + (size_t)kDiffSlot * h.nse + 29ull * h.ne);
+    if (need + ((size_t)1 << 30) > free_b)
+      return fail(c, SUBFV_E_DEVICE, "viscous terms: " + std::to_string(need >> 20) + " MiB needed, " +
+                                         std::to_string(free_b >> 20) + " MiB free");
+    if ((r = upload(c, &c->doff, off))) return r;
+    if ((r = dalloc(c, &c->pre, (size_t)off[h.nv]))) return r;
+    c->pre_n = off[h.nv];
+    if ((r = dalloc(c, &c->PD, (size_t)kDiffSlot * h.nse))) return r;
+    if ((r = dalloc(c, &c->rdiff, 4ull * h.ne))) return r;
+  }
+  if (c->dparams.suth && !c->mu_cell) {
     if ((r = dalloc(c, &c->mu_cell, (size_t)h.ne))) return r;
     if ((r = dalloc(c, &c->mu_blk, (size_t)nblk(h.ne, kCellThreads) + 1))) return r;
   }
@@ -999,6 +1019,11 @@ int64_t subfv_mesh_count(const subfv_ctx *c, const char *what) {
   if (w == "n_max_sub_faces") return c->hmp->nmax_sfn;
   if (w == "n_groups") return c->dm.ngroups;
   if (w == "diff_block_doubles") return c->diff_ready ? c->pre_n : 0;
+  if (w == "device_bytes_used") {  // taken from the context's device chunks for this mesh
+    int64_t u = (int64_t)c->chunk_off;
+    for (size_t i = 0; i < c->chunk_i && i < c->chunks.size(); i++) u += (int64_t)c->chunks[i].second;
+    return u;
+  }
   return -1;
 }
 
@@ -1395,13 +1420,14 @@ int subfv_acoustic_run(subfv_ctx *c, double *sol, int32_t n_steps, double cfl, i
 static int residual(subfv_ctx *c, const double *U, int fixed_dt) {
   int r;
   if (c->diff_on && (r = launch_diffusion(c, U))) return r;
+  if ((r = launch_prim(c, U))) return r;
   if (c->cfg.second_order) {
-    if ((r = launch_grad(c, U, c->cfg.method))) return r;
-    if ((r = launch_flux(c, U, nullptr, c->P, false))) return r;
-    if ((r = launch_flux(c, U, c->G, c->P2, false))) return r;
+    if ((r = launch_grad(c, U, c->cfg.method, c->W))) return r;
+    if ((r = launch_flux(c, U, nullptr, c->P, false, c->W))) return r;
+    if ((r = launch_flux(c, U, c->G, c->P2, false, c->W))) return r;
     return launch_gather(c, c->P, c->P2, c->rhs, c->rhs2, c->sl, fixed_dt);
   }
-  if ((r = launch_flux(c, U, nullptr, c->P, false))) return r;
+  if ((r = launch_flux(c, U, nullptr, c->P, false, c->W))) return r;
   return launch_gather(c, c->P, nullptr, c->rhs, nullptr, c->sl, fixed_dt);
 }
 

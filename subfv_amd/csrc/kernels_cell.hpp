@@ -292,6 +292,7 @@ struct GradArgs {
   double *VMM;       // [V][4] rho_min, rho_max, p_min, p_max over the vertex's cells
   double *G;         // [C][15]
   double *NGP;       // [V][9] pinv of sum a n (x) (x_R - x_L): geometry only, cached per mesh
+  const double *W;   // [C][6] primitives of U (k_prim), or null
   int method;
   int per_xcd, nblocks;    // vertex groups (k_nodal_grad)
   int cper_xcd, cblocks;   // cell blocks (k_cell_grad)
@@ -333,13 +334,21 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
   const long C = m.C, SF = m.SF;
   const int le = m.sf_le[s], re = m.sf_re[s];
   const bool inner = active && re >= 0;
-  double ul[5], ur[5], wl[5], wr[5];
-  for (int k = 0; k < 5; k++) {
-    ul[k] = a.U[5L * le + k];
-    ur[k] = a.U[5L * (re >= 0 ? re : le) + k];
+  double wl[5], wr[5];
+  if (a.W) {
+    for (int k = 0; k < 5; k++) {
+      wl[k] = a.W[6L * le + k];
+      wr[k] = a.W[6L * (re >= 0 ? re : le) + k];
+    }
+  } else {
+    double ul[5], ur[5];
+    for (int k = 0; k < 5; k++) {
+      ul[k] = a.U[5L * le + k];
+      ur[k] = a.U[5L * (re >= 0 ? re : le) + k];
+    }
+    to_prim(ul, wl);
+    to_prim(ur, wr);
   }
-  to_prim(ul, wl);
-  to_prim(ur, wr);
   const double n[3] = {m.sfn[s], m.sfn[SF + s], m.sfn[2 * SF + s]};
   const double ar = m.sfn[3 * SF + s];
   // pass 1: sum a n (x) (x_R - x_L) (unless cached) and the bounds

@@ -28,6 +28,7 @@ struct FluxArgs {
   BCParams bc;
   const double *U;      // [C][5] conservative state, sol(5, n_elems)
   const double *G;      // [C][15] cell gradients, 3*k + d fastest (second order)
+  const double *W;      // [C][6] primitives and sound speed of U (k_prim), or null
   double *P;            // [SE][6] node-sorted slots
   double *vp;           // [3][V] nodal velocity output or null
   int *err;
@@ -161,10 +162,17 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
   // ---- left/right primitive states (reconstruct_lr_w)
   const long C = m.C;
   double wl[5], wr[5];
+  double al_c = 0.0, ar_c = 0.0;  // the cells' sound speeds from W (first order)
+  const double *W = a.W;
   {
     double u[5];
-    for (int k = 0; k < 5; k++) u[k] = U[5L * le + k];
-    to_prim(u, wl);
+    if (W) {
+      for (int k = 0; k < 5; k++) wl[k] = W[6L * le + k];
+      al_c = W[6L * le + 5];
+    } else {
+      for (int k = 0; k < 5; k++) u[k] = U[5L * le + k];
+      to_prim(u, wl);
+    }
     if (a.second_order) {
       double dx[3];
       const double *g = a.G + 15L * le;
@@ -172,8 +180,13 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
       for (int k = 0; k < 5; k++) wl[k] = wl[k] + (g[3 * k] * dx[0] + g[3 * k + 1] * dx[1] + g[3 * k + 2] * dx[2]);
     }
     if (re >= 0) {
-      for (int k = 0; k < 5; k++) u[k] = U[5L * re + k];
-      to_prim(u, wr);
+      if (W) {
+        for (int k = 0; k < 5; k++) wr[k] = W[6L * re + k];
+        ar_c = W[6L * re + 5];
+      } else {
+        for (int k = 0; k < 5; k++) u[k] = U[5L * re + k];
+        to_prim(u, wr);
+      }
       if (a.second_order) {
         double dx[3];
         const double *g = a.G + 15L * re;
@@ -198,10 +211,12 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
   double v_node[3] = {0.0, 0.0, 0.0};
   double p_nodal = 0.0;
   double warea = 1.0;
+  // the cached sound speeds hold for unreconstructed cell states only
+  const bool al_cached = W && !a.second_order, ar_cached = al_cached && re >= 0;
   if (SCHEME == kMultiPoint || SCHEME == kMultiPointIso) {
     const bool iso = (SCHEME == kMultiPointIso);
-    const double rhol = wl[0], pl = wl[4], al = sound_speed(wl);
-    const double rhor = wr[0], pr = wr[4], ar = sound_speed(wr);
+    const double rhol = wl[0], pl = wl[4], al = al_cached ? al_c : sound_speed(wl);
+    const double rhor = wr[0], pr = wr[4], ar = ar_cached ? ar_c : sound_speed(wr);
     const double vn_l = wl[1] * n[0] + wl[2] * n[1] + wl[3] * n[2];
     const double vn_r = wr[1] * n[0] + wr[2] * n[1] + wr[3] * n[2];
     for (int iter = 1; iter <= 4; iter++) {
@@ -276,8 +291,8 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
       warea = L.sum[vl][0] / area;
     }
   } else if (SCHEME == kMultiPointPressure || SCHEME == kMultiPointPressurePh) {
-    const double rhol = wl[0], pl = wl[4], al = sound_speed(wl);
-    const double rhor = wr[0], pr = wr[4], ar = sound_speed(wr);
+    const double rhol = wl[0], pl = wl[4], al = al_cached ? al_c : sound_speed(wl);
+    const double rhor = wr[0], pr = wr[4], ar = ar_cached ? ar_c : sound_speed(wr);
     const double vn_l = wl[1] * n[0] + wl[2] * n[1] + wl[3] * n[2];
     const double vn_r = wr[1] * n[0] + wr[2] * n[1] + wr[3] * n[2];
     const bool w1 = (bc.bc_style == 1 && wall);
@@ -367,6 +382,18 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
 #pragma unroll
     for (int c = 0; c < 6; c++) out[c] = acc[c];
   }
+}
+
+// W[6 c + k] = conserv_to_primit(U_c)(k), W[6 c + 5] = sound_speed_w of it:
+// the cell values every sub-face of the sweeps would otherwise recompute
+__global__ __launch_bounds__(256) void k_prim(long C, const double *__restrict__ U, double *__restrict__ W) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  double u[5], w[5];
+  for (int k = 0; k < 5; k++) u[k] = U[5 * c + k];
+  to_prim(u, w);
+  for (int k = 0; k < 5; k++) W[6 * c + k] = w[k];
+  W[6 * c + 5] = sound_speed(w);
 }
 
 template <int SCHEME>

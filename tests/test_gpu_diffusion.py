@@ -257,3 +257,34 @@ def test_sutherland_off_restores_constant_viscosity(solver):
     solver.set_diffusion(False)
     assert not np.array_equal(a[1], b[1])
     assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_viscous_parameter_changes_reuse_the_buffers(solver):
+    """Changing mu, toggling Sutherland's law and switching the viscous terms
+    off and on re-form the blocks in the buffers of the first viscous
+    residual: the context's device memory stays flat."""
+    print("[gpu] diffusion parameter sweep memory", flush=True)
+    mesh = mg.tri_mesh(12)
+    solver.build_mesh(mesh)
+    sol = warm_state(solver.elem_coord, 2)
+    solver.set_config(make_config("multi_point"))
+    solver.put_sol(sol)
+    solver.set_diffusion(True, mu=2e-5)
+    solver.compute_rhs_ns()
+    first = solver.get("rhs")
+    solver.set_sutherland(True)  # the per-cell viscosities join the buffers once
+    solver.compute_rhs_ns()
+    solver.set_sutherland(False)
+    used = solver.count("device_bytes_used")
+    for k in range(6):
+        solver.set_diffusion(True, mu=2e-5 * (1 + k))
+        solver.set_sutherland(k % 2 == 1)
+        solver.compute_rhs_ns()
+        solver.set_diffusion(False)
+        solver.compute_rhs_ns()
+    solver.set_sutherland(False)
+    solver.set_diffusion(True, mu=2e-5)
+    solver.compute_rhs_ns()
+    assert solver.count("device_bytes_used") == used
+    assert np.array_equal(solver.get("rhs"), first)
+    solver.set_diffusion(False)
