@@ -563,7 +563,9 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, DiffApplyLds &L, int lane, int n
 __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
   __shared__ DiffApplyLds lds_all[kDiffWaves];
   const DevMesh &m = a.m;
-  const int wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wavefront's index is uniform: the vertex's offsets come through the
+  // scalar cache
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int lb = xcd_block(a.vper_xcd);  // neighbouring vertices share cells
   if (lb >= a.vblocks) return;
   const long v = (long)lb * kDiffWaves + wib;
