@@ -75,3 +75,54 @@ def test_async_writer(tmp_path):
     wr.close()
     assert all(f.done() and f.exception() is None for f in futs)
     assert all((tmp_path / f"s{i}.dat").exists() for i in range(4)) and (tmp_path / "s.vtu").exists()
+
+
+def _oracle_snapshot(om, sol, method=4):
+    """reference_fields' inputs from the oracle (its numbering is build_mesh's)."""
+    g, ng = om.compute_cell_grad(sol, method)
+    snap = {"sol": sol, "elem_coord": om.elem_coord, "vert_coord": om.getd("vert_coord"), "cell_grad": g,
+            "nodal_grad": ng}
+    for nm in ("vert_n_sub_faces", "vert_sub_face", "sub_face_face", "face_left", "face_right", "sub_elem_elem",
+               "sub_elem_vert", "vert_is_bound"):
+        snap[nm] = om.geti(nm)
+    for nm in ("vert_volume", "sub_elem_volume", "sub_face_norm", "sub_face_area"):
+        snap[nm] = om.getd(nm)
+    return snap
+
+
+def test_reference_fields_closed_forms():
+    """Uniform flow: the nodal velocity is the flow velocity at interior
+    vertices and its wall-tangent part at wall vertices, the nodal pressure
+    the pressure, the Ducros sensor zero; a pure rotation u = (-y, x) has
+    zero divergence, so Ducros stays zero, while a pure expansion u = (x, y)
+    gives a positive sensor below one; derived cell fields follow their
+    definitions."""
+    import oracle as O
+
+    mesh = mg.tri_mesh(8)
+    om = O.OracleMesh(mesh)
+    xc = om.elem_coord
+    n = om.n_elems
+    w = np.tile([1.2, 0.3, -0.2, 0.0, 0.9], (n, 1))
+    f = OUT.reference_fields(_oracle_snapshot(om, mg.primit_to_conserv(w)))
+    pt = {k: v for k, v, _ in f["point"]}
+    ce = {k: v for k, v, _ in f["cell"]}
+    bound = om.geti("vert_is_bound") != 0
+    assert np.allclose(pt["Nodal_Velocity"][~bound], [0.3, -0.2, 0.0], atol=1e-14)
+    assert np.allclose(pt["Nodal_Pressure"], 0.9, atol=1e-14)
+    # on the square's sides the wall-normal part is removed
+    X = om.getd("vert_coord").reshape(-1, 3)
+    side = bound & (np.abs(X[:, 0]) < 1e-12) & (X[:, 1] > 0.05) & (X[:, 1] < 0.95)
+    assert side.any() and np.allclose(pt["Nodal_Velocity"][side, 0], 0.0, atol=1e-13)
+    assert np.all(ce["Ducros"] == 0.0)
+    assert np.allclose(ce["Mach"], np.sqrt(0.3 ** 2 + 0.2 ** 2) / np.sqrt(1.4 * 0.9 / 1.2))
+    assert np.allclose(ce["H"], (0.9 / 0.4 + 0.5 * 1.2 * 0.13) / 1.2 + 0.9 / 1.2)
+    for case, expect in (("rot", 0.0), ("exp", None)):
+        u = np.stack([-xc[:, 1], xc[:, 0]], 1) if case == "rot" else xc[:, :2].copy()
+        wv = np.column_stack([np.ones(n), 0.1 * u, np.zeros(n), np.ones(n)])
+        d = {k: v for k, v, _ in OUT.reference_fields(_oracle_snapshot(om, mg.primit_to_conserv(wv)))["cell"]}
+        if expect is not None:
+            assert np.abs(d["Ducros"]).max() < 1e-12
+        else:
+            inner = d["Ducros"] > 0
+            assert inner.sum() > n // 2 and d["Ducros"].max() < 1.0
