@@ -103,7 +103,8 @@ const char *subfv_error(const subfv_ctx *ctx);
 /* Mesh in msh 2.2 form: coord(3, n_vert) in file order, gmsh element kinds
  * (4 tet, 5 hex, 6 prism, 7 pyramid) with 1-based vertex ids, optional tagged
  * boundary faces (tag = BC index, 0 = default wall).  Builds the subface
- * connectivity, uploads it and computes the geometry on the device. */
+ * connectivity on the device (the reference's numbering), and computes the
+ * geometry there. */
 int subfv_mesh_build(subfv_ctx *ctx, int32_t n_vert, const double *coord, int32_t n_elems,
                      const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
                      const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag,
@@ -123,6 +124,12 @@ int64_t subfv_mesh_get_i32(subfv_ctx *ctx, const char *name, int32_t *out, int64
  * "sub_face_norm", "sub_face_area", "face_coord", ... */
 int64_t subfv_mesh_get_f64(subfv_ctx *ctx, const char *name, double *out, int64_t cap);
 
+/* The connectivity of subfv_mesh_build (build_mesh, mesh_connectivity_module.F90:12-59)
+ * built on the context's device, the same numbering, into a mesh handle that
+ * subfv_mesh_upload takes (served requests: build on arrival, then upload). */
+int subfv_host_mesh_build_on_device(subfv_host_mesh **out, subfv_ctx *ctx, int32_t n_vert, const double *coord,
+                                    int32_t n_elems, const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
+                                    const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag);
 /* Host-only connectivity build (no device needed). */
 int subfv_host_mesh_build(subfv_host_mesh **out, int32_t n_vert, const double *coord, int32_t n_elems,
                           const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,

@@ -103,6 +103,7 @@ SIGNATURES = {
     "subfv_mesh_get_i32": (_i64, [_P, C.c_char_p, _ip, _i64]),
     "subfv_mesh_get_f64": (_i64, [_P, C.c_char_p, _dp, _i64]),
     "subfv_host_mesh_build": (C.c_int, [_PP, _i32, _dp, _i32, _ip, _ip, _i32, _ip, _ip, _ip]),
+    "subfv_host_mesh_build_on_device": (C.c_int, [_PP, _P, _i32, _dp, _i32, _ip, _ip, _i32, _ip, _ip, _ip]),
     "subfv_host_mesh_make_periodic": (C.c_int, [_P]),
     "subfv_host_mesh_get_i32": (_i64, [_P, C.c_char_p, _ip, _i64]),
     "subfv_host_mesh_free": (None, [_P]),

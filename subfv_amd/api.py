@@ -24,9 +24,10 @@ class SubfvError(RuntimeError):
 
 
 class HostMesh:
-    """Connectivity of build_mesh computed on the host only (no device)."""
+    """Connectivity of build_mesh in a mesh handle: computed on the host, or
+    on the device of `solver` (subfv_host_mesh_build_on_device)."""
 
-    def __init__(self, mesh, periodic: bool = False):
+    def __init__(self, mesh, periodic: bool = False, solver: "Solver | None" = None):
         """periodic: build_mesh with periodic_mesh (faces on x, y, z = -0.5 / 0.5
         stitched to their partners, subfv_host_mesh_make_periodic)."""
         L = _lib.lib()
@@ -35,8 +36,15 @@ class HostMesh:
         kind = _lib.i32(mesh.kind)
         ev = _lib.i32(mesh.elem_vert)
         bn, bv, bt = _lib.i32(mesh.bf_nvert), _lib.i32(mesh.bf_vert), _lib.i32(mesh.bf_tag)
-        st = L.subfv_host_mesh_build(C.byref(h), mesh.n_vert, _lib.dptr(coord), mesh.n_elems, _lib.iptr(kind),
-                                     _lib.iptr(ev), int(bn.size), _lib.iptr(bn), _lib.iptr(bv), _lib.iptr(bt))
+        if solver is not None:
+            st = L.subfv_host_mesh_build_on_device(C.byref(h), solver.h, mesh.n_vert, _lib.dptr(coord),
+                                                   mesh.n_elems, _lib.iptr(kind), _lib.iptr(ev), int(bn.size),
+                                                   _lib.iptr(bn), _lib.iptr(bv), _lib.iptr(bt))
+            if st:
+                solver._check(st)
+        else:
+            st = L.subfv_host_mesh_build(C.byref(h), mesh.n_vert, _lib.dptr(coord), mesh.n_elems, _lib.iptr(kind),
+                                         _lib.iptr(ev), int(bn.size), _lib.iptr(bn), _lib.iptr(bv), _lib.iptr(bt))
         if st:
             raise SubfvError(st, "host mesh build failed")
         self.h = h

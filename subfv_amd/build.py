@@ -71,7 +71,7 @@ def build_fortran(verbose: bool = False):
 
 
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, "capi.hip"), os.path.join(CSRC, "mesh_build.cpp")]
+    srcs = [os.path.join(CSRC, "capi.hip"), os.path.join(CSRC, "mesh_device.hip"), os.path.join(CSRC, "mesh_build.cpp")]
     tmp = LIB + ".tmp"
     cmd = ["hipcc", *HIP_FLAGS, "-o", tmp, *srcs]
     if verbose:

@@ -991,12 +991,33 @@ int subfv_mesh_build(subfv_ctx *c, int32_t n_vert, const double *coord, int32_t 
   HIPCHK(c, hipSetDevice(c->device));
   free_mesh(c);
   auto hm = std::make_shared<subfv::HostMesh>();
-  std::string e = subfv::build_host_mesh(*hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
-                                         bf_vert, bf_tag);
+  // build_mesh's connectivity on the device (mesh_device.hip)
+  std::string e = subfv::build_device_mesh(*hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
+                                           bf_vert, bf_tag, (void *)c->stream);
   if (!e.empty()) return fail(c, SUBFV_E_INVALID, e);
   c->tab = make_tables(*hm);
   c->hmp = hm;
   return finish_mesh(c, boundary_2d);
+}
+
+int subfv_host_mesh_build_on_device(subfv_host_mesh **out, subfv_ctx *c, int32_t n_vert, const double *coord,
+                                    int32_t n_elems, const int32_t *kind, const int32_t *elem_vert, int32_t n_bfaces,
+                                    const int32_t *bf_nvert, const int32_t *bf_vert, const int32_t *bf_tag) {
+  if (!out || !c || !coord || !kind || !elem_vert) return fail(c, SUBFV_E_INVALID, "null argument");
+  if (n_bfaces > 0 && (!bf_nvert || !bf_vert || !bf_tag)) return fail(c, SUBFV_E_INVALID, "null boundary array");
+  *out = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto *m = new subfv_host_mesh();
+  m->hm = std::make_shared<subfv::HostMesh>();
+  std::string e = subfv::build_device_mesh(*m->hm, n_vert, coord, n_elems, kind, elem_vert, n_bfaces, bf_nvert,
+                                           bf_vert, bf_tag, (void *)c->stream);
+  if (!e.empty()) {
+    delete m;
+    return fail(c, SUBFV_E_INVALID, e);
+  }
+  m->tab = make_tables(*m->hm);
+  *out = m;
+  return 0;
 }
 
 int subfv_mesh_upload(subfv_ctx *c, const subfv_host_mesh *hm, int32_t boundary_2d) {

@@ -279,6 +279,8 @@ std::string build_host_mesh(HostMesh &m, int nv, const double *coord, int ne, co
       }
     m.f_vert.swap(fv);
   }
+  for (int q = 0; q < m.e_foff[ne]; q++)
+    if (m.e_face[q] < 0) return "face shared by more than two elements";
   // ---- build_node_face_connectivity (:299-333)
   cnt.assign(nv, 0);
   for (int v : m.f_vert) cnt[v]++;

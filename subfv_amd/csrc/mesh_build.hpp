@@ -50,6 +50,14 @@ std::string build_host_mesh(HostMesh &m, int nv, const double *coord, int ne, co
                             const int32_t *elem_vert, int nbf, const int32_t *bf_nvert,
                             const int32_t *bf_vert, const int32_t *bf_tag);
 
+// The same connectivity built on the device (mesh_device.hip): identical
+// numbering, computed by data-parallel passes, stable sorts and a
+// level-synchronous breadth-first search on `stream` (a hipStream_t), then
+// copied into m.
+std::string build_device_mesh(HostMesh &m, int nv, const double *coord, int ne, const int32_t *kind,
+                              const int32_t *elem_vert, int nbf, const int32_t *bf_nvert, const int32_t *bf_vert,
+                              const int32_t *bf_tag, void *stream);
+
 // The periodic part of build_mesh (src/mesh/mesh_connectivity_module.F90:44-54)
 // on a built connectivity: faces on x, y, z = -0.5 and 0.5 are stitched to
 // their partners, and the vertices there take their partners' sub-faces and

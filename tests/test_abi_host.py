@@ -71,3 +71,14 @@ def test_host_mesh_rejects_bad_input():
     bad = mg.Mesh(mesh.coord, mesh.kind, ev, mesh.bf_nvert, mesh.bf_vert, mesh.bf_tag, mesh.boundary_2d, "bad")
     with pytest.raises(Exception):
         HostMesh(bad)
+
+
+def test_host_mesh_rejects_face_of_three_elements():
+    """Three hexahedra on one quadrilateral (a non-manifold face) would leave
+    an element's face unset: the build refuses the mesh."""
+    coord = np.random.default_rng(0).random((16, 3))
+    base = np.arange(8)
+    ev = np.concatenate([base, [4, 5, 6, 7, 8, 9, 10, 11], [4, 5, 6, 7, 12, 13, 14, 15]]) + 1
+    bad = mg.Mesh(coord, np.full(3, 5, np.int32), ev.astype(np.int32), name="three")
+    with pytest.raises(Exception):
+        HostMesh(bad)
