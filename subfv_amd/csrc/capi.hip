@@ -130,6 +130,8 @@ struct subfv_ctx {
   DiffParams dparams{};
   long *doff = nullptr;
   double *pre = nullptr, *PD = nullptr, *rdiff = nullptr;
+  uint32_t *dssf = nullptr;  // [SE] node-sorted slot -> its sub-element's vertex-local sub-faces, packed
+  double *dsvol = nullptr;   // [SE] node-sorted slot -> sub-element volume
   double *mu_cell = nullptr, *mu_blk = nullptr;  // Sutherland: [C] viscosities; block minima + mu0corr
   int diff_fm = 0, diff_em = 0;  // setup LDS extents: 3 * max sub-faces, 3 * max sub-elements of a vertex
   long pre_n = 0;  // doubles in the packed viscous blocks
@@ -230,7 +232,8 @@ void free_mesh(subfv_ctx *c) {
   c->NG = c->VMM = c->G = c->vp = c->diag = c->blk = c->NGP = c->W = nullptr;
   c->ngp_ready = false;
   c->doff = nullptr;
-  c->pre = c->PD = c->rdiff = nullptr;
+  c->pre = c->PD = c->rdiff = c->dsvol = nullptr;
+  c->dssf = nullptr;
   c->mu_cell = c->mu_blk = nullptr;
   c->diff_ready = false;
 }
@@ -671,6 +674,26 @@ int ensure_diffusion(subfv_ctx *c) {
     c->pre_n = off[h.nv];
     if ((r = dalloc(c, &c->PD, (size_t)kDiffSlot * h.nse))) return r;
     if ((r = dalloc(c, &c->rdiff, 4ull * h.ne))) return r;
+    // each slot's sub-element sub-faces as vertex-local indices, one byte
+    // each in list order (0xFF ends the list), and its volume: the apply
+    // reads them at the slot's index instead of through the cell-major lists
+    std::vector<uint32_t> ssf(h.nse, 0xFFFFFFFFu);
+    for (int v = 0; v < h.nv; v++)
+      for (int q = h.v_seoff[v]; q < h.v_seoff[v + 1]; q++) {
+        const int se = h.v_se[q], pb = h.se_sfoff[se], pe = h.se_sfoff[se + 1];
+        if (pe - pb > kSlotSrc)
+          return fail(c, SUBFV_E_INVALID, "viscous terms: a sub-element has more than " +
+                                              std::to_string(kSlotSrc) + " sub-faces");
+        for (int pp = pb; pp < pe; pp++) {
+          const uint32_t loc = (uint32_t)(h.se_sf[pp] - h.sf_off[v]);
+          ssf[q] &= ~(0xFFu << (8 * (pp - pb)));
+          ssf[q] |= loc << (8 * (pp - pb));
+        }
+      }
+    if ((r = upload(c, &c->dssf, ssf))) return r;
+    if ((r = dalloc(c, &c->dsvol, (size_t)h.nse))) return r;
+    hipLaunchKernelGGL(k_slot_vol, dim3(nblk(h.nse, 256)), dim3(256), 0, c->stream, (long)h.nse,
+                       (const int *)c->dm.v_seid, (const double *)c->dm.se_vol, c->dsvol);
   }
   if (c->dparams.suth && !c->mu_cell) {
     if ((r = dalloc(c, &c->mu_cell, (size_t)h.ne))) return r;
@@ -733,6 +756,8 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
   a.PD = c->PD;
   a.rdiff = c->rdiff;
   a.diag = c->diag;
+  a.slot_ssf = c->dssf;
+  a.slot_vol = c->dsvol;
   Prof pf(c, "diffusion");
   if (c->dparams.suth) {
     Prof ps(c, "diff_setup");

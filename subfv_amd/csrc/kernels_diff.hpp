@@ -81,7 +81,15 @@ struct DiffArgs {
   const double *mu_min;   // mu0corr, their minimum (Sutherland)
   int vper_xcd, vblocks;  // XCD slices of the vertex blocks (apply)
   int cper_xcd, cblocks;  // and of the cell blocks (gather)
+  const uint32_t *slot_ssf;  // [SE] node-sorted slot: its sub-element's vertex-local sub-faces, a byte each, 0xFF ends
+  const double *slot_vol;    // [SE] node-sorted slot: its sub-element's volume
 };
+
+// slot_vol of every node-sorted slot from the cell-major sub-element volumes
+__global__ __launch_bounds__(256) void k_slot_vol(long n, const int *v_seid, const double *se_vol, double *out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = se_vol[v_seid[q]];
+}
 
 SFV_DEV void wsync_d() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -582,13 +590,18 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
   if (lane < ns) {
     const int l = lane;
     const int c = m.se_cell[se0 + l];
-    const long se = m.v_seid[se0 + l];
+    const uint32_t code = a.slot_ssf[se0 + l];
+    L.vol[l] = a.slot_vol[se0 + l];
     double u[5];
     for (int k = 0; k < 5; k++) u[k] = a.U[5L * c + k];
-    const int pb = m.se_sfoff[se], pe = m.se_sfoff[se + 1];
-    L.ssf[l][0] = pe - pb;  // at most kSlotSrc (checked when the groups are built)
-    for (int pp = pb; pp < pe; pp++) L.ssf[l][1 + pp - pb] = m.se_sf[pp] - sf0;
-    L.vol[l] = m.se_vol[se];
+    int cnt = 0;
+    for (int e = 0; e < kSlotSrc; e++) {
+      const int j = (code >> (8 * e)) & 0xFFu;
+      if (j == 0xFF) break;
+      L.ssf[l][1 + e] = j;
+      cnt++;
+    }
+    L.ssf[l][0] = cnt;
     for (int k = 0; k < 3; k++) L.Ve[3 * l + k] = u[k + 1] / u[0];
     L.rho[l] = u[0];
     L.cell[l] = c;
