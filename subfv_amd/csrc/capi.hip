@@ -497,6 +497,13 @@ int finish_mesh(subfv_ctx *c, int32_t boundary_2d) {
   r = apply_bc(c);
   if (r) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the copies are done: staging blocks beyond the first are returned, so
+  // pinned host memory held by idle contexts stays at one block each (a
+  // server keeps hundreds of contexts)
+  while (c->pins.size() > 1) {
+    (void)hipHostFree(c->pins.back().first);
+    c->pins.pop_back();
+  }
   return 0;
 }
 

@@ -47,6 +47,11 @@ __constant__ int8_t c_faces[4][6][4] = {
 
 __device__ inline int d_nface(int kind) { return kind == 4 ? 4 : (kind == 5 ? 6 : 5); }
 
+// the stream of the build running on this thread: buffers are allocated and
+// freed in its order (hipMallocAsync / hipFreeAsync), so a build never waits
+// for work of other streams on the device, e.g. solves running beside it
+thread_local hipStream_t t_stream = nullptr;
+
 // device buffer owned for the duration of one build
 template <class T>
 struct Buf {
@@ -56,13 +61,13 @@ struct Buf {
   Buf(const Buf &) = delete;
   Buf &operator=(const Buf &) = delete;
   ~Buf() {
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, t_stream);
   }
   hipError_t alloc(size_t count) {
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, t_stream);
     p = nullptr;
     n = count;
-    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+    return hipMallocAsync((void **)&p, std::max<size_t>(count, 1) * sizeof(T), t_stream);
   }
   operator T *() const { return p; }
 };
@@ -533,6 +538,7 @@ std::string build_device_mesh(HostMesh &m, int nv, const double *coord, int ne, 
   }
   const int nq = h_evoff[ne], nmf = h_mfoff[ne];
 
+  t_stream = (hipStream_t)stream;
   Ctx c;
   c.s = (hipStream_t)stream;
   auto fail = [&](const char *where) { return std::string("device mesh build (") + where + "): " + hipGetErrorString(c.err); };

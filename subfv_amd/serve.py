@@ -1,19 +1,26 @@
 """Served mixed-size explicit solves on one GPU.
 
-A request carries a mesh (quad or prism grid of a seeded size, connectivity
-already built on the client with HostMesh, as a mesh file would be read),
-a seeded two-state initial condition (init_1drp, ns_euler_module.F90:36-43)
-and a step count.  The server admits arriving requests while the HBM budget
-and the batch capacity allow, keeps every admitted state resident, and
-advances all of them together with the multi-instance launches of
-subfv_batch_run (continuous batching: requests that arrive while others run
-are admitted between chunks of steps, and finished ones leave).  Each
-instance's arithmetic is that of a solve alone, so results are bit-identical
-to subfv_run on the same request.
+A request carries a raw mesh (quad or prism grid of a seeded size, as a
+msh file is read: vertices, elements, boundary faces), a seeded two-state
+initial condition (init_1drp, ns_euler_module.F90:36-43) and a step count.
+The server builds the connectivity after the request arrives (build_mesh,
+inside the request's latency: on the device, or on the host), admits
+arriving requests while the HBM budget, the batch capacity and the cap on
+cells in flight allow, keeps every admitted state resident, and advances
+all of them together with the multi-instance launches of subfv_batch_run
+(continuous batching: requests that arrive while others run are admitted
+between chunks of steps, in arrival order, and finished ones leave).  Each
+instance's arithmetic is that of a solve alone, so results are
+bit-identical to subfv_run on the same request.
+
+The cap on cells in flight keeps the batch at the size that fills the GPU:
+beyond it, more instances share the same throughput and every one of them
+finishes later.
 
 `serve_plain` is the baseline: the same requests one at a time, in arrival
-order, each with its own launches (subfv_run).  `autotune` picks the chunk
-length, the batch capacity and graph replay by timing candidates on a
+order, each with its own connectivity build and launches (subfv_run).
+`autotune` picks the chunk length, the batch capacity, graph replay, the
+cells cap and where connectivity is built by timing candidates on a
 calibration sample.
 """
 from __future__ import annotations
@@ -44,7 +51,8 @@ class Request:
     wr: np.ndarray      # primitive right state
     x_split: float
     steps: int
-    host: HostMesh | None = None
+    host: HostMesh | None = None   # connectivity built beforehand (else built on admission)
+    mesh: mg.Mesh | None = None    # the raw mesh
     boundary_2d: bool = True
     cells: int = 0
     mesh_path: str | None = None  # a msh 2.2 file instead of the generated mesh
@@ -72,9 +80,10 @@ def make_requests(n_req: int = 4096, rate: float = 200.0, steps: int = 300, seed
     return out
 
 
-def prepare(reqs: list[Request], workers: int = 16) -> list[Request]:
-    """Client side: generate each mesh (or read its msh 2.2 file) and build
-    its connectivity (threads; the C-ABI call releases the interpreter lock)."""
+def prepare(reqs: list[Request], workers: int = 16, connectivity: bool = False) -> list[Request]:
+    """Client side: generate each mesh (or read its msh 2.2 file); with
+    `connectivity`, also build it here on the host (threads; the C-ABI call
+    releases the interpreter lock), outside the server."""
     def one(r: Request):
         if r.mesh_path:
             mesh = mg.read_msh(r.mesh_path, r.bc_names, r.boundary_2d)
@@ -82,7 +91,9 @@ def prepare(reqs: list[Request], workers: int = 16) -> list[Request]:
             mesh = mg.quad_mesh(r.n, seed=SEED + r.rid)
         else:
             mesh = mg.tri_mesh(r.n, seed=SEED + r.rid)
-        r.host = HostMesh(mesh)
+        r.mesh = mesh
+        if connectivity:
+            r.host = HostMesh(mesh)
         r.boundary_2d = mesh.boundary_2d
         r.cells = mesh.n_elems
         return r
@@ -95,9 +106,18 @@ def config_for(r: Request):
                        second_order=False, cfl=0.4, tmax=1e30)
 
 
-def admit(s: Solver, r: Request):
-    """Upload the mesh, set the configuration and the init_1drp state."""
-    s.upload_mesh(r.host, r.boundary_2d)
+def admit(s: Solver, r: Request, connectivity: str = "device"):
+    """Build the connectivity (on the device of `s`, or on the host) unless
+    the request brings it, upload the mesh, set the configuration and the
+    init_1drp state."""
+    if r.host is not None:
+        s.upload_mesh(r.host, r.boundary_2d)
+    elif connectivity == "device":
+        s.build_mesh(r.mesh)
+    elif connectivity == "host":
+        s.upload_mesh(HostMesh(r.mesh), r.boundary_2d)
+    else:
+        raise ValueError(f"connectivity {connectivity!r}: 'device' or 'host'")
     s.set_config(config_for(r))
     s.init_1drp(r.x_split, r.wl, r.wr)
     s.set_time(0.0)
@@ -148,8 +168,10 @@ class Params:
     use_graph: bool = True
     budget: int = 96 << 30   # HBM bytes for admitted instances
     admit_workers: int = 8   # host threads uploading admitted requests
-    prefetch: int = 256      # instances uploaded ahead of free batch capacity
+    prefetch: int = 64       # instances uploaded ahead of free batch capacity
     first: int = 64          # uploads to wait for before the first chunk of an idle server
+    max_cells: int = 1 << 40  # cells of the instances in flight (one instance always fits)
+    connectivity: str = "device"  # where admission builds a request's connectivity
 
 
 def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep: bool = False,
@@ -191,7 +213,7 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
             used += device_bytes(r)
             # each upload works on its own context and stream; the C-ABI calls
             # release the interpreter lock, so uploads overlap the batch
-            uploading[ex.submit(admit, s, r)] = [r, s, r.steps]
+            uploading[ex.submit(admit, s, r, params.connectivity)] = [r, s, r.steps]
             n += 1
 
     def collect(wait_for=0):
@@ -213,9 +235,17 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
     try:
         while pi < len(pending) or active or staged or uploading:
             collect()
-            room = params.max_batch - len(active)
-            active += staged[:room]
-            del staged[:max(room, 0)]
+            # staged instances join in arrival order while the batch capacity
+            # and the cells cap allow
+            staged.sort(key=lambda e: e[0].arrival)
+            cells = sum(a[0].cells for a in active)
+            k = 0
+            while (k < len(staged) and len(active) < params.max_batch
+                   and (not active or cells + staged[k][0].cells <= params.max_cells)):
+                cells += staged[k][0].cells
+                active.append(staged[k])
+                k += 1
+            del staged[:k]
             if not active:
                 start_uploads(params.max_batch + params.prefetch - len(uploading))
                 if uploading:
@@ -262,9 +292,11 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
 
 
 def serve_plain(reqs: list[Request], device: int = 0, keep: bool = False,
-                realtime: bool = True, solver: Solver | None = None) -> tuple[list[Result], float]:
-    """One solve at a time in arrival order, each with its own launches
-    (on `solver` if given, kept open)."""
+                realtime: bool = True, solver: Solver | None = None,
+                connectivity: str = "host", on_finish=None) -> tuple[list[Result], float]:
+    """One solve at a time in arrival order, each with its own connectivity
+    build and launches (on `solver` if given, kept open).  `on_finish` as
+    for serve."""
     s = solver if solver is not None else Solver(device)
     results = []
     t0 = time.perf_counter()
@@ -274,11 +306,13 @@ def serve_plain(reqs: list[Request], device: int = 0, keep: bool = False,
                 wait = r.arrival - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(wait)
-            admit(s, r)
+            admit(s, r, connectivity)
             s.run(r.steps, 0)
             sol = s.get_sol() if keep else None
             t = time.perf_counter() - t0
             results.append(Result(r.rid, t - (r.arrival if realtime else 0.0), t, sol))
+            if on_finish is not None:
+                on_finish(r, s)
     finally:
         makespan = time.perf_counter() - t0
         if solver is None:
@@ -292,19 +326,27 @@ def summarize(results: list[Result], makespan: float) -> dict:
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
 
 
-def autotune(sample: list[Request], device: int = 0, candidates=None) -> tuple[Params, list]:
-    """Times each candidate on the sample (all requests available at once)
-    and returns the fastest with the table of trials."""
+def autotune(sample: list[Request], device: int = 0, candidates=None,
+             slack: float = 0.95) -> tuple[Params, list]:
+    """Times each candidate on the sample (all requests available at once:
+    latency = finish time) and returns, among the candidates within `slack`
+    of the best throughput, the one with the lowest median latency, with
+    the table of trials."""
     if candidates is None:
-        candidates = [Params(chunk=c, max_batch=b, use_graph=g)
-                      for c in (15, 30, 60) for b in (64, 256) for g in (False, True)]
+        candidates = [Params(chunk=c, max_batch=b, use_graph=True, max_cells=mc, connectivity=cn)
+                      for c in (30, 60) for b in (64, 256) for mc in (1 << 20, 4 << 20, 1 << 40)
+                      for cn in ("device", "host")]
     trials = []
     pool = [Solver(device) for _ in range(max(p.max_batch + p.prefetch for p in candidates))]
     for p in candidates:
-        _, ms = serve(sample, p, device, realtime=False, pool=pool)
+        res, ms = serve(sample, p, device, realtime=False, pool=pool)
+        lat = np.array([r.latency for r in res])
         trials.append({"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
-                       "solves_per_s": len(sample) / ms})
+                       "max_cells": p.max_cells, "connectivity": p.connectivity,
+                       "solves_per_s": len(sample) / ms, "p50_s": float(np.percentile(lat, 50))})
     for s in pool:
         s.close()
-    best = max(range(len(trials)), key=lambda i: trials[i]["solves_per_s"])
+    top = max(t["solves_per_s"] for t in trials)
+    ok = [i for i, t in enumerate(trials) if t["solves_per_s"] >= slack * top]
+    best = min(ok, key=lambda i: (trials[i]["p50_s"], -trials[i]["solves_per_s"]))
     return candidates[best], trials

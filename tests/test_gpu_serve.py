@@ -33,6 +33,21 @@ def test_batched_results_equal_solves_alone(requests, graph):
         assert np.array_equal(a[rid], m[rid]), rid
 
 
+def test_cells_cap_and_connectivity_on_admission_keep_results(requests):
+    """Admission under a cap on cells in flight, with the connectivity built
+    on the host at admission, against solves alone whose connectivity was
+    built on the device."""
+    print("[gpu] serve cells cap, connectivity on admission", flush=True)
+    alone, _ = S.serve_plain(requests, keep=True, realtime=False, connectivity="device")
+    capped, _ = S.serve(requests, S.Params(chunk=6, max_batch=8, max_cells=600, connectivity="host"), keep=True,
+                        realtime=False)
+    a = {r.rid: r.sol for r in alone}
+    m = {r.rid: r.sol for r in capped}
+    assert set(a) == set(m) == {r.rid for r in requests}
+    for rid in a:
+        assert np.array_equal(a[rid], m[rid]), rid
+
+
 def test_autotuner_picks_a_candidate(requests):
     print("[gpu] serve autotune", flush=True)
     best, trials = S.autotune(requests[:4], candidates=[S.Params(chunk=3, max_batch=2),

@@ -1,7 +1,8 @@
 """The served-request stream (config-5 shape) on the CPU: seeded and
 deterministic, quad/tri about 50/50, cell counts log-uniform in
 [32^2, 512^2], states inside the stated ranges, Poisson arrivals at the
-requested rate; client-side preparation builds each connectivity."""
+requested rate; client-side preparation generates each mesh (and, when
+asked, builds its connectivity)."""
 import numpy as np
 
 from subfv_amd import serve as S
@@ -30,10 +31,13 @@ def test_stream_is_seeded_and_in_range():
 
 
 def test_prepare_builds_connectivity():
-    reqs = S.prepare(S.make_requests(6, rate=10.0, cmin=8 ** 2, cmax=20 ** 2), workers=3)
+    reqs = S.prepare(S.make_requests(6, rate=10.0, cmin=8 ** 2, cmax=20 ** 2), workers=3, connectivity=True)
     for r in reqs:
         assert r.cells == r.n * r.n * (1 if r.kind == "quad" else 2)
         assert r.host.geti("elem_vert").size > 0
+    raw = S.prepare(S.make_requests(2, rate=10.0, cmin=8 ** 2, cmax=20 ** 2), workers=2)
+    for r in raw:
+        assert r.host is None and r.mesh.n_elems == r.cells
 
 
 def test_requests_from_mesh_files(tmp_path):
@@ -41,12 +45,12 @@ def test_requests_from_mesh_files(tmp_path):
     the generated mesh the file was written from."""
     from subfv_amd import meshgen as mg
     reqs = S.make_requests(2, rate=10.0, cmin=8 ** 2, cmax=12 ** 2)
-    gen = S.prepare([S.Request(**{**r.__dict__}) for r in reqs], workers=2)
+    gen = S.prepare([S.Request(**{**r.__dict__}) for r in reqs], workers=2, connectivity=True)
     for r in reqs:
         mesh = mg.quad_mesh(r.n, seed=S.SEED + r.rid) if r.kind == "quad" else mg.tri_mesh(r.n, seed=S.SEED + r.rid)
         r.mesh_path = str(tmp_path / f"r{r.rid}.msh")
         mg.write_msh(mesh, r.mesh_path)
-    from_files = S.prepare(reqs, workers=2)
+    from_files = S.prepare(reqs, workers=2, connectivity=True)
     for a, b in zip(gen, from_files):
         assert a.cells == b.cells
         for name in ("elem_vert", "sub_face_left_elem", "face_right", "vert_sub_face"):
