@@ -202,6 +202,7 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cm
         "throughput_vs_plain": served["solves_per_s"] / plain["solves_per_s"],
         "served_vs_alone_mismatches": mismatches,
         "autotuned": {"chunk": best.chunk, "max_batch": best.max_batch, "use_graph": best.use_graph,
+                      "persistent": best.persistent, "wgs_per_cu": best.wgs_per_cu,
                       "max_cells": best.max_cells, "connectivity": best.connectivity},
         "autotune_trials": trials,
         "note": ("same requests and arrival times for both; connectivity built inside each latency"

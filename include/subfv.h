@@ -307,14 +307,19 @@ int subfv_bcsr_add_block(subfv_bcsr *A, int32_t i, int32_t j, const double *M);
  * its own mesh, state, dt and omega, so its results are bit-identical to
  * subfv_run(ctx, n_steps, 0, ...) alone.  Members must share the device and
  * the scheme and run the first-order inviscid forward-Euler loop.
- * subfv_batch_run enqueues n_steps steps on the batch's stream and returns;
- * with use_graph, whole 3-step cycles replay one captured hipGraph (re-captured
- * when the membership changes).  Call subfv_batch_wait before using a member
- * context again. */
+ * subfv_batch_run enqueues n_steps steps on the batch's stream and returns.
+ * use_graph 0: six multi-instance launches per step; 1: whole 3-step cycles
+ * replay one captured hipGraph (re-captured when the membership changes);
+ * 2: one persistent cooperative launch for all n_steps (a resident grid
+ * with device-side work queues and grid barriers between the phases).
+ * Call subfv_batch_wait before using a member context again.
+ * subfv_batch_set_grid: workgroups per CU of the persistent launch (0, the
+ * default: as many as fit). */
 typedef struct subfv_batch subfv_batch;
 int subfv_batch_create(subfv_batch **b, int device);
 void subfv_batch_destroy(subfv_batch *b);
 const char *subfv_batch_error(const subfv_batch *b);
+int subfv_batch_set_grid(subfv_batch *b, int32_t wgs_per_cu);
 int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps, int32_t use_graph);
 int subfv_batch_wait(subfv_batch *b);
 

@@ -127,6 +127,7 @@ SIGNATURES = {
     "subfv_batch_destroy": (None, [_P]),
     "subfv_batch_error": (C.c_char_p, [_P]),
     "subfv_batch_run": (C.c_int, [_P, C.POINTER(C.c_void_p), _i32, _i32, _i32]),
+    "subfv_batch_set_grid": (C.c_int, [_P, _i32]),
     "subfv_batch_wait": (C.c_int, [_P]),
     "subfv_init_1drp": (C.c_int, [_P, C.c_double, _dp, _dp]),
     "subfv_compute_rhs_lagrange": (C.c_int, [_P, _dp, _dp, _i32, _dp, _dp]),

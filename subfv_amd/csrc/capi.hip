@@ -88,6 +88,11 @@ struct subfv_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool graph_ok = false;
+  // persistent form: workgroups per CU (0: as many as fit) and the
+  // queue / arrival counters of one launch
+  int wgs_per_cu = 0;
+  unsigned *d_q = nullptr;
+  size_t cap_q = 0;
 };
 
 struct subfv_ctx {
@@ -2425,6 +2430,50 @@ void launch_phase(subfv_batch *b, int n, int phase) {
                      (const InstDesc *)b->d_desc, (const int *)b->d_ctab, b->total_c, phase);
 }
 
+static_assert(kGrpThreads == kCellThreads, "the persistent kernel runs node groups and cell blocks alike");
+
+// n_steps steps in one cooperative launch of k_batch_persistent
+template <int S>
+int launch_persistent(subfv_batch *b, int n, int n_steps) {
+  auto kern = k_batch_persistent<S>;
+  int per_cu = 0, cus = 0;
+  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, kGrpThreads, 0));
+  BHIP(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  if (b->wgs_per_cu > 0) per_cu = std::min(per_cu, b->wgs_per_cu);
+  // every XCD slice of the queues needs workgroups: a multiple of 8
+  const int grid = (per_cu * cus) / 8 * 8;
+  if (grid < 8) return bfail(b, SUBFV_E_DEVICE, "persistent batch kernel: no room for a resident grid");
+  const size_t nq = 32ull * n_steps + 2ull * n_steps * n;
+  int r;
+  if ((r = grow(b, &b->d_q, &b->cap_q, nq))) return r;
+  BHIP(b, hipMemsetAsync(b->d_q, 0, nq * sizeof(unsigned), b->stream));
+  PersistArgs a;
+  a.d = b->d_desc;
+  a.gtab = b->d_gtab;
+  a.ctab = b->d_ctab;
+  a.total_g = b->total_g;
+  a.total_c = b->total_c;
+  a.n_inst = n;
+  a.n_steps = n_steps;
+  a.qctr = b->d_q;
+  a.done = b->d_q + 32ull * n_steps;
+  void *args[] = {&a};
+  BHIP(b, hipLaunchCooperativeKernel((const void *)kern, dim3(grid), dim3(kGrpThreads), args, 0, b->stream));
+  return 0;
+}
+
+int launch_persistent_any(subfv_batch *b, int n, int n_steps) {
+  switch (b->scheme) {
+    case kMultiPoint: return launch_persistent<kMultiPoint>(b, n, n_steps);
+    case kMultiPointIso: return launch_persistent<kMultiPointIso>(b, n, n_steps);
+    case kThreeWave: return launch_persistent<kThreeWave>(b, n, n_steps);
+    case kTwoWave: return launch_persistent<kTwoWave>(b, n, n_steps);
+    case kModifiedThreeWave: return launch_persistent<kModifiedThreeWave>(b, n, n_steps);
+    case kMultiPointPressure: return launch_persistent<kMultiPointPressure>(b, n, n_steps);
+    default: return launch_persistent<kMultiPointPressurePh>(b, n, n_steps);
+  }
+}
+
 void launch_phase_any(subfv_batch *b, int n, int phase) {
   switch (b->scheme) {
     case kMultiPoint: launch_phase<kMultiPoint>(b, n, phase); break;
@@ -2462,14 +2511,21 @@ void subfv_batch_destroy(subfv_batch *b) {
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_gtab) (void)hipFree(b->d_gtab);
   if (b->d_ctab) (void)hipFree(b->d_ctab);
+  if (b->d_q) (void)hipFree(b->d_q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
 const char *subfv_batch_error(const subfv_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
+int subfv_batch_set_grid(subfv_batch *b, int32_t wgs_per_cu) {
+  if (!b || wgs_per_cu < 0) return SUBFV_E_INVALID;
+  b->wgs_per_cu = wgs_per_cu;
+  return 0;
+}
+
 int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps, int32_t use_graph) {
-  if (!b || !ctxs || n < 1 || n_steps < 0) return SUBFV_E_INVALID;
+  if (!b || !ctxs || n < 1 || n_steps < 0 || use_graph < 0 || use_graph > 2) return SUBFV_E_INVALID;
   if (n > (1 << (31 - kTabShift)) - 1) return bfail(b, SUBFV_E_INVALID, "too many instances in one batch");
   BHIP(b, hipSetDevice(b->device));
   const int scheme = ctxs[0] ? ctxs[0]->bc.scheme : 0;
@@ -2523,7 +2579,10 @@ int subfv_batch_run(subfv_batch *b, subfv_ctx **ctxs, int32_t n, int32_t n_steps
   for (int i = 0; i < n; i++) fill_desc(ctxs[i], hd[i]);
   BHIP(b, hipMemcpyAsync(b->d_desc, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, b->stream));
   int s = 0;
-  if (use_graph && n_steps >= 3) {
+  if (use_graph == 2 && n_steps > 0) {
+    if ((r = launch_persistent_any(b, n, n_steps))) return r;
+    s = n_steps;
+  } else if (use_graph == 1 && n_steps >= 3) {
     if (!b->graph_ok) {
       batch_drop_graph(b);
       BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

@@ -19,7 +19,8 @@ finishes later.
 
 `serve_plain` is the baseline: the same requests one at a time, in arrival
 order, each with its own connectivity build and launches (subfv_run).
-`autotune` picks the chunk length, the batch capacity, graph replay, the
+`autotune` picks the chunk length, the launch form (per-step launches,
+graph replay, or the persistent kernel and its workgroups per CU), the
 cells cap and where connectivity is built by timing candidates on a
 calibration sample.
 """
@@ -136,9 +137,17 @@ class Batch:
             raise SubfvError(st, "subfv_batch_create failed")
         self.h = h
 
-    def run(self, solvers, n_steps: int, use_graph: bool = True):
+    def set_grid(self, wgs_per_cu: int):
+        """Workgroups per CU of the persistent launch (0: as many as fit)."""
+        st = _lib.lib().subfv_batch_set_grid(self.h, wgs_per_cu)
+        if st:
+            raise SubfvError(st, "subfv_batch_set_grid failed")
+
+    def run(self, solvers, n_steps: int, mode: int = 1):
+        """mode 0: one multi-instance launch per kernel of a step, 1: replayed
+        3-step hipGraph, 2: one persistent cooperative launch."""
         arr = (C.c_void_p * len(solvers))(*[s.h.value for s in solvers])
-        st = _lib.lib().subfv_batch_run(self.h, arr, len(solvers), n_steps, int(use_graph))
+        st = _lib.lib().subfv_batch_run(self.h, arr, len(solvers), n_steps, int(mode))
         if st:
             raise SubfvError(st, _lib.lib().subfv_batch_error(self.h).decode())
 
@@ -166,6 +175,8 @@ class Params:
     chunk: int = 30          # steps per batch call (a multiple of 3 keeps the graph cycle aligned)
     max_batch: int = 256     # instances resident at once
     use_graph: bool = True
+    persistent: bool = False  # one persistent cooperative launch per chunk (overrides use_graph)
+    wgs_per_cu: int = 0       # its workgroups per CU (0: as many as fit)
     budget: int = 96 << 30   # HBM bytes for admitted instances
     admit_workers: int = 8   # host threads uploading admitted requests
     prefetch: int = 64       # instances uploaded ahead of free batch capacity
@@ -259,7 +270,8 @@ def serve(reqs: list[Request], params: Params = Params(), device: int = 0, keep:
             steps = min(params.chunk, min(a[2] for a in active))
             if batch is None:
                 batch = Batch(device)
-            batch.run([a[1] for a in active], steps, params.use_graph)
+                batch.set_grid(params.wgs_per_cu)
+            batch.run([a[1] for a in active], steps, 2 if params.persistent else int(params.use_graph))
             start_uploads(params.max_batch - len(active) + params.prefetch - len(staged) - len(uploading))
             batch.wait()
             done_t = time.perf_counter() - t0
@@ -333,8 +345,12 @@ def autotune(sample: list[Request], device: int = 0, candidates=None,
     of the best throughput, the one with the lowest median latency, with
     the table of trials."""
     if candidates is None:
-        candidates = [Params(chunk=c, max_batch=b, use_graph=True, max_cells=mc, connectivity=cn)
-                      for c in (30, 60) for b in (64, 256) for mc in (1 << 20, 4 << 20, 1 << 40)
+        # launch form: per-step launches, replayed graph, or the persistent
+        # kernel with all the workgroups that fit or two per CU
+        forms = [dict(use_graph=False), dict(use_graph=True), dict(persistent=True),
+                 dict(persistent=True, wgs_per_cu=2)]
+        candidates = [Params(chunk=c, max_batch=256, max_cells=mc, connectivity=cn, **f)
+                      for c in (30, 60) for f in forms for mc in (1 << 20, 4 << 20, 1 << 40)
                       for cn in ("device", "host")]
     trials = []
     pool = [Solver(device) for _ in range(max(p.max_batch + p.prefetch for p in candidates))]
@@ -342,6 +358,7 @@ def autotune(sample: list[Request], device: int = 0, candidates=None,
         res, ms = serve(sample, p, device, realtime=False, pool=pool)
         lat = np.array([r.latency for r in res])
         trials.append({"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
+                       "persistent": p.persistent, "wgs_per_cu": p.wgs_per_cu,
                        "max_cells": p.max_cells, "connectivity": p.connectivity,
                        "solves_per_s": len(sample) / ms, "p50_s": float(np.percentile(lat, 50))})
     for s in pool:

@@ -20,12 +20,17 @@ def requests():
     return S.prepare(reqs, workers=4)
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_batched_results_equal_solves_alone(requests, graph):
-    print(f"[gpu] serve equality graph={graph}", flush=True)
+@pytest.mark.parametrize("mode", ["launches", "graph", "persistent", "persistent1"])
+def test_batched_results_equal_solves_alone(requests, mode):
+    """Per-step launches, replayed graphs, and the persistent cooperative
+    kernel (all workgroups that fit, or one per CU so that each takes many
+    queue items)."""
+    print(f"[gpu] serve equality mode={mode}", flush=True)
     alone, _ = S.serve_plain(requests, keep=True, realtime=False)
     # capacity 3 and 6-step chunks: instances are admitted while others run
-    mixed, _ = S.serve(requests, S.Params(chunk=6, max_batch=3, use_graph=graph), keep=True, realtime=False)
+    p = S.Params(chunk=6, max_batch=3, use_graph=mode == "graph", persistent=mode.startswith("persistent"),
+                 wgs_per_cu=1 if mode == "persistent1" else 0)
+    mixed, _ = S.serve(requests, p, keep=True, realtime=False)
     a = {r.rid: r.sol for r in alone}
     m = {r.rid: r.sol for r in mixed}
     assert set(a) == set(m) == {r.rid for r in requests}
@@ -111,15 +116,15 @@ def test_finished_solves_are_written_in_the_background(requests, tmp_path):
         assert np.array_equal(rows[:, 3:8], r.sol)
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_pooled_context_reused_for_a_different_mesh(requests, graph):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_pooled_context_reused_for_a_different_mesh(requests, mode):
     """One context serves every request in turn (capacity 1, no prefetch):
     each new request uploads a mesh of another size into the context the
     batch used before, so the batch must rebuild its tables and graph."""
-    print(f"[gpu] serve pooled context reuse graph={graph}", flush=True)
+    print(f"[gpu] serve pooled context reuse mode={mode}", flush=True)
     alone, _ = S.serve_plain(requests, keep=True, realtime=False)
-    one, _ = S.serve(requests, S.Params(chunk=6, max_batch=1, prefetch=0, first=1, use_graph=graph), keep=True,
-                     realtime=False)
+    p = S.Params(chunk=6, max_batch=1, prefetch=0, first=1, use_graph=mode == 1, persistent=mode == 2)
+    one, _ = S.serve(requests, p, keep=True, realtime=False)
     a = {r.rid: r.sol for r in alone}
     m = {r.rid: r.sol for r in one}
     assert set(a) == set(m)
