@@ -191,7 +191,9 @@ int subfv_compute_rhs_ns(subfv_ctx *ctx);
  * i = 1..n_bc (arrays 0-based): bc_v_id / bc_t_id 1 = neumann, 2 = dirichlet
  * (bc_type_V / bc_type_T), bc_val_v(3, n_bc), bc_val_t(n_bc).  Applies to
  * subfv_compute_rhs_ns (rhs and "diag") and to the stages of subfv_run.
- * Vertices may have at most 16 sub-faces and 16 sub-elements. */
+ * Vertices may have at most 64 sub-faces and 64 sub-elements (a mesh whose
+ * vertices all have at most 16 takes kernels with the setup's matrices in LDS;
+ * others, e.g. tetrahedral meshes, keep them in a global workspace). */
 int subfv_set_diffusion(subfv_ctx *ctx, int32_t activate, double mu_p, double cv_p, double prandtl,
                         int32_t n_bc, const int32_t *bc_v_id, const double *bc_val_v, const int32_t *bc_t_id,
                         const double *bc_val_t);

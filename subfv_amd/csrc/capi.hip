@@ -139,6 +139,8 @@ struct subfv_ctx {
   double *dsvol = nullptr;   // [SE] node-sorted slot -> sub-element volume
   double *mu_cell = nullptr, *mu_blk = nullptr;  // Sutherland: [C] viscosities; block minima + mu0corr
   int diff_fm = 0, diff_em = 0;  // setup LDS extents: 3 * max sub-faces, 3 * max sub-elements of a vertex
+  bool diff_large = false;        // a vertex beyond the small size class: large-class kernels
+  double *dws = nullptr;          // large class: the setup's workspace, kDiffWsBlocks blocks
   long pre_n = 0;  // doubles in the packed viscous blocks
   int *derr = nullptr;  // [2]: error bits of the path, failed check_mesh cells
   StepState *st = nullptr;
@@ -240,6 +242,7 @@ void free_mesh(subfv_ctx *c) {
   c->pre = c->PD = c->rdiff = c->dsvol = nullptr;
   c->dssf = nullptr;
   c->mu_cell = c->mu_blk = nullptr;
+  c->dws = nullptr;
   c->diff_ready = false;
 }
 
@@ -649,6 +652,28 @@ int launch_update(subfv_ctx *c, const double *U, const double *U0, double *Uout,
 }
 
 // ------------------------------------------------------------- diffusion
+// The setup kernel over every vertex: the small class with its matrices in
+// dynamic LDS in one launch; the large class with them in the workspace,
+// kDiffWsBlocks vertices per launch.
+void launch_diff_setup(subfv_ctx *c, DiffArgs a) {
+  a.set_fm = c->diff_fm;
+  a.set_em = c->diff_em;
+  const long V = c->dm.V;
+  if (!c->diff_large) {
+    a.v0 = 0;
+    hipLaunchKernelGGL((k_diff_setup<kDiffMaxSub, false>), dim3((unsigned)V), dim3(64),
+                       diff_setup_lds_bytes(a.set_fm, a.set_em), c->stream, a);
+    return;
+  }
+  a.ws = c->dws;
+  a.ws_stride = diff_setup_ws_stride(a.set_fm, a.set_em);
+  for (long v0 = 0; v0 < V; v0 += kDiffWsBlocks) {
+    a.v0 = (int)v0;
+    hipLaunchKernelGGL((k_diff_setup<kDiffMaxSubL, true>), dim3((unsigned)std::min<long>(kDiffWsBlocks, V - v0)),
+                       dim3(64), 0, c->stream, a);
+  }
+}
+
 // Once per mesh and parameter set: offsets of the packed per-vertex blocks,
 // buffers, and the setup kernel (nodal systems, inverses, products).
 int ensure_diffusion(subfv_ctx *c) {
@@ -659,18 +684,20 @@ int ensure_diffusion(subfv_ctx *c) {
   int mf = 1, me = 1;
   for (int v = 0; v < h.nv; v++) {
     const int nf = h.sf_off[v + 1] - h.sf_off[v], ns = h.v_seoff[v + 1] - h.v_seoff[v];
-    if (nf > kDiffMaxSub || ns > kDiffMaxSub)
+    if (nf > kDiffMaxSubL || ns > kDiffMaxSubL)
       return fail(c, SUBFV_E_INVALID, "viscous terms: vertex " + std::to_string(v + 1) + " has more than " +
-                                          std::to_string(kDiffMaxSub) + " sub-faces or sub-elements");
+                                          std::to_string(kDiffMaxSubL) + " sub-faces or sub-elements");
     off[v + 1] = off[v] + diff_block_size(nf, ns);
     mf = std::max(mf, nf);
     me = std::max(me, ns);
   }
   c->diff_fm = 3 * mf;
   c->diff_em = 3 * me;
+  c->diff_large = mf > kDiffMaxSub || me > kDiffMaxSub;
   const size_t lds = diff_setup_lds_bytes(c->diff_fm, c->diff_em);
-  if (lds > (64u << 10))
-    HIPCHK(c, hipFuncSetAttribute((const void *)k_diff_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (!c->diff_large && lds > (64u << 10))
+    HIPCHK(c, hipFuncSetAttribute((const void *)k_diff_setup<kDiffMaxSub, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int r;
   // the buffers' sizes depend on the mesh only: allocated once per mesh and
   // kept when the viscous parameters change (a new mesh resets them)
@@ -703,6 +730,9 @@ int ensure_diffusion(subfv_ctx *c) {
         }
       }
     if ((r = upload(c, &c->dssf, ssf))) return r;
+    if (c->diff_large &&
+        (r = dalloc(c, &c->dws, (size_t)kDiffWsBlocks * diff_setup_ws_stride(c->diff_fm, c->diff_em))))
+      return r;
     if ((r = dalloc(c, &c->dsvol, (size_t)h.nse))) return r;
     hipLaunchKernelGGL(k_slot_vol, dim3(nblk(h.nse, 256)), dim3(256), 0, c->stream, (long)h.nse,
                        (const int *)c->dm.v_seid, (const double *)c->dm.se_vol, c->dsvol);
@@ -724,9 +754,7 @@ int ensure_diffusion(subfv_ctx *c) {
   a.rdiff = c->rdiff;
   a.diag = c->diag;
   // with Sutherland's law the blocks depend on the state: formed per residual
-  a.set_fm = c->diff_fm;
-  a.set_em = c->diff_em;
-  if (!c->dparams.suth) hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)h.nv), dim3(64), lds, c->stream, a);
+  if (!c->dparams.suth) launch_diff_setup(c, a);
   HIPCHK(c, hipGetLastError());
   c->diff_ready = true;
   return 0;
@@ -780,18 +808,20 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
                        c->mu_blk + nb);
     a.mu_cell = c->mu_cell;
     a.mu_min = c->mu_blk + nb;
-    a.set_fm = c->diff_fm;
-    a.set_em = c->diff_em;
-    hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)c->dm.V), dim3(64), diff_setup_lds_bytes(a.set_fm, a.set_em),
-                       c->stream, a);
+    launch_diff_setup(c, a);
   }
-  a.vblocks = (int)nblk(c->dm.V, kDiffWaves);
+  const int waves = c->diff_large ? 1 : kDiffWaves;
+  a.vblocks = (int)nblk(c->dm.V, waves);
   a.vper_xcd = (a.vblocks + 7) / 8;
   a.cblocks = (int)nblk(c->dm.C, 256);
   a.cper_xcd = (a.cblocks + 7) / 8;
   {
     Prof pa(c, "diff_apply");
-    hipLaunchKernelGGL(k_diff_apply, dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0, c->stream, a);
+    if (c->diff_large)
+      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSubL, 1>), dim3(8 * a.vper_xcd), dim3(64), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSub, kDiffWaves>), dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0,
+                         c->stream, a);
   }
   hipLaunchKernelGGL(k_diff_gather, dim3(8 * a.cper_xcd), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());

@@ -9,7 +9,8 @@
 // law (compute_mu, ns_vectorial_diffusion_module.F90:322-359) mu varies per
 // cell and step: the same setup kernel then runs before every residual, on
 // the cell viscosities and their minimum mu0corr (one wavefront per
-// vertex, matrices in LDS, the inverse in the order of LU + unblocked
+// vertex, matrices in LDS, or in a global workspace for vertices beyond 16
+// sub-faces or sub-elements, the inverse in the order of LU + unblocked
 // dgetri, products summed over k in increasing order):
 //   momentum: SNS = (S~^T N^-1) S, SB = S~^T (N^-1 B), the sub-element sums of
 //             S~^T, and N^-1 S, N^-1 B (the face velocities Vf = N^-1 (S Ve + B)
@@ -27,8 +28,13 @@
 
 namespace sfv {
 
-constexpr int kDiffMaxSub = 16;             // sub-faces (and sub-elements) per vertex
+// Two size classes of vertex: up to kDiffMaxSub sub-faces and sub-elements
+// (prism, hex and quad meshes; the setup's matrices in LDS, four vertices per
+// apply workgroup) and up to kDiffMaxSubL (tetrahedral meshes; the setup's
+// matrices in a global workspace, one vertex per apply workgroup).
+constexpr int kDiffMaxSub = 16;             // sub-faces (and sub-elements) per vertex, small class
 constexpr int kDiffMaxF = 3 * kDiffMaxSub;  // 48
+constexpr int kDiffMaxSubL = 64;            // large class
 constexpr int kDiffSlot = 18;               // partials per slot
 
 struct DiffParams {
@@ -54,17 +60,22 @@ SFV_DEV __host__ long diff_block_size(int nf, int ns) {
 SFV_DEV __host__ inline size_t diff_setup_lds_bytes(int fm, int em) {
   return sizeof(double) * ((size_t)fm * fm + 3ull * fm * em);
 }
+// the large class keeps the same four matrices in a global workspace, one
+// stride of doubles per setup block, kDiffWsBlocks blocks per launch
+SFV_DEV __host__ inline long diff_setup_ws_stride(int fm, int em) { return (long)fm * fm + 3L * fm * em; }
+constexpr int kDiffWsBlocks = 2048;
 
+template <int MS>
 struct DiffSetupLds {
-  double B[kDiffMaxF], x[kDiffMaxF], work[kDiffMaxF];
-  int piv[kDiffMaxF];
-  int lse[kDiffMaxSub], rse[kDiffMaxSub];  // local slot of each sub-face's left/right cell (-1 none)
+  double B[3 * MS], x[3 * MS], work[3 * MS];
+  int piv[3 * MS];
+  int lse[MS], rse[MS];                    // local slot of each sub-face's left/right cell (-1 none)
   // the vertex's mesh data, staged once so the assembly loops read LDS only
-  double g[kDiffMaxSub][4];                // unit normal and area of each sub-face
-  int le[kDiffMaxSub], re[kDiffMaxSub];    // its left and right neighbour codes
-  double sevol[kDiffMaxSub], semu[kDiffMaxSub];  // volume and cell viscosity of each sub-element
-  int secell[kDiffMaxSub];
-  int ssf[kDiffMaxSub][kSlotSrc + 1];      // local sub-faces of each sub-element, count first
+  double g[MS][4];                         // unit normal and area of each sub-face
+  int le[MS], re[MS];                      // its left and right neighbour codes
+  double sevol[MS], semu[MS];              // volume and cell viscosity of each sub-element
+  int secell[MS];
+  int ssf[MS][kSlotSrc + 1];               // local sub-faces of each sub-element, count first
 };
 
 struct DiffArgs {
@@ -81,6 +92,9 @@ struct DiffArgs {
   const double *mu_min;   // mu0corr, their minimum (Sutherland)
   int vper_xcd, vblocks;  // XCD slices of the vertex blocks (apply)
   int cper_xcd, cblocks;  // and of the cell blocks (gather)
+  double *ws;             // setup of the large class: the matrices of each block, ws_stride doubles apart
+  long ws_stride;
+  int v0;                 // setup: first vertex of the launch
   const uint32_t *slot_ssf;  // [SE] node-sorted slot: its sub-element's vertex-local sub-faces, a byte each, 0xFF ends
   const double *slot_vol;    // [SE] node-sorted slot: its sub-element's volume
 };
@@ -104,14 +118,20 @@ SFV_DEV void wave_inverse(double *A, int n, int *piv, double *work, int lane) {
   for (int k = 0; k < n; k++) {
     // pivot: the first row of the largest |A(i, k)|, i >= k, as the serial
     // scan with a strict > finds it (NaN never replaces the running maximum,
-    // so a NaN diagonal keeps p = k): a butterfly over the wave, ties to the
-    // lower row (n <= 48 <= 64 lanes)
+    // so a NaN diagonal keeps p = k): each lane scans rows k + lane + 64 t in
+    // increasing order, then a butterfly over the wave, ties to the lower row
     int p;
     {
-      const int i = k + lane;
-      double v = i < n ? fabs(AA(i, k)) : -1.0;
-      if (!(v == v)) v = -1.0;
-      int idx = i;
+      double v = -1.0;
+      int idx = k + lane;
+      for (int i = k + lane; i < n; i += 64) {
+        double a = fabs(AA(i, k));
+        if (!(a == a)) a = -1.0;
+        if (a > v) {
+          v = a;
+          idx = i;
+        }
+      }
       for (int off = 32; off > 0; off >>= 1) {
         const double ov = __shfl_xor(v, off);
         const int oi = __shfl_xor(idx, off);
@@ -228,28 +248,31 @@ SFV_DEV double blc_mu_ij(double s, double mu, double mu0, const double *nl, cons
   return s * (mu * d * (i == j ? 1.0 : 0.0) + (mu - mu0) * (np[i] * nl[j]) + (mu0 - (2.0 / 3.0) * mu) * (np[j] * nl[i]));
 }
 
-// One 64-thread workgroup per vertex.
+// One 64-thread workgroup per vertex (vertex v0 + blockIdx.x).  MS: the size
+// class; GWS: the four matrices in the global workspace (else dynamic LDS).
+template <int MS, bool GWS>
 __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
-  __shared__ DiffSetupLds L;
+  __shared__ DiffSetupLds<MS> L;
   extern __shared__ double dyn_lds[];
   const int fm = a.set_fm, em = a.set_em;
-  double *const Nm = dyn_lds, *const Sm = Nm + fm * fm, *const Tm = Sm + fm * em, *const Wm = Tm + fm * em;
+  double *const Nm = GWS ? a.ws + a.ws_stride * blockIdx.x : dyn_lds;
+  double *const Sm = Nm + fm * fm, *const Tm = Sm + fm * em, *const Wm = Tm + fm * em;
   const DevMesh &m = a.m;
   const DiffParams &P = a.p;
-  const long v = blockIdx.x;
+  const long v = (long)a.v0 + blockIdx.x;
   if (v >= m.V) return;
   const int lane = threadIdx.x;
   const int sf0 = m.sf_off[v], nf = m.sf_off[v + 1] - sf0;
   const int se0 = m.se_off[v], ns = m.se_off[v + 1] - se0;
   const int F = 3 * nf, E = 3 * ns;
   double *out = a.pre + a.doff[v];
-  if (lane < nf) {
-    const long s = sf0 + lane;
-    for (int d = 0; d < 4; d++) L.g[lane][d] = m.sfn[d * m.SF + s];
-    L.le[lane] = m.sf_le[s];
-    L.re[lane] = m.sf_re[s];
-  } else if (lane >= 32 && lane - 32 < ns) {
-    const int q = lane - 32;
+  for (int l = lane; l < nf; l += 64) {
+    const long s = sf0 + l;
+    for (int d = 0; d < 4; d++) L.g[l][d] = m.sfn[d * m.SF + s];
+    L.le[l] = m.sf_le[s];
+    L.re[l] = m.sf_re[s];
+  }
+  for (int q = lane; q < ns; q += 64) {
     const long se = m.v_seid[se0 + q];
     const int cell = m.se_cell[se0 + q];
     L.secell[q] = cell;
@@ -261,28 +284,28 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   }
   wsync_d();
   // local slots of the left / right cells of each sub-face
-  if (lane < nf) {
-    const int le = L.le[lane], re = L.re[lane];
+  for (int l = lane; l < nf; l += 64) {
+    const int le = L.le[l], re = L.re[l];
     int ls = -1, rs = -1;
     for (int q = 0; q < ns; q++) {
       const int c = L.secell[q];
       if (c == le) ls = q;
       if (re >= 0 && c == re) rs = q;
     }
-    L.lse[lane] = ls;
-    L.rse[lane] = rs;
+    L.lse[l] = ls;
+    L.rse[l] = rs;
   }
   for (int q = lane; q < fm * fm; q += 64) Nm[q] = 0.0;
   for (int q = lane; q < fm * em; q += 64) Sm[q] = Tm[q] = 0.0;
-  for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
+  for (int q = lane; q < 3 * MS; q += 64) L.B[q] = 0.0;
   wsync_d();
 
   // ---------------- heat (scalar entries, reference loop order).  Sub-face l
   // only writes row l of N and S, column l of S~^T and B[l], so one lane per
   // sub-face adds every entry's terms in the serial loop's order.
-  if (lane < nf) {
+  for (int l0 = lane; l0 < nf; l0 += 64) {
     {
-      const int l = lane;
+      const int l = l0;
       const double al = L.g[l][3];
       for (int side = 0; side < 2; side++) {
         const int so = side == 0 ? L.lse[l] : L.rse[l];
@@ -335,7 +358,7 @@ __global__ __launch_bounds__(64) void k_diff_setup(DiffArgs a) {
   // ---------------- momentum (3x3 blocks, lane (i, j) owns entry (i, j) of a block)
   for (int q = lane; q < fm * fm; q += 64) Nm[q] = 0.0;
   for (int q = lane; q < fm * em; q += 64) Sm[q] = Tm[q] = 0.0;
-  for (int q = lane; q < kDiffMaxF; q += 64) L.B[q] = 0.0;
+  for (int q = lane; q < 3 * MS; q += 64) L.B[q] = 0.0;
   wsync_d();
   // sub-face l only writes its rows of N and S and its columns of S~^T:
   // lanes (l mod 7, i, j), l = lane / 9, lane / 9 + 7, ...
@@ -450,14 +473,17 @@ constexpr int kDiffWaves = 4;
 // with more sub-elements read them from the block in HBM instead
 constexpr int kDiffStage = 320;
 
+template <int MS>
 struct DiffApplyLds {
-  double Ve[kDiffMaxF], Vf[kDiffMaxF], SVe[kDiffMaxF];
-  double th[kDiffMaxSub], rho[kDiffMaxSub], P[kDiffMaxSub][5], vol[kDiffMaxSub], pu[kDiffMaxSub];
-  double nrm[kDiffMaxSub][4];  // unit normal and area of each sub-face of the vertex
-  int le[kDiffMaxSub];         // left cell of each sub-face
-  int cell[kDiffMaxSub];       // cell of each slot
-  int ssf[kDiffMaxSub][kSlotSrc + 1];  // local sub-faces of each slot's sub-element, count first
-  double pc[kDiffStage];       // the slot terms' block data, staged while the row pass streams
+  static constexpr int kPc = kDiffStage > 13 * MS ? kDiffStage : 13 * MS;
+  double Ve[3 * MS], Vf[3 * MS], SVe[3 * MS];
+  double th[MS], rho[MS], P[MS][5], vol[MS], pu[MS];
+  double nrm[MS][4];           // unit normal and area of each sub-face of the vertex
+  int le[MS];                  // left cell of each sub-face
+  int cell[MS];                // cell of each slot
+  int ssf[MS][kSlotSrc + 1];   // local sub-faces of each slot's sub-element, count first
+  double pc[kPc];              // the slot terms' block data, staged while the row pass streams;
+                               // then the stresses (9 ns) and work terms (4 ns)
 };
 
 // (c) of k_diff_apply: the slot rows of one vertex from the row pass's Vf and
@@ -466,8 +492,8 @@ struct DiffApplyLds {
 // The viscous work is spread over (slot, tensor entry) and (slot, sub-face)
 // lanes; every value is formed with the same operations in the same order
 // as one lane per slot would.
-template <bool STAGED>
-SFV_DEV void diff_slot_terms(const DiffArgs &a, DiffApplyLds &L, int lane, int ns, int se0, int E, const double *sns,
+template <bool STAGED, class LdsT>
+SFV_DEV void diff_slot_terms(const DiffArgs &a, LdsT &L, int lane, int ns, int se0, int E, const double *sns,
                              const double *sb, const double *blcs, const double *snsh) {
   const int o_tail = 12 * ns, o_diag = 12 * ns + ns * ns + 2 * ns;
   const double *c_sb = STAGED ? L.pc : sb, *c_blcs = STAGED ? L.pc + 3 * ns : blcs;
@@ -542,8 +568,8 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, DiffApplyLds &L, int lane, int n
   }
   wsync_d();
   // viscous work (its terms in sub-face order) and heat rhs, one lane per slot
-  if (lane < ns) {
-    const int l = lane, cnt = L.ssf[l][0];
+  for (int l = lane; l < ns; l += 64) {
+    const int cnt = L.ssf[l][0];
     double work = 0.0;
     for (int e = 0; e < cnt; e++) work = work + wt[kSlotSrc * l + e];
     double h = 0.0;
@@ -568,17 +594,19 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, DiffApplyLds &L, int lane, int n
 //     SVe = S~^T N^-1 S Ve, coalesced;
 // (c) the slot terms, spread over the lanes by (slot, entry), read only LDS
 //     and the vertex's block the wavefront has just streamed.
-__global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
-  __shared__ DiffApplyLds lds_all[kDiffWaves];
+// MS: the size class; WAVES: vertices (wavefronts) per workgroup.
+template <int MS, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 6) void k_diff_apply(DiffArgs a) {
+  __shared__ DiffApplyLds<MS> lds_all[WAVES];
   const DevMesh &m = a.m;
   // the wavefront's index is uniform: the vertex's offsets come through the
   // scalar cache
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int lb = xcd_block(a.vper_xcd);  // neighbouring vertices share cells
   if (lb >= a.vblocks) return;
-  const long v = (long)lb * kDiffWaves + wib;
+  const long v = (long)lb * WAVES + wib;
   if (v >= m.V) return;  // whole wavefront
-  DiffApplyLds &L = lds_all[wib];
+  DiffApplyLds<MS> &L = lds_all[wib];
   const int sf0 = m.sf_off[v], nf = m.sf_off[v + 1] - sf0;
   const int se0 = m.se_off[v], ns = m.se_off[v + 1] - se0;
   const int F = 3 * nf, E = 3 * ns;
@@ -587,8 +615,7 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
   const double *sns = pre, *sb = sns + (long)E * E, *blcs = sb + E, *ninvs = blcs + 9L * ns;
   const double *ninvb = ninvs + (long)F * E, *snsh = ninvb + F, *sbh = snsh + (long)ns * ns, *b1 = sbh + ns;
   // ---- (a)
-  if (lane < ns) {
-    const int l = lane;
+  for (int l = lane; l < ns; l += 64) {
     const int c = m.se_cell[se0 + l];
     const uint32_t code = a.slot_ssf[se0 + l];
     L.vol[l] = a.slot_vol[se0 + l];
@@ -613,10 +640,10 @@ __global__ __launch_bounds__(64 * kDiffWaves, 6) void k_diff_apply(DiffArgs a) {
     for (int k = 0; k < 5; k++) L.P[l][k] = P[k];
     L.pu[l] = pu;
   }
-  if (lane < nf) {
-    const long s = sf0 + lane;
-    for (int k = 0; k < 4; k++) L.nrm[lane][k] = m.sfn[k * m.SF + s];
-    L.le[lane] = m.sf_le[s];
+  for (int l = lane; l < nf; l += 64) {
+    const long s = sf0 + l;
+    for (int k = 0; k < 4; k++) L.nrm[l][k] = m.sfn[k * m.SF + s];
+    L.le[l] = m.sf_le[s];
   }
   // stage the slot terms' block data: the loads overlap the row pass below
   const bool staged = ns * ns + 23 * ns <= kDiffStage;

@@ -4,7 +4,10 @@ adds each cell's terms as one sum per vertex, where the reference adds them
 one by one; results agree to round-off, not bit for bit: the residual within
 1e-12 of the viscous terms' scale (measured: 2e-14 on tri 8^2, 4e-17 for the
 point-implicit diagonal) and the state after 100 viscous steps within 1e-12
-relative (measured: 1e-15 on tri, quad, 3-D hex and a 10-sub-element fan)."""
+relative (measured: 1e-15 on tri, quad, 3-D hex and a 10-sub-element fan).
+Tetrahedral meshes (36 sub-faces and 24 sub-elements at an interior vertex)
+take the large-class kernels: setup matrices in a global workspace, one
+vertex per apply workgroup."""
 import numpy as np
 import pytest
 
@@ -38,6 +41,7 @@ CASES = {
                           dict(bcs=[("freestream", (1.0, 0.2, 0.0, 0.0, 1.0)), ("outflowsupersonic", (0, 0, 0, 0, 0))]),
                           dict(bcs_v=[("dirichlet", (0.1, -0.05, 0.0)), ("neumann", (0, 0, 0))],
                                bcs_t=[("dirichlet", 1.5e-3), ("neumann", 0.0)])),
+    "tet3": (lambda: mg.tet_mesh(3), dict(boundary_2d=False), dict()),  # large size class
 }
 
 
@@ -118,6 +122,7 @@ LONG_CASES = {
     "tri8": lambda: mg.tri_mesh(8),
     "hex3": lambda: mg.hex_mesh(3),  # 3-D interior vertices: 12 sub-faces, 8 sub-elements
     "fan5": fan_prism_mesh,          # 10 sub-elements: slot terms read from HBM
+    "tet3": lambda: mg.tet_mesh(3),  # 36 sub-faces, 24 sub-elements: the large size class
 }
 
 
@@ -149,7 +154,7 @@ def test_viscous_100_steps_within_1e12(solver, name, ts):
 
 def test_vertices_beyond_the_limit_are_rejected(solver):
     print("[gpu] diffusion limit", flush=True)
-    solver.build_mesh(mg.tet_mesh(3))  # interior vertices have 36 sub-faces
+    solver.build_mesh(fan_prism_mesh(k=40))  # the axis vertex between the layers has 80 sub-elements
     solver.set_config(make_config("multi_point", boundary_2d=False))
     solver.set_diffusion(True, mu=1e-3)
     solver.put_sol(mg.primit_to_conserv(np.tile([1.0, 0, 0, 0, 1.0], (solver.n_elems, 1))))
