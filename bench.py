@@ -221,6 +221,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=192)
     ap.add_argument("--inviscid", action="store_true", help="drop the viscous terms (Euler only)")
+    ap.add_argument("--arith", choices=("exact", "fast"), default="exact",
+                    help="node-sweep arithmetic: exact (bit-identical to the reference, the default) or fast "
+                         "(fused multiply-adds and reciprocal products, within 1e-12)")
     ap.add_argument("--served-requests", type=int, default=192,
                     help="requests in the served-solves measurement (0 skips it)")
     ap.add_argument("--no-bcsr", dest="bcsr", action="store_false",
@@ -261,6 +264,7 @@ def main():
     SF, SE = s.n_sub_faces, s.n_sub_elems
     o2 = not args.first_order
     s.set_config(make_config("multi_point", second_order=o2, method=4, cfl=0.4))
+    s.set_arith(args.arith == "fast")
     viscous = not (args.inviscid or args.first_order)  # M1 is the inviscid first-order stage
     if viscous:
         s.set_diffusion(True, mu=1e-3, prandtl=0.72)
@@ -348,7 +352,7 @@ def main():
                             "second order (method 4 limited gradients), SSP-RK3")
                          + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid")),
             "cells": C, "stages_per_step": stages, "viscous": viscous, "mu": 1e-3 if viscous else 0.0,
-            "prandtl": 0.72 if viscous else None,
+            "prandtl": 0.72 if viscous else None, "node_sweep_arith": args.arith,
         },
         "roofline": roofline,
         **extra,

@@ -108,6 +108,7 @@ SIGNATURES = {
     "subfv_host_mesh_get_i32": (_i64, [_P, C.c_char_p, _ip, _i64]),
     "subfv_host_mesh_free": (None, [_P]),
     "subfv_set_config": (C.c_int, [_P, C.POINTER(Config)]),
+    "subfv_set_arith": (C.c_int, [_P, _i32]),
     "subfv_set_time": (C.c_int, [_P, C.c_double]),
     "subfv_state_put": (C.c_int, [_P, _dp]),
     "subfv_state_get": (C.c_int, [_P, _dp]),

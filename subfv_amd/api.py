@@ -159,6 +159,12 @@ class Solver:
     def set_config(self, cfg: Config):
         self._check(_lib.lib().subfv_set_config(self.h, C.byref(cfg)))
 
+    def set_arith(self, fast: bool):
+        """The node sweep's arithmetic: exact (the reference's expressions as
+        written, bit-identical) or fast (contracted multiply-adds and
+        reciprocal products: within 1e-12 relative, subfv_set_arith)."""
+        self._check(_lib.lib().subfv_set_arith(self.h, 1 if fast else 0))
+
     def set_time(self, t: float):
         self._check(_lib.lib().subfv_set_time(self.h, float(t)))
 

@@ -22,6 +22,11 @@
 
 using namespace sfv;
 
+namespace sfv {
+// flux_fast.hip: the node sweep compiled for SUBFV_ARITH_FAST
+void launch_node_flux_fast(int scheme, unsigned grid, hipStream_t s, const FluxArgs &a);
+}
+
 #include <mutex>
 
 // Workspace of subfv_bcsr_gmres_diag_noalloc: the Krylov basis V and the
@@ -107,6 +112,7 @@ struct subfv_ctx {
   std::string err;
   bool has_mesh = false;
   bool b2d = false;
+  int arith = SUBFV_ARITH_EXACT;  // the node sweep's arithmetic (subfv_set_arith)
   std::shared_ptr<const subfv::HostMesh> hmp;  // shared with the host mesh handle it came from
   std::shared_ptr<const DevTables> tab;
   DevMesh dm;
@@ -546,6 +552,11 @@ int launch_flux(subfv_ctx *c, const double *U, const double *G, double *P, bool 
   a.nblocks = nb;
   dim3 grid(8 * a.per_xcd), blk(kGrpThreads);
   Prof pf(c, G ? "node_flux_o2" : "node_flux");
+  if (c->arith == SUBFV_ARITH_FAST) {
+    launch_node_flux_fast(c->bc.scheme, grid.x, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+  }
   switch (c->bc.scheme) {
     case kMultiPoint: hipLaunchKernelGGL(k_node_flux<kMultiPoint>, grid, blk, 0, c->stream, a); break;
     case kMultiPointIso: hipLaunchKernelGGL(k_node_flux<kMultiPointIso>, grid, blk, 0, c->stream, a); break;
@@ -1169,6 +1180,15 @@ int64_t subfv_mesh_get_f64(subfv_ctx *c, const char *name, double *out, int64_t 
     return dev_to_host_soa(c, d.sfn, 3, d.SF, out) ? -1 : 3 * d.SF;
   }
   return -1;
+}
+
+int subfv_set_arith(subfv_ctx *c, int32_t mode) {
+  if (!c) return SUBFV_E_INVALID;
+  if (mode != SUBFV_ARITH_EXACT && mode != SUBFV_ARITH_FAST)
+    return fail(c, SUBFV_E_INVALID, "arithmetic mode: SUBFV_ARITH_EXACT or SUBFV_ARITH_FAST");
+  c->arith = mode;
+  c->gen++;
+  return 0;
 }
 
 int subfv_set_config(subfv_ctx *c, const subfv_config *cfg) {

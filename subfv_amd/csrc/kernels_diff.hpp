@@ -471,7 +471,7 @@ constexpr int kDiffWaves = 4;
 // slot-term data staged in LDS per vertex: sb and blcs (12 ns), snsh, sbh and
 // b1 (ns^2 + 2 ns), the diagonal 3x3 blocks of S~^T N^-1 S (9 ns); vertices
 // with more sub-elements read them from the block in HBM instead
-constexpr int kDiffStage = 320;
+constexpr int kDiffStage = 240;
 
 template <int MS>
 struct DiffApplyLds {
@@ -596,7 +596,7 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, LdsT &L, int lane, int ns, int s
 //     and the vertex's block the wavefront has just streamed.
 // MS: the size class; WAVES: vertices (wavefronts) per workgroup.
 template <int MS, int WAVES>
-__global__ __launch_bounds__(64 * WAVES, 6) void k_diff_apply(DiffArgs a) {
+__global__ __launch_bounds__(64 * WAVES, 7) void k_diff_apply(DiffArgs a) {
   __shared__ DiffApplyLds<MS> lds_all[WAVES];
   const DevMesh &m = a.m;
   // the wavefront's index is uniform: the vertex's offsets come through the

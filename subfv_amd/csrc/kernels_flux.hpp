@@ -386,7 +386,7 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
 
 // W[6 c + k] = conserv_to_primit(U_c)(k), W[6 c + 5] = sound_speed_w of it:
 // the cell values every sub-face of the sweeps would otherwise recompute
-__global__ __launch_bounds__(256) void k_prim(long C, const double *__restrict__ U, double *__restrict__ W) {
+static __global__ __launch_bounds__(256) void k_prim(long C, const double *__restrict__ U, double *__restrict__ W) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   double u[5], w[5];
