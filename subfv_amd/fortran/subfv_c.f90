@@ -109,6 +109,12 @@ module subfv_c
       type(c_ptr), value :: ctx
       type(subfv_config), intent(in) :: cfg
     end function
+    ! the node sweep's arithmetic: 0 exact (bit-identical), 1 fast (within 1e-12)
+    integer(c_int) function subfv_set_arith(ctx, mode) bind(C, name="subfv_set_arith")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+    end function
     integer(c_int) function subfv_set_time(ctx, t) bind(C, name="subfv_set_time")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
