@@ -171,7 +171,7 @@ def served_bench(n_req=192, steps=300, rate=1000.0, calib_n=24, cmin=32 ** 2, cm
     reqs = S.prepare(S.make_requests(n_req, rate=rate, steps=steps, cmin=cmin, cmax=cmax))
     calib = S.prepare(S.make_requests(calib_n, rate=1e9, steps=30, seed=S.SEED + 1, cmin=cmin, cmax=cmax))
     note(f"{n_req} requests generated; autotuning on {calib_n}")
-    best, trials = S.autotune(calib)
+    best, trials = S.autotune(calib, log=lambda t: note(f"trial {t}"))
     note(f"autotuned: {best}")
     # contexts are created before the timed runs, as a server holds them
     one = Solver(0)
@@ -302,6 +302,21 @@ def main():
     checks["ok"] = checks["finite"] and checks["mass_drift_rel"] <= checks["mass_drift_bound"]
 
     value = C * stages * args.steps / elapsed
+    # the same steps again in the other node-sweep arithmetic (exact <-> fast),
+    # continuing from the timed state: reported beside `value`, not in it
+    other = "fast" if args.arith == "exact" else "exact"
+    s.set_arith(other == "fast")
+    s.run(1, scheme)
+    s.profile(True)
+    t2 = time.perf_counter()
+    s.run(args.steps, scheme)
+    el2 = time.perf_counter() - t2
+    sw2 = [s.profile_read(p) for p in ("node_flux", "node_flux_o2")]
+    s.profile(False)
+    s.set_arith(args.arith == "fast")
+    other_arith = {"node_sweep_arith": other, "value": C * stages * args.steps / el2,
+                   "us_per_stage": 1e6 * el2 / (stages * args.steps),
+                   "node_sweep_us_per_launch": 1e3 * sum(m for m, _ in sw2) / max(1, sum(c for _, c in sw2))}
     # node sweep k_node_flux, launched once per stage for the first-order
     # residual and, in second order, once more for the reconstructed one; its
     # roofline averages over both launch kinds
@@ -362,18 +377,22 @@ def main():
             "m1_frac_of_peak": m1 / (stage_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
             "phases_us_avg": {k: v["us_avg"] for k, v in phases.items()},
         },
+        "other_arith": other_arith,
         "t_final": tf,
         "checks": checks,
     }
     s.close()
+    print(f"# timed stage run: {1e6 * elapsed / (stages * args.steps):.1f} us per stage", file=sys.stderr, flush=True)
     if args.served_requests > 0 and not args.first_order:
         out["served"] = served_bench(args.served_requests)
         # the same loop on small meshes (32^2 .. 96^2 cells), where one solve
         # alone cannot fill the GPU and launches dominate the plain path
         out["served_small"] = served_bench(2 * args.served_requests, rate=4000.0, calib_n=96, cmax=96 ** 2)
     if args.bcsr and not args.first_order:
+        print("# implicit low-Mach GMRES", file=sys.stderr, flush=True)
         out["bcsr_gmres"] = bcsr_bench()
     if not args.no_cpu_baseline:
+        print("# CPU baseline (oracle, one core)", file=sys.stderr, flush=True)
         # samples of about 10-15 s of single-core work
         if args.first_order:
             out["cpu_baseline"] = cpu_baseline(args.cpu_n, steps=40, viscous=False, first_order=True)

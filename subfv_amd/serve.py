@@ -338,32 +338,50 @@ def summarize(results: list[Result], makespan: float) -> dict:
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
 
 
-def autotune(sample: list[Request], device: int = 0, candidates=None,
-             slack: float = 0.95) -> tuple[Params, list]:
-    """Times each candidate on the sample (all requests available at once:
+def autotune(sample: list[Request], device: int = 0, candidates=None, slack: float = 0.95,
+             log=None) -> tuple[Params, list]:
+    """Times candidates on the sample (all requests available at once:
     latency = finish time) and returns, among the candidates within `slack`
-    of the best throughput, the one with the lowest median latency, with
-    the table of trials."""
-    if candidates is None:
-        # launch form: per-step launches, replayed graph, or the persistent
-        # kernel with all the workgroups that fit or two per CU
-        forms = [dict(use_graph=False), dict(use_graph=True), dict(persistent=True),
-                 dict(persistent=True, wgs_per_cu=2)]
-        candidates = [Params(chunk=c, max_batch=256, max_cells=mc, connectivity=cn, **f)
-                      for c in (30, 60) for f in forms for mc in (1 << 20, 4 << 20, 1 << 40)
-                      for cn in ("device", "host")]
+    of the best throughput, the one with the lowest median latency, with the
+    table of trials.  Without `candidates`, two rounds: the launch form
+    (per-step launches, replayed graph, the persistent kernel with all the
+    workgroups that fit or two per CU) x where connectivity is built, then
+    the chunk length x the cells cap for the best of those.  `log(trial)` is
+    called after each trial."""
+    cap = max(p.max_batch + p.prefetch for p in candidates) if candidates is not None else 256 + Params().prefetch
+    pool = [Solver(device) for _ in range(cap)]
     trials = []
-    pool = [Solver(device) for _ in range(max(p.max_batch + p.prefetch for p in candidates))]
-    for p in candidates:
+
+    def trial(p):
         res, ms = serve(sample, p, device, realtime=False, pool=pool)
         lat = np.array([r.latency for r in res])
-        trials.append({"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
-                       "persistent": p.persistent, "wgs_per_cu": p.wgs_per_cu,
-                       "max_cells": p.max_cells, "connectivity": p.connectivity,
-                       "solves_per_s": len(sample) / ms, "p50_s": float(np.percentile(lat, 50))})
-    for s in pool:
-        s.close()
-    top = max(t["solves_per_s"] for t in trials)
-    ok = [i for i, t in enumerate(trials) if t["solves_per_s"] >= slack * top]
-    best = min(ok, key=lambda i: (trials[i]["p50_s"], -trials[i]["solves_per_s"]))
-    return candidates[best], trials
+        t = {"chunk": p.chunk, "max_batch": p.max_batch, "use_graph": p.use_graph,
+             "persistent": p.persistent, "wgs_per_cu": p.wgs_per_cu,
+             "max_cells": p.max_cells, "connectivity": p.connectivity,
+             "solves_per_s": len(sample) / ms, "p50_s": float(np.percentile(lat, 50))}
+        trials.append(t)
+        if log is not None:
+            log(t)
+        return t
+
+    def pick(cands, ts):
+        top = max(t["solves_per_s"] for t in ts)
+        ok = [i for i, t in enumerate(ts) if t["solves_per_s"] >= slack * top]
+        return cands[min(ok, key=lambda i: (ts[i]["p50_s"], -ts[i]["solves_per_s"]))]
+
+    try:
+        if candidates is not None:
+            best = pick(candidates, [trial(p) for p in candidates])
+        else:
+            forms = [dict(use_graph=False), dict(use_graph=True), dict(persistent=True),
+                     dict(persistent=True, wgs_per_cu=2)]
+            first = [Params(chunk=30, max_batch=256, connectivity=cn, **f) for f in forms for cn in ("device", "host")]
+            b = pick(first, [trial(p) for p in first])
+            second = [Params(chunk=c, max_batch=256, max_cells=mc, connectivity=b.connectivity, use_graph=b.use_graph,
+                             persistent=b.persistent, wgs_per_cu=b.wgs_per_cu)
+                      for c in (30, 60) for mc in (1 << 20, 4 << 20, 1 << 40)]
+            best = pick(second, [trial(p) for p in second])
+    finally:
+        for s in pool:
+            s.close()
+    return best, trials
