@@ -14,8 +14,9 @@ the context's stream during the timed region and divides its algorithmic
 bytes per launch (mesh/state bytes it must read, intermediates excluded) by
 the average launch time: the viscous apply k_diff_apply when the viscous
 terms are on (``roofline_node_sweep`` then gives the node sweep's), the node
-sweep k_node_flux otherwise.  The CPU baseline times the oracle (single core)
-on a bounded sample of the same workload.
+sweep k_node_flux otherwise.  The CPU baseline times the oracle on the host
+cores (one process per core of the GPU's CPU share) on bounded samples of the
+same workload.
 """
 from __future__ import annotations
 
@@ -77,10 +78,10 @@ def read_traffic(kernel_tag):
     return read_value(f"pmc_{kernel_tag}", "traffic_bytes_per_launch")
 
 
-def cpu_baseline(n=192, steps=1, viscous=True, first_order=False):
-    """Oracle (single core) on an n x n tri mesh with the timed workload's
-    scheme: second order, SSP-RK3 and the same viscous terms, or the
-    inviscid first-order forward-Euler stage (`steps` stages then)."""
+def _cpu_sample(job):
+    """One process of the CPU baseline: the oracle on its own n x n tri mesh;
+    returns (cells, wall-clock start, end, status) of the timed run."""
+    n, steps, viscous, first_order = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -90,19 +91,39 @@ def cpu_baseline(n=192, steps=1, viscous=True, first_order=False):
     sol = mg.vortex_state(om.elem_coord)
     cfg = O.make_config("multi_point", second_order=not first_order, method=4, cfl=0.4,
                         diffusion=viscous and not first_order, mu=1e-3, prandtl=0.72)
-    stages = 1 if first_order else 3
-    t0 = time.perf_counter()
+    t0 = time.time()
     st, _, _ = om.run(cfg, sol, steps, 0 if first_order else 1)
-    dt = time.perf_counter() - t0
-    if st:
-        raise RuntimeError(f"oracle status {st}")
+    return om.n_elems, t0, time.time(), st
+
+
+def cpu_baseline(n=192, steps=1, viscous=True, first_order=False, procs=None):
+    """Oracle on the host cores: `procs` processes (the CPU share of one GPU,
+    16, or fewer if the process may use fewer cores), each on its own n x n
+    tri mesh with the timed workload's scheme (second order, SSP-RK3 and the
+    same viscous terms, or the inviscid first-order forward-Euler stage);
+    value = all processes' cell updates / the wall time from the first start
+    to the last end.  Run before anything touches the GPU (the processes are
+    started fresh)."""
+    import multiprocessing
+
+    if procs is None:
+        procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    stages = 1 if first_order else 3
+    with multiprocessing.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_cpu_sample, [(n, steps, viscous, first_order)] * procs)
+    if any(r[3] for r in res):
+        raise RuntimeError(f"oracle status {[r[3] for r in res]}")
+    wall = max(r[2] for r in res) - min(r[1] for r in res)
+    cells = sum(r[0] for r in res)
+    one = sorted(r[0] * stages * steps / (r[2] - r[1]) for r in res)[len(res) // 2]
     if first_order:
         what = f"{steps} forward-Euler step(s), first order, multi_point, inviscid"
     else:
         what = (f"{steps} SSP-RK3 step(s), second order, multi_point"
                 + (", viscous mu=1e-3 Pr=0.72" if viscous else ", inviscid"))
-    return {"value": om.n_elems * stages * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"tri {n}x{n} ({om.n_elems} prisms), " + what}
+    return {"value": cells * stages * steps / wall, "unit": "cell-updates/s", "cores": procs, "kind": "port",
+            "per_core_value": one,
+            "sample": f"{procs} processes, each tri {n}x{n} ({res[0][0]} prisms), {what}; wall {wall:.1f} s"}
 
 
 def bcsr_bench(n=512, restart=30):
@@ -253,6 +274,15 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}), flush=True)
         return
 
+    cpu = None
+    if not args.no_cpu_baseline:
+        # before anything touches the GPU: the baseline's processes start fresh
+        print("# CPU baseline (oracle on the host cores)", file=sys.stderr, flush=True)
+        if args.first_order:
+            cpu = cpu_baseline(args.cpu_n, steps=40, viscous=False, first_order=True)
+        else:
+            cpu = cpu_baseline(args.cpu_n, steps=2, viscous=not args.inviscid)
+
     from subfv_amd import meshgen as mg
     from subfv_amd.api import Solver, make_config
 
@@ -391,13 +421,9 @@ def main():
     if args.bcsr and not args.first_order:
         print("# implicit low-Mach GMRES", file=sys.stderr, flush=True)
         out["bcsr_gmres"] = bcsr_bench()
-    if not args.no_cpu_baseline:
-        print("# CPU baseline (oracle, one core)", file=sys.stderr, flush=True)
-        # samples of about 10-15 s of single-core work
-        if args.first_order:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_n, steps=40, viscous=False, first_order=True)
-        else:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_n, steps=2, viscous=viscous)
+    if cpu is not None:
+        # samples of about 10-15 s of work per process
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     if not checks["ok"]:
         print(f"bench: timed run failed its checks: {checks}", file=sys.stderr, flush=True)

@@ -450,8 +450,8 @@ int upload_mesh(subfv_ctx *c) {
   const long C = h.ne, V = h.nv, SE = h.nse;
   for (int i = 0; i < 3; i++)
     if ((r = dalloc(c, &c->U[i], 5 * C))) return r;
-  if ((r = dalloc(c, &c->P, 6 * SE))) return r;
-  if ((r = dalloc(c, &c->P2, 6 * SE))) return r;
+  if ((r = dalloc(c, &c->P, (size_t)kSlotStride * SE))) return r;
+  if ((r = dalloc(c, &c->P2, (size_t)kSlotStride * SE))) return r;
   if ((r = dalloc(c, &c->rhs, 5 * C))) return r;
   if ((r = dalloc(c, &c->rhs2, 5 * C))) return r;
   if ((r = dalloc(c, &c->sl, C))) return r;

@@ -16,6 +16,10 @@ constexpr int kGrpSlots = 256;  // sub-element slots per group at most
 constexpr int kGrpHdr = 8;      // ints per group header: vb, nv, sf0, nsf, se0, nse, -, -
 constexpr int kSlotSrc = 4;     // sub-faces of one cell meeting at a vertex (pyramid apex: 4)
 constexpr int kRedStride = 257; // padded LDS row (doubles) against bank conflicts
+// node-sorted slot rows of the node sweep's partials: 6 values in an 8-double
+// (64-byte, aligned) row, so the gather's scattered row reads touch one
+// 128-byte line each instead of straddling two
+constexpr int kSlotStride = 8;
 
 struct DevMesh {
   long V = 0, C = 0, F = 0, SF = 0, SE = 0;

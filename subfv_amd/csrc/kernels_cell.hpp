@@ -145,7 +145,7 @@ SFV_DEV void gather_block(const GatherArgs &a, long bidx) {
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double lam = 1e-12;
     for (int j = 0; j < nv; j++) {
-      const double *p = a.P + 6L * m.c_slot[b + j];
+      const double *p = a.P + (long)kSlotStride * m.c_slot[b + j];
       for (int k = 0; k < 5; k++) r[k] = r[k] - p[k];
       lam = lam + p[5];
     }
@@ -154,7 +154,7 @@ SFV_DEV void gather_block(const GatherArgs &a, long bidx) {
     if (a.P2) {
       double r2[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       for (int j = 0; j < nv; j++) {
-        const double *p = a.P2 + 6L * m.c_slot[b + j];
+        const double *p = a.P2 + (long)kSlotStride * m.c_slot[b + j];
         for (int k = 0; k < 5; k++) r2[k] = r2[k] - p[k];
       }
       for (int k = 0; k < 5; k++) a.rhs2[k * C + c] = r2[k];

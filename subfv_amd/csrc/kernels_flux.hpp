@@ -378,7 +378,7 @@ SFV_DEV void node_flux_group(const FluxArgs &a, const double *__restrict__ U, in
 #pragma unroll
       for (int c = 0; c < 6; c++) acc[c] = acc[c] + L.red[r0 + c][j];
     }
-    double *out = a.P + 6L * (se0 + tid);
+    double *out = a.P + (long)kSlotStride * (se0 + tid);
 #pragma unroll
     for (int c = 0; c < 6; c++) out[c] = acc[c];
   }
