@@ -824,7 +824,7 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
   const int waves = c->diff_large ? 1 : kDiffWaves;
   a.vblocks = (int)nblk(c->dm.V, waves);
   a.vper_xcd = (a.vblocks + 7) / 8;
-  a.cblocks = (int)nblk(c->dm.C, 256);
+  a.cblocks = (int)nblk(c->dm.C, kDgCells);
   a.cper_xcd = (a.cblocks + 7) / 8;
   {
     Prof pa(c, "diff_apply");

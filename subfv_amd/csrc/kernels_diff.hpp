@@ -678,28 +678,44 @@ __global__ __launch_bounds__(64 * WAVES, 7) void k_diff_apply(DiffArgs a) {
     diff_slot_terms<false>(a, L, lane, ns, se0, E, sns, sb, blcs, snsh);
 }
 
-// rdiff and diag of each cell: its slots' rows in increasing vertex id
+// rdiff and diag of each cell: its slots' rows in increasing vertex id.  One
+// lane per (cell, row entry), kDgCells cells per workgroup, so the 18 entries
+// of a row are read by consecutive lanes; the sums are then assembled from
+// LDS into rdiff (4 per cell) and diag (25 per cell), written by consecutive
+// lanes too.
+constexpr int kDgCells = 14;  // 14 x 18 = 252 of 256 lanes
 __global__ __launch_bounds__(256) void k_diff_gather(DiffArgs a) {
+  __shared__ double s[kDgCells][kDiffSlot];
   const DevMesh &m = a.m;
   const int lb = xcd_block(a.cper_xcd);
-  if (lb >= a.cblocks) return;
-  const long c = (long)lb * blockDim.x + threadIdx.x;
-  if (c >= m.C) return;
-  const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
-  double r[4] = {0, 0, 0, 0}, d[14];
-  for (int k = 0; k < 14; k++) d[k] = 0.0;
-  for (int j = 0; j < nv; j++) {
-    const double *p = a.PD + (long)kDiffSlot * m.c_slot[b + j];
-    for (int k = 0; k < 4; k++) r[k] = r[k] + p[k];
-    for (int k = 0; k < 14; k++) d[k] = d[k] + p[4 + k];
+  if (lb >= a.cblocks) return;  // uniform over the workgroup
+  const int t = threadIdx.x, cl = t / kDiffSlot, k = t - cl * kDiffSlot;
+  const long c0 = (long)lb * kDgCells;
+  if (cl < kDgCells && c0 + cl < m.C) {
+    const long c = c0 + cl;
+    const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
+    double acc = 0.0;
+    for (int j = 0; j < nv; j++) acc = acc + a.PD[(long)kDiffSlot * m.c_slot[b + j] + k];
+    s[cl][k] = acc;
   }
-  for (int k = 0; k < 4; k++) a.rdiff[4 * c + k] = r[k];
-  double D[25];
-  for (int q = 0; q < 25; q++) D[q] = 0.0;
-  for (int j = 0; j < 3; j++)
-    for (int i = 0; i < 3; i++) D[(1 + i) + 5 * (1 + j)] = d[i + 3 * j];
-  for (int k = 0; k < 5; k++) D[4 + 5 * k] = D[4 + 5 * k] + d[9 + k];
-  for (int q = 0; q < 25; q++) a.diag[25 * c + q] = D[q];
+  __syncthreads();
+  // [0..3] rhs of rho u, rho v, rho w, E; [4..12] diag(2:4, 2:4) column-major;
+  // [13..17] diag(5, 1:5), added to zero as the 5x5 block is formed
+  for (int q = t; q < kDgCells * 29; q += 256) {
+    const int cc = q / 29, o = q - cc * 29;
+    const long c = c0 + cc;
+    if (c >= m.C) continue;
+    const double *d = s[cc];
+    if (o < 4) {
+      a.rdiff[4 * c + o] = d[o];
+    } else {
+      const int e = o - 4, row = e % 5, col = e / 5;
+      double x = 0.0;
+      if (row >= 1 && row <= 3 && col >= 1 && col <= 3) x = d[4 + (row - 1) + 3 * (col - 1)];
+      if (row == 4) x = 0.0 + d[13 + col];
+      a.diag[25 * c + e] = x;
+    }
+  }
 }
 
 }  // namespace sfv
