@@ -480,6 +480,21 @@ int om_add_diffusion(const om_mesh *m, const om_config *cfg, const double *sol, 
   return OM_OK;
 }
 
+/* the two builders for given cell viscosities mu_arr[0..ne) and mu0corr:
+ * parts 1 build_momentum_diff_mat_into_global_mat_diag, 2
+ * build_diff_mat_into_global_mat_diag, 3 both (momentum first) */
+int om_add_diffusion_mu(const om_mesh *m, const om_config *cfg, const double *sol, const double *mu_cells,
+                        double mu0corr, int parts, double *rhs, double *diag) {
+  double *mu_arr = (double *)malloc(sizeof(double) * ((size_t)m->ne + 1));
+  if (!mu_arr) return OM_ERR_BAD_INPUT;
+  mu_arr[0] = 0.0;
+  memcpy(mu_arr + 1, mu_cells, sizeof(double) * (size_t)m->ne);
+  if (parts & 1) mom_into_global(m, cfg, mu_arr, mu0corr, sol, rhs, diag);
+  if (parts & 2) heat_into_global(m, cfg, mu_arr, sol, rhs, diag);
+  free(mu_arr);
+  return OM_OK;
+}
+
 int om_diffusion_available(void) { return 1; }
 
 /* test hook: temp_u with Cv_p */

@@ -123,5 +123,7 @@ void *om_arena_alloc(om_mesh *m, size_t bytes);
 
 /* viscous block of compute_rhs_ns (om_diffusion.c) */
 int om_add_diffusion(const om_mesh *m, const om_config *cfg, const double *sol, double *rhs, double *diag);
+int om_add_diffusion_mu(const om_mesh *m, const om_config *cfg, const double *sol, const double *mu_cells,
+                        double mu0corr, int parts, double *rhs, double *diag);
 
 #endif

@@ -159,6 +159,7 @@ def lib():
                                          C.c_int, ip, ip, ip, dp]
         L.om_bcsr_pcgm.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, dp]
         L.om_add_diffusion.argtypes = [P, cp, dp, dp, dp]
+        L.om_add_diffusion_mu.argtypes = [P, cp, dp, dp, C.c_double, C.c_int, dp, dp]
         L.om_compute_mu.restype = C.c_double
         L.om_compute_mu.argtypes = [C.c_int, cp, dp, dp]
         L.om_test_inv.argtypes = [C.c_int, dp, dp]
@@ -303,6 +304,23 @@ class OracleMesh:
         st = lib().om_add_diffusion(self.h, C.byref(cfg), sol.ravel(), rhs.reshape(-1), diag.reshape(-1))
         if st:
             raise RuntimeError(f"oracle add_diffusion status {st}")
+        return rhs, diag
+
+    def add_diffusion_mu(self, cfg: Config, sol: np.ndarray, mu_arr, mu0corr: float, parts: int = 3,
+                         rhs=None, diag=None):
+        """build_momentum_diff_mat_into_global_mat_diag (parts 1) and
+        build_diff_mat_into_global_mat_diag (parts 2) for given cell
+        viscosities mu_arr (n_elems) and mu0corr, added to rhs and diag in the
+        layouts of add_diffusion."""
+        ne = self.n_elems
+        sol = np.ascontiguousarray(sol, np.float64).reshape(ne, 5)
+        mu = np.ascontiguousarray(mu_arr, np.float64).reshape(ne)
+        rhs = np.zeros((ne, 5)) if rhs is None else np.ascontiguousarray(rhs, np.float64).copy()
+        diag = np.zeros((ne, 5, 5)) if diag is None else np.ascontiguousarray(diag, np.float64).copy()
+        st = lib().om_add_diffusion_mu(self.h, C.byref(cfg), sol.ravel(), mu, float(mu0corr), int(parts),
+                                       rhs.reshape(-1), diag.reshape(-1))
+        if st:
+            raise RuntimeError(f"oracle add_diffusion_mu status {st}")
         return rhs, diag
 
     def compute_cell_grad(self, sol: np.ndarray, method: int = 4):

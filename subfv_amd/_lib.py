@@ -121,6 +121,8 @@ SIGNATURES = {
     "subfv_compute_rhs_ns": (C.c_int, [_P]),
     "subfv_set_diffusion": (C.c_int, [_P, _i32, C.c_double, C.c_double, C.c_double, _i32, _ip, _dp, _ip, _dp]),
     "subfv_set_sutherland": (C.c_int, [_P, _i32, C.c_double, C.c_double]),
+    "subfv_compute_mu": (C.c_int, [_P, _dp, _dp, _dp]),
+    "subfv_add_diffusion": (C.c_int, [_P, _dp, _dp, C.c_double, _i32, _dp, _dp]),
     "subfv_run": (C.c_int, [_P, _i32, _i32, _dp, _dp]),
     "subfv_profile_enable": (C.c_int, [_P, _i32]),
     "subfv_profile_read": (C.c_int, [_P, C.c_char_p, _dp, C.POINTER(C.c_int64)]),

@@ -257,6 +257,32 @@ class Solver:
         temperature (ns_vectorial_diffusion_module.F90:322-359)."""
         self._check(_lib.lib().subfv_set_sutherland(self.h, int(activate), mu0, T0))
 
+    def compute_mu(self, sol):
+        """compute_mu (ns_vectorial_diffusion_module.F90:322-347): (mu_arr
+        (n_elems), mu0corr) of the state sol (n_elems, 5)."""
+        sol = _lib.f64(sol).reshape(self.n_elems, 5)
+        mu = np.zeros(self.n_elems)
+        m0 = np.zeros(1)
+        self._check(_lib.lib().subfv_compute_mu(self.h, _lib.dptr(sol), _lib.dptr(mu), _lib.dptr(m0)))
+        return mu, float(m0[0])
+
+    def add_diffusion(self, sol, mu_arr, mu0corr: float, parts: int = 3, rhs=None, diag=None):
+        """build_momentum_diff_mat_into_global_mat_diag (parts 1,
+        ns_vectorial_diffusion_module.F90:200-320) and
+        build_diff_mat_into_global_mat_diag (parts 2,
+        ns_diffusion_module.F90:112-184) for the state sol (n_elems, 5) and the
+        cell viscosities mu_arr: (rhs, diag) with their contributions added to
+        the given rhs (n_elems, 5) and diag (n_elems, 5, 5 column-major blocks),
+        zeros if not given."""
+        C_ = self.n_elems
+        sol = _lib.f64(sol).reshape(C_, 5)
+        mu = _lib.f64(mu_arr).reshape(C_)
+        rhs = np.zeros((C_, 5)) if rhs is None else np.array(rhs, np.float64).reshape(C_, 5)
+        diag = np.zeros((C_, 5, 5)) if diag is None else np.array(diag, np.float64).reshape(C_, 5, 5)
+        self._check(_lib.lib().subfv_add_diffusion(self.h, _lib.dptr(sol), _lib.dptr(mu), float(mu0corr),
+                                                   int(parts), _lib.dptr(rhs), _lib.dptr(diag)))
+        return rhs, diag
+
     def compute_rhs_ns(self):
         self._check(_lib.lib().subfv_compute_rhs_ns(self.h))
         return self.get("rhs"), self.get("sum_lambda")

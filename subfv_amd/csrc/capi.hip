@@ -795,10 +795,16 @@ __global__ __launch_bounds__(kCellThreads) void k_mu_min(const double *blk, int 
 
 // rdiff and diag of the current state (build_momentum_diff_mat_into_global_mat_diag
 // and build_diff_mat_into_global_mat_diag)
-int launch_diffusion(subfv_ctx *c, const double *U) {
+// rdiff and diag of the state U.  mu_cell / mu_min: given cell viscosities
+// and their mu0corr (device), for which the nodal systems are formed anew
+// (the constant-viscosity blocks are then invalidated); parts: 1 momentum,
+// 2 heat, 3 both.
+int launch_diffusion(subfv_ctx *c, const double *U, const double *mu_cell = nullptr,
+                     const double *mu_min = nullptr, int parts = 3) {
   int r = ensure_diffusion(c);
   if (r) return r;
   DiffArgs a{};
+  a.parts = parts;
   a.m = c->dm;
   a.p = c->dparams;
   a.doff = c->doff;
@@ -810,7 +816,12 @@ int launch_diffusion(subfv_ctx *c, const double *U) {
   a.slot_ssf = c->dssf;
   a.slot_vol = c->dsvol;
   Prof pf(c, "diffusion");
-  if (c->dparams.suth) {
+  if (mu_cell) {
+    a.mu_cell = mu_cell;
+    a.mu_min = mu_min;
+    launch_diff_setup(c, a);
+    if (!c->dparams.suth) c->diff_ready = false;  // the cached constant-viscosity blocks are overwritten
+  } else if (c->dparams.suth) {
     Prof ps(c, "diff_setup");
     const unsigned nb = nblk(c->dm.C, kCellThreads);
     hipLaunchKernelGGL(k_compute_mu, dim3(nb), dim3(kCellThreads), 0, c->stream, (long)c->dm.C, U, c->dparams.cv,
@@ -1612,6 +1623,83 @@ int subfv_set_sutherland(subfv_ctx *c, int32_t activate, double mu0, double T0) 
     c->dparams = p;
     c->diff_ready = false;
   }
+  return 0;
+}
+
+// the cell viscosity buffers (and, past the blocks' minima, mu0corr), once per mesh
+int ensure_mu_buffers(subfv_ctx *c) {
+  int r;
+  if (!c->mu_cell && (r = dalloc(c, &c->mu_cell, (size_t)c->dm.C))) return r;
+  if (!c->mu_blk && (r = dalloc(c, &c->mu_blk, (size_t)nblk(c->dm.C, kCellThreads) + 1))) return r;
+  return 0;
+}
+
+int subfv_compute_mu(subfv_ctx *c, const double *sol, double *mu_arr, double *mu0corr) {
+  if (!c || !c->has_mesh || !sol || !mu_arr || !mu0corr) return fail(c, SUBFV_E_INVALID, "no mesh or null array");
+  if (!c->diff_on) return fail(c, SUBFV_E_INVALID, "compute_mu: viscous terms not activated (subfv_set_diffusion)");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long C = c->dm.C;
+  int r;
+  if (!c->dparams.suth) {  // mu_arr = mu_p, mu0corr = mu_p (:344-345)
+    for (long e = 0; e < C; e++) mu_arr[e] = c->dparams.mu;
+    *mu0corr = c->dparams.mu;
+    return 0;
+  }
+  if ((r = subfv_state_put(c, sol))) return r;
+  if ((r = ensure_mu_buffers(c))) return r;
+  const unsigned nb = nblk(C, kCellThreads);
+  hipLaunchKernelGGL(k_compute_mu, dim3(nb), dim3(kCellThreads), 0, c->stream, C, (const double *)c->U[c->cur],
+                     c->dparams.cv, c->dparams.suth_mu0, c->dparams.suth_T0, c->mu_cell, c->mu_blk);
+  hipLaunchKernelGGL(k_mu_min, dim3(1), dim3(kCellThreads), 0, c->stream, (const double *)c->mu_blk, (int)nb,
+                     c->mu_blk + nb);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(mu_arr, c->mu_cell, C * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(mu0corr, c->mu_blk + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"
+
+// rhs(2:5, c) += rdiff(1:4, c) and diag(:, :, c) += vdiag(:, :, c) in the
+// caller's layouts sol(5, C) and diag(5, 5, C)
+__global__ void k_add_diffusion(long C, double *rhs, double *diag, const double *rdiff, const double *vdiag) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  for (int k = 0; k < 4; k++) rhs[5 * c + 1 + k] = rhs[5 * c + 1 + k] + rdiff[4 * c + k];
+  for (int q = 0; q < 25; q++) diag[25 * c + q] = diag[25 * c + q] + vdiag[25 * c + q];
+}
+
+extern "C" {
+
+int subfv_add_diffusion(subfv_ctx *c, const double *sol, const double *mu_arr, double mu0corr, int32_t parts,
+                        double *rhs, double *diag) {
+  if (!c || !c->has_mesh || !sol || !mu_arr || !rhs || !diag) return fail(c, SUBFV_E_INVALID, "no mesh or null array");
+  if (parts < 1 || parts > 3) return fail(c, SUBFV_E_INVALID, "parts: 1 momentum, 2 heat, 3 both");
+  if (!c->diff_on) return fail(c, SUBFV_E_INVALID, "viscous terms not activated (subfv_set_diffusion)");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long C = c->dm.C;
+  int r;
+  if ((r = subfv_state_put(c, sol))) return r;
+  if ((r = ensure_diffusion(c))) return r;
+  if ((r = ensure_mu_buffers(c))) return r;
+  const unsigned nb = nblk(C, kCellThreads);
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->mu_cell, mu_arr, C * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->mu_blk + nb, &mu0corr, sizeof(double), hipMemcpyHostToDevice, s));
+  if ((r = launch_diffusion(c, c->U[c->cur], c->mu_cell, c->mu_blk + nb, parts))) return r;
+  Scratch sc;
+  double *drhs, *ddiag;
+  HIPCHK(c, sc.get(&drhs, 5 * C));
+  HIPCHK(c, sc.get(&ddiag, 25 * C));
+  HIPCHK(c, hipMemcpyAsync(drhs, rhs, 5 * C * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(ddiag, diag, 25 * C * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_add_diffusion, dim3(nblk(C, 256)), dim3(256), 0, s, C, drhs, ddiag, (const double *)c->rdiff,
+                     (const double *)c->diag);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(rhs, drhs, 5 * C * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(diag, ddiag, 25 * C * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
   return 0;
 }
 

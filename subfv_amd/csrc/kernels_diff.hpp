@@ -92,6 +92,7 @@ struct DiffArgs {
   const double *mu_min;   // mu0corr, their minimum (Sutherland)
   int vper_xcd, vblocks;  // XCD slices of the vertex blocks (apply)
   int cper_xcd, cblocks;  // and of the cell blocks (gather)
+  int parts;              // apply: 1 momentum system, 2 heat system, 3 both
   double *ws;             // setup of the large class: the matrices of each block, ws_stride doubles apart
   long ws_stride;
   int v0;                 // setup: first vertex of the launch
@@ -504,24 +505,27 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, LdsT &L, int lane, int ns, int s
     return STAGED ? L.pc[o_diag + 9 * l + i + 3 * j] : sns[(3 * l + i) + (long)E * (3 * l + j)];
   };
   double *PD = a.PD + (long)kDiffSlot * se0;
+  // the momentum system's terms (build_momentum_diff_mat_into_global_mat_diag)
+  // and the heat system's (build_diff_mat_into_global_mat_diag), or both
+  const bool mom = a.parts & 1, heat = a.parts & 2;
   // diagonal block of the momentum rows, 9 entries per slot
   for (int t = lane; t < 9 * ns; t += 64) {
     const int l = t / 9, ij = t - 9 * l, i = ij % 3, j = ij / 3;
     const double rho = L.rho[l];
-    PD[(long)kDiffSlot * l + 4 + ij] = -sdiag(l, i, j) / rho + c_blcs[9 * l + j + 3 * i] / rho;
+    PD[(long)kDiffSlot * l + 4 + ij] = mom ? -sdiag(l, i, j) / rho + c_blcs[9 * l + j + 3 * i] / rho : 0.0;
   }
   // momentum rhs, 3 per slot
   for (int t = lane; t < 3 * ns; t += 64) {
     const int l = t / 3, i = t - 3 * l;
     double mv = 0.0;
     for (int j = 0; j < 3; j++) mv = mv + sdiag(l, i, j) * L.Ve[3 * l + j];
-    PD[(long)kDiffSlot * l + i] = (L.SVe[3 * l + i] - mv) + c_sb[3 * l + i];
+    PD[(long)kDiffSlot * l + i] = mom ? (L.SVe[3 * l + i] - mv) + c_sb[3 * l + i] : 0.0;
   }
   // heat row of the diagonal, 5 per slot
   for (int t = lane; t < 5 * ns; t += 64) {
     const int l = t / 5, k = t - 5 * l;
     const double sll = c_snsh[l + (long)ns * l];
-    PD[(long)kDiffSlot * l + 13 + k] = (c_b1[l] - sll) * L.P[l][k];
+    PD[(long)kDiffSlot * l + 13 + k] = heat ? (c_b1[l] - sll) * L.P[l][k] : 0.0;
   }
   wsync_d();
   // the viscous stress of each slot's sub-element, entry q = i + 3 j per
@@ -577,7 +581,7 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, LdsT &L, int lane, int ns, int s
     const double sll = c_snsh[l + (long)ns * l];
     const double pu = L.pu[l];
     h = h - sll * pu + (c_b1[l] - sll) * pu - c_b1[l] * L.th[l] + c_sbh[l];
-    PD[(long)kDiffSlot * l + 3] = work + h;
+    PD[(long)kDiffSlot * l + 3] = mom && heat ? work + h : (mom ? work : h);
   }
 }
 

@@ -177,6 +177,22 @@ module subfv_c
       integer(c_int32_t), value :: activate
       real(c_double), value :: mu0, t0
     end function
+    integer(c_int) function subfv_compute_mu(ctx, sol, mu_arr, mu0corr) bind(C, name="subfv_compute_mu")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: sol(*)
+      real(c_double), intent(out) :: mu_arr(*)
+      real(c_double), intent(out) :: mu0corr
+    end function
+    integer(c_int) function subfv_add_diffusion(ctx, sol, mu_arr, mu0corr, parts, rhs, diag) &
+        bind(C, name="subfv_add_diffusion")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: sol(*), mu_arr(*)
+      real(c_double), value :: mu0corr
+      integer(c_int32_t), value :: parts
+      real(c_double), intent(inout) :: rhs(*), diag(*)
+    end function
     integer(c_int) function subfv_set_diffusion(ctx, activate, mu_p, cv_p, prandtl, n_bc, bc_v_id, bc_val_v, &
         bc_t_id, bc_val_t) bind(C, name="subfv_set_diffusion")
       import :: c_ptr, c_int, c_int32_t, c_double

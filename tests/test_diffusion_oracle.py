@@ -144,3 +144,34 @@ def test_shear_layer_loses_kinetic_energy_to_viscosity():
     _, a, _ = om.run(inv_cfg, sol, 10, 0)
     _, b, _ = om.run(vis_cfg, sol, 10, 0)
     assert ke(b) < ke(a)
+
+
+def test_builders_split_compose_to_the_viscous_block():
+    """The two builders for compute_mu's viscosities, momentum then heat on
+    the same arrays, give the viscous block of compute_rhs_ns bit for bit,
+    with and without Sutherland's law; with a zero viscosity of one builder's
+    system only that builder's rows move."""
+    mesh = mg.quad_mesh(7, tags="lr")
+    om = O.OracleMesh(mesh)
+    xc = om.elem_coord
+    rng = np.random.default_rng(2)
+    T = 300.0 + 40.0 * np.sin(2 * np.pi * xc[:, 0]) + rng.random(om.n_elems)
+    w = np.stack([1.0 + 0.1 * np.cos(3 * xc[:, 1]), 20 * np.sin(2 * xc[:, 1]), -10 * np.cos(3 * xc[:, 0]),
+                  0.0 * T, (1.0 + 0.1 * np.cos(3 * xc[:, 1])) * 0.4 * 720.19471 * T], 1)
+    sol = mg.primit_to_conserv(w)
+    for suth in (False, True):
+        cfg = O.make_config("multi_point", diffusion=True, mu=2e-3, sutherland=suth,
+                            bcs_v=[("dirichlet", (0.1, 0.0, 0.0)), ("neumann", (0, 0, 0))],
+                            bcs_t=[("dirichlet", 300.0), ("neumann", 0.0)])
+        mu, m0 = O.compute_mu(cfg, sol)
+        r1, d1 = om.add_diffusion_mu(cfg, sol, mu, m0, 1)
+        r2, d2 = om.add_diffusion_mu(cfg, sol, mu, m0, 2, r1, d1)
+        rf, df = om.add_diffusion(cfg, sol)
+        assert np.array_equal(r2, rf) and np.array_equal(d2, df)
+        r3, d3 = om.add_diffusion_mu(cfg, sol, mu, m0, 3)
+        assert np.array_equal(r3, rf) and np.array_equal(d3, df)
+        # momentum only: the heat row of diag untouched, the energy rhs is the work term
+        assert np.all(d1[:, :, 4] == 0.0) and np.any(d1[:, 1:4, 1:4] != 0.0)
+        rh, dh = om.add_diffusion_mu(cfg, sol, mu, m0, 2)
+        assert np.all(rh[:, :4] == 0.0) and np.all(dh[:, 1:4, 1:4] == 0.0) and np.any(dh[:, :, 4] != 0.0)
+        assert np.allclose(r1[:, 4] + rh[:, 4], rf[:, 4], rtol=1e-13, atol=1e-13 * np.abs(rf[:, 4]).max())

@@ -293,3 +293,87 @@ def test_viscous_parameter_changes_reuse_the_buffers(solver):
     assert solver.count("device_bytes_used") == used
     assert np.array_equal(solver.get("rhs"), first)
     solver.set_diffusion(False)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_reference_builders_for_given_viscosities(solver, name):
+    """build_momentum_diff_mat_into_global_mat_diag (parts 1) and
+    build_diff_mat_into_global_mat_diag (parts 2) for cell viscosities the
+    caller gives (varying by a factor 3), each added to a nonzero rhs and
+    diag as the reference's builders add them, against the oracle's builders;
+    parts 3 is both.  Within 1e-12 of the viscous terms' scale.  A residual
+    afterwards uses the constant viscosity again."""
+    print(f"[gpu] reference builders {name}", flush=True)
+    mk, cfgkw, dkw = CASES[name]
+    mesh = mk()
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    ne = om.n_elems
+    sol = smooth_state(om.elem_coord)
+    rng = np.random.default_rng(11)
+    mu = 2e-3 * (0.5 + rng.random(ne))
+    mu0 = float(mu.min())
+    rhs0 = rng.standard_normal((ne, 5))
+    diag0 = rng.standard_normal((ne, 5, 5))
+    solver.set_config(make_config("multi_point", **cfgkw))
+    solver.set_diffusion(True, mu=2e-3, **dkw)
+    ocfg = O.make_config("multi_point", diffusion=True, mu=2e-3, **cfgkw, **dkw)
+    try:
+        for parts in (1, 2, 3):
+            r, d = solver.add_diffusion(sol, mu, mu0, parts, rhs0, diag0)
+            ro, do = om.add_diffusion_mu(ocfg, sol, mu, mu0, parts, rhs0, diag0)
+            zr, zd = om.add_diffusion_mu(ocfg, sol, mu, mu0, parts)
+            scale_r = np.abs(zr).max() + 1e-300
+            scale_d = np.abs(zd).max() + 1e-300
+            # the terms land in rhs0 and diag0 in another order: within 1e-12
+            # of their scale plus the rounding of those sums
+            assert np.all(np.abs(r - ro) <= 1e-12 * scale_r + 1e-14 * np.abs(ro)), (parts, np.abs(r - ro).max())
+            assert np.all(np.abs(d - do) <= 1e-12 * scale_d + 1e-14 * np.abs(do)), (parts, np.abs(d - do).max())
+            # the rows and blocks a builder does not touch stay as given
+            assert np.array_equal(r[:, 0], rhs0[:, 0])
+            # (blocks column-major: d[c, j, i] is diag(i + 1, j + 1, c + 1))
+            if parts == 1:
+                assert np.array_equal(d[:, :, 4], diag0[:, :, 4])  # the heat row diag(5, :)
+            if parts == 2:
+                assert np.array_equal(r[:, 1:4], rhs0[:, 1:4])
+                assert np.array_equal(d[:, 1:4, 1:4], diag0[:, 1:4, 1:4])
+        # compute_mu without Sutherland: mu_p everywhere
+        m, m0 = solver.compute_mu(sol)
+        assert np.all(m == 2e-3) and m0 == 2e-3
+        # the context's own residual uses mu_p again
+        solver.put_sol(sol)
+        solver.compute_rhs_ns()
+        rd, dd = om.add_diffusion(ocfg, sol)
+        r0, _ = om.compute_euler_flux(ocfg, sol)
+        scale_r = np.abs(rd).max(axis=0) + 1e-300
+        assert np.all(np.abs(solver.get("rhs") - (r0 + rd)) <= 1e-12 * scale_r[None, :] + 1e-14 * np.abs(r0 + rd))
+    finally:
+        solver.set_diffusion(False)
+
+
+def test_compute_mu_sutherland_matches_oracle(solver):
+    """compute_mu with use_sutherland: per-cell viscosities and mu0corr (their
+    minimum); T^1.5 goes through pow on both sides: within 4 ulp."""
+    print("[gpu] compute_mu sutherland", flush=True)
+    mesh = mg.tri_mesh(9)
+    solver.build_mesh(mesh)
+    om = O.OracleMesh(mesh)
+    sol = warm_state(om.elem_coord, 3)
+    solver.set_config(make_config("multi_point"))
+    solver.set_diffusion(True, mu=1.0)
+    solver.set_sutherland(True)
+    try:
+        m, m0 = solver.compute_mu(sol)
+        # the builders on these viscosities: the context's Sutherland residual
+        r, d = solver.add_diffusion(sol, m, m0, 3)
+    finally:
+        solver.set_sutherland(False)
+        solver.set_diffusion(False)
+    ocfg = O.make_config("multi_point", diffusion=True, mu=1.0, sutherland=True)
+    mo, m0o = O.compute_mu(ocfg, sol)
+    assert mo.max() > 1.2 * m0o
+    assert np.abs(m - mo).max() <= 4 * np.spacing(mo).max()
+    assert abs(m0 - m0o) <= 4 * np.spacing(m0o)
+    rd, dd = om.add_diffusion(ocfg, sol)
+    assert np.abs(r - rd).max() <= 1e-12 * np.abs(rd).max()
+    assert np.abs(d - dd).max() <= 1e-12 * np.abs(dd).max()

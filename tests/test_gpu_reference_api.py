@@ -1,11 +1,15 @@
 """The reference-signature Fortran procedures (subfv_amd/fortran/
 subfv_reference_api.f90) driven as the reference's own driver calls them:
-compute_euler_flux (first and second order), compute_cell_grad_from_nodal_grad,
-flux_limiting and add_timedisc_and_solve on one residual and stage update
-(ns_main.F90:130-179, ns_euler_module.F90:521-609), then bcsr_matmul,
-bcsr_diag and bcsr_gmres_diag on a block matrix whose values are refilled in
-place between calls.  Results equal the oracle's bit for bit (GMRES: to
-its tolerance)."""
+read_mesh_msh_2, build_mesh and compute_geometry_mesh (the renumbered
+vertices, elements and boundary faces, the counts and the cell and vertex
+geometry), compute_euler_flux (first and second order),
+compute_cell_grad_from_nodal_grad, flux_limiting and add_timedisc_and_solve
+on one residual and stage update (ns_main.F90:130-179,
+ns_euler_module.F90:521-609), compute_mu and the two viscous builders, then
+bcsr_matmul, bcsr_diag and bcsr_gmres_diag on a block matrix whose values
+are refilled in place between calls.  Results equal the oracle's bit for bit
+(viscous builders: within 1e-12 of their terms' scale; Sutherland's law:
+within 4 ulp; GMRES: to its tolerance)."""
 import os
 import subprocess
 
@@ -36,7 +40,9 @@ def _bcsr(m, b, seed):
 
 def test_reference_signature_procedures(tmp_path):
     print("[gpu] reference-signature Fortran procedures", flush=True)
-    mesh = mg.quad_mesh(12)
+    # boundary faces in the file (all walls: tag 0), so build_mesh renumbers their vertices
+    mesh = mg.quad_mesh(12, tags="lr")
+    mesh.bf_tag[:] = 0
     mg.write_msh(mesh, str(tmp_path / "mesh.msh"))
     m, b = 200, 4
     row, col, v = _bcsr(m, b, 3)
@@ -54,6 +60,8 @@ def test_reference_signature_procedures(tmp_path):
 
     om = O.OracleMesh(mesh)
     ne, nv = om.n_elems, om.n_vert
+    _check_mesh(tmp_path / "mesh.bin", mesh, om)
+    _check_viscous(tmp_path / "viscous.bin", om)
     raw = np.fromfile(tmp_path / "residual.bin", dtype=np.uint8)
     assert tuple(raw[:8].view(np.int32)) == (ne, nv)
     d = raw[8:].view(np.float64)
@@ -92,3 +100,84 @@ def test_reference_signature_procedures(tmp_path):
     r = x - O.bcsr_matmul(m, b, row, col, v2, xg)
     assert np.linalg.norm(r) < tol
     assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def _take(buf, pos, n, dt):
+    size = np.dtype(dt).itemsize * n
+    return buf[pos:pos + size].view(dt).copy(), pos + size
+
+
+def _check_mesh(path, mesh, om):
+    """mesh_type after build_mesh + compute_geometry_mesh: the oracle's
+    renumbered mesh (vertices by renumber_vert_mesh, elements by
+    renumber_elem_mesh, boundary-face vertex ids renumbered in place)."""
+    b = np.fromfile(path, dtype=np.uint8)
+    cnt, p = _take(b, 0, 7, np.int32)
+    nb = int(mesh.bf_nvert.size)
+    assert tuple(cnt) == (om.n_vert, om.n_elems, om.n_faces, om.n_sub_faces, om.n_sub_elems, om.nmax_sfn, nb)
+    nv, ne = om.n_vert, om.n_elems
+    vc, p = _take(b, p, 3 * nv, np.float64)
+    vv, p = _take(b, p, nv, np.float64)
+    vb, p = _take(b, p, nv, np.int32)
+    kind, p = _take(b, p, ne, np.int32)
+    tag, p = _take(b, p, ne, np.int32)
+    ev, p = _take(b, p, int(om.geti("elem_vert").size), np.int32)
+    ec, p = _take(b, p, 3 * ne, np.float64)
+    evol, p = _take(b, p, ne, np.float64)
+    bv, p = _take(b, p, int(mesh.bf_nvert.sum()), np.int32)
+    assert p == b.size
+    assert np.array_equal(vc, om.getd("vert_coord"))
+    assert np.array_equal(vv, om.getd("vert_volume"))
+    assert np.array_equal(vb, om.geti("vert_is_bound"))
+    assert np.array_equal(kind, om.geti("elem_kind"))
+    assert np.array_equal(tag, om.geti("elem_tag"))
+    assert np.array_equal(ev, om.geti("elem_vert"))
+    assert np.array_equal(ec, om.getd("elem_coord"))
+    assert np.array_equal(evol, om.getd("elem_volume"))
+    nto = om.geti("vert_nto")
+    otn = np.empty(nv, np.int32)
+    otn[nto - 1] = np.arange(1, nv + 1)
+    assert np.array_equal(bv, otn[np.asarray(mesh.bf_vert) - 1])
+    # the renumbering moved something, so the comparison is not vacuous
+    assert not np.array_equal(nto, np.arange(1, nv + 1))
+
+
+def _check_viscous(path, om):
+    ne = om.n_elems
+    b = np.fromfile(path, dtype=np.uint8)
+    p = 0
+    mmu, p = _take(b, p, ne, np.float64)
+    m0c, p = _take(b, p, 1, np.float64)
+    mu, p = _take(b, p, ne, np.float64)
+    sv1, p = _take(b, p, 5 * ne, np.float64)
+    rm, p = _take(b, p, 5 * ne, np.float64)
+    dm, p = _take(b, p, 25 * ne, np.float64)
+    r, p = _take(b, p, 5 * ne, np.float64)
+    d, p = _take(b, p, 25 * ne, np.float64)
+    sv, p = _take(b, p, 5 * ne, np.float64)
+    mus, p = _take(b, p, ne, np.float64)
+    m0s, p = _take(b, p, 1, np.float64)
+    assert p == b.size
+    assert np.all(mmu == 2e-3) and m0c[0] == 2e-3
+    cfg = O.make_config("multi_point", bc_style=2, boundary_2d=True, diffusion=True, mu=2e-3)
+    rhs0 = np.ones((ne, 5))
+    diag0 = np.tile(np.eye(5), (ne, 1, 1))
+    sv1 = sv1.reshape(ne, 5)
+    assert mu.max() > 1.3 * mu.min()
+    r1, d1 = om.add_diffusion_mu(cfg, sv1, mu, mu.min(), 1, rhs0, diag0)
+    r2, d2 = om.add_diffusion_mu(cfg, sv1, mu, mu.min(), 2, r1, d1)
+    sr = np.abs(r2 - rhs0).max()
+    sd = np.abs(d2 - diag0).max()
+    assert sr > 0 and sd > 0
+    # the terms are added to rhs and diag in another order: within 1e-12 of
+    # their scale plus the rounding of the sums they land in
+    assert np.all(np.abs(rm.reshape(ne, 5) - r1) <= 1e-12 * sr + 1e-14 * np.abs(r1))
+    assert np.all(np.abs(dm.reshape(ne, 5, 5) - d1) <= 1e-12 * sd + 1e-14 * np.abs(d1))
+    assert np.all(np.abs(r.reshape(ne, 5) - r2) <= 1e-12 * sr + 1e-14 * np.abs(r2))
+    assert np.all(np.abs(d.reshape(ne, 5, 5) - d2) <= 1e-12 * sd + 1e-14 * np.abs(d2))
+    cs = O.make_config("multi_point", bc_style=2, boundary_2d=True, diffusion=True, mu=2e-3, sutherland=True,
+                       mu0=1.716e-5, T0=273.15)
+    mo, m0o = O.compute_mu(cs, sv.reshape(ne, 5))
+    assert mo.max() > 1.05 * m0o
+    assert np.abs(mus - mo).max() <= 4 * np.spacing(mo).max()
+    assert abs(m0s[0] - m0o) <= 4 * np.spacing(m0o)
