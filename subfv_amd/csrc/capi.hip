@@ -1807,21 +1807,23 @@ int spmv(subfv_bcsr *A, bool fp32, const double *x, double *y) {
   return 0;
 }
 
-int bjacobi(subfv_bcsr *A, bool fp32, const double *bvec, double *x) {
+// TI: the type of bvec (float for a column of an FP32 Krylov basis)
+template <typename TI = double>
+int bjacobi(subfv_bcsr *A, bool fp32, const TI *bvec, double *x) {
   hipStream_t s = A->ctx->stream;
   dim3 g(nblk(A->m, 256)), bl(256);
   switch (A->b) {
     case 3:
-      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 3>), g, bl, 0, s, A->m, A->F32, bvec, x);
-      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 3>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 3, TI>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 3, TI>), g, bl, 0, s, A->m, A->F64, bvec, x);
       break;
     case 4:
-      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 4>), g, bl, 0, s, A->m, A->F32, bvec, x);
-      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 4>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 4, TI>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 4, TI>), g, bl, 0, s, A->m, A->F64, bvec, x);
       break;
     default:
-      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 5>), g, bl, 0, s, A->m, A->F32, bvec, x);
-      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 5>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 5, TI>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 5, TI>), g, bl, 0, s, A->m, A->F64, bvec, x);
   }
   HIPCHK(A->ctx, hipGetLastError());
   return 0;
@@ -2064,16 +2066,19 @@ namespace {
 // CGS2 passes of one Arnoldi step against the k columns of V (k <= kGsMax):
 // hcol(1:k) = h1 + h2 and w orthogonalised; the partials of <w, w> are left
 // in A->gp3 for k_scale_norm.
-template <int KM>
-void gs_project_k(subfv_bcsr *A, int k, const double *V, long ldv, double *w, double *hcol) {
+template <int KM, typename VT>
+void gs_project_k(subfv_bcsr *A, int k, const VT *V, long ldv, double *w, double *hcol) {
   hipStream_t s = A->ctx->stream;
   dim3 g(kGsBlocks), b(kVecThreads);
-  hipLaunchKernelGGL((k_gs_dot<KM>), g, b, 0, s, A->n, k, V, ldv, (const double *)w, A->gp1);
-  hipLaunchKernelGGL((k_gs_update<KM>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp1, hcol, 0, 1, A->gp2);
-  hipLaunchKernelGGL((k_gs_update<KM>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp2, hcol, 1, 0, A->gp3);
+  hipLaunchKernelGGL((k_gs_dot<KM, VT>), g, b, 0, s, A->n, k, V, ldv, (const double *)w, A->gp1);
+  hipLaunchKernelGGL((k_gs_update<KM, VT>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp1, hcol, 0, 1,
+                     A->gp2);
+  hipLaunchKernelGGL((k_gs_update<KM, VT>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp2, hcol, 1, 0,
+                     A->gp3);
 }
 
-void gs_project(subfv_bcsr *A, int k, const double *V, long ldv, double *w, double *hcol) {
+template <typename VT>
+void gs_project(subfv_bcsr *A, int k, const VT *V, long ldv, double *w, double *hcol) {
   switch ((k + 3) / 4) {
     case 1: gs_project_k<4>(A, k, V, ldv, w, hcol); break;
     case 2: gs_project_k<8>(A, k, V, ldv, w, hcol); break;
@@ -2113,6 +2118,12 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
     Hb = A->H;
   }
   auto Vc = [&](int j) { return Vb + (long)(j - 1) * n; };  // 1-based column
+  // FP32-inner block-Jacobi GMRES keeps its Krylov basis in FP32 (in the
+  // same workspace, half of it): the orthogonalisation reads half the bytes;
+  // the products and sums stay FP64 and each restart's residual is FP64
+  const bool vf32 = fp32 && kind == 1 && mr <= kGsMax;
+  float *Vf = reinterpret_cast<float *>(Vb);
+  auto Vfc = [&](int j) { return Vf + (long)(j - 1) * n; };
   auto Hd = [&](int i, int j) { return Hb + (i - 1) + (long)ldh * (j - 1); };
   dim3 vg(kRedBlocks), vb(kVecThreads);
   long nspmv = 0;
@@ -2132,8 +2143,21 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       std::fill(y.begin(), y.end(), 0.0);
       y[1] = rho;
       HIPCHK(c, hipMemcpyAsync(A->scal, &rho, sizeof(double), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
+      if (vf32)
+        hipLaunchKernelGGL(k_scale_div<float>, vg, vb, 0, s, n, A->r, Vfc(1), A->scal, 0, (double *)nullptr);
+      else
+        hipLaunchKernelGGL(k_scale_div<double>, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
       for (int j = 2; j <= mr + 1; j++) {
+        if (vf32) {
+          // w = A M^-1 v(j-1); CGS2 against the FP32 columns; v(j) = w / ||w||
+          if ((r = bjacobi<float>(A, true, Vfc(j - 1), A->tmp))) return r;
+          if ((r = spmv(A, true, A->tmp, A->w))) return r;
+          nspmv++;
+          gs_project<float>(A, j - 1, Vf, n, A->w, Hd(1, j - 1));
+          hipLaunchKernelGGL(k_scale_norm<float>, dim3(kGsBlocks), vb, 0, s, n, (const double *)A->w,
+                             (const double *)A->gp3, Hd(j, j - 1), Vfc(j));
+          continue;
+        }
         if (kind == 0) {
           if ((r = spmv(A, fp32, Vc(j - 1), Vc(j)))) return r;
         } else {
@@ -2145,8 +2169,9 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
         if (mr <= kGsMax) {
           // classical Gram-Schmidt twice against V(1..j-1): h(1:j-1, j-1) =
           // h1 + h2, then h(j, j-1) = ||w|| and V(:, j) = w / h(j, j-1)
-          gs_project(A, j - 1, Vb, n, w, Hd(1, j - 1));
-          hipLaunchKernelGGL(k_scale_norm, dim3(kGsBlocks), vb, 0, s, n, w, (const double *)A->gp3, Hd(j, j - 1));
+          gs_project<double>(A, j - 1, Vb, n, w, Hd(1, j - 1));
+          hipLaunchKernelGGL(k_scale_norm<double>, dim3(kGsBlocks), vb, 0, s, n, (const double *)w,
+                             (const double *)A->gp3, Hd(j, j - 1), w);
         } else {
           // modified Gram-Schmidt against V(1..j-1): each step sums the
           // previous step's dot (h(i-1, j-1)) itself, so partial buffers alternate
@@ -2158,7 +2183,8 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
                                Hd(i - 1, j - 1), (const double *)(i < j ? Vc(i) : nullptr), pb);
             std::swap(pa, pb);
           }
-          hipLaunchKernelGGL(k_scale_norm, vg, vb, 0, s, n, w, (const double *)pa, Hd(j, j - 1));
+          hipLaunchKernelGGL(k_scale_norm<double>, vg, vb, 0, s, n, (const double *)w, (const double *)pa,
+                             Hd(j, j - 1), w);
         }
       }
       HIPCHK(c, hipGetLastError());
@@ -2201,7 +2227,11 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       if (kind == 0) {
         hipLaunchKernelGGL(k_add_combination, vg, vb, 0, s, n, nr, (const double *)Vb, n, (const double *)A->z, A->x);
       } else {
-        hipLaunchKernelGGL(k_combine, vg, vb, 0, s, n, nr, Vb, n, A->z, A->r);
+        if (vf32)
+          hipLaunchKernelGGL(k_combine<float>, vg, vb, 0, s, n, nr, (const float *)Vf, n, (const double *)A->z, A->r);
+        else
+          hipLaunchKernelGGL(k_combine<double>, vg, vb, 0, s, n, nr, (const double *)Vb, n, (const double *)A->z,
+                             A->r);
         if ((r = precond(A, kind, fp32, A->r, A->zp))) return r;
         hipLaunchKernelGGL(k_add, vg, vb, 0, s, n, A->x, A->zp);
       }

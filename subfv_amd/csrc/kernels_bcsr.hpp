@@ -81,8 +81,9 @@ SFV_DEV void noswp_apply(const T *f, T *w) {
 }
 
 // x = D^-1 b with the cached factors, in lu_solve_noswp's operation order.
-template <typename T, int B>
-__global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
+// TI: the type of b (float when it is a column of an FP32 Krylov basis)
+template <typename T, int B, typename TI = double>
+__global__ void k_bcsr_bjacobi(int m, const T *F, const TI *b, double *x) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const T *f = F + i * B * B;
@@ -388,9 +389,10 @@ SFV_DEV void gs_coeffs(const double *part, int k, int nb, double *h) {
   __syncthreads();
 }
 
-// partials of <V[:, c], w>, c < k, into part[c * gridDim.x + blockIdx.x]
-template <int KM>
-__global__ __launch_bounds__(kVecThreads) void k_gs_dot(long n, int k, const double *V, long ldv, const double *w,
+// partials of <V[:, c], w>, c < k, into part[c * gridDim.x + blockIdx.x];
+// VT: the basis' storage type (float for an FP32 basis, summed in double)
+template <int KM, typename VT = double>
+__global__ __launch_bounds__(kVecThreads) void k_gs_dot(long n, int k, const VT *V, long ldv, const double *w,
                                                         double *part) {
   double acc[KM];
 #pragma unroll
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(kVecThreads) void k_gs_dot(long n, int k, const dou
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double vv[KM];
 #pragma unroll
-    for (int c = 0; c < KM; c++) vv[c] = V[(long)(c < k ? c : k - 1) * ldv + i];
+    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < k ? c : k - 1) * ldv + i];
     const double wi = w[i];
 #pragma unroll
     for (int c = 0; c < KM; c++) acc[c] = acc[c] + vv[c] * wi;
@@ -412,8 +414,8 @@ __global__ __launch_bounds__(kVecThreads) void k_gs_dot(long n, int k, const dou
 // sum_c h[c] V[:, c]; block 0 stores h to hcol (or adds it, accumulate).
 // project: then partials of <V[:, c], w> into part_out (layout of k_gs_dot);
 // otherwise partials of <w, w>, one per block.
-template <int KM>
-__global__ __launch_bounds__(kVecThreads) void k_gs_update(long n, int k, const double *V, long ldv, double *w,
+template <int KM, typename VT = double>
+__global__ __launch_bounds__(kVecThreads) void k_gs_update(long n, int k, const VT *V, long ldv, double *w,
                                                            const double *part_in, double *hcol, int accumulate,
                                                            int project, double *part_out) {
   __shared__ double h[KM];
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(kVecThreads) void k_gs_update(long n, int k, const 
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double vv[KM];
 #pragma unroll
-    for (int c = 0; c < KM; c++) vv[c] = V[(long)(c < k ? c : k - 1) * ldv + i];
+    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < k ? c : k - 1) * ldv + i];
     double x = w[i];
 #pragma unroll
     for (int c = 0; c < KM; c++) x = (c < k) ? x - h[c] * vv[c] : x;
@@ -447,21 +449,24 @@ __global__ __launch_bounds__(kVecThreads) void k_gs_update(long n, int k, const 
     block_sum_store(nrm, part_out);
 }
 
-// w = w / sqrt(sum of part) (the norm of the last MGS step; block 0 stores
-// it to *h_out)
-__global__ __launch_bounds__(kVecThreads) void k_scale_norm(long n, double *w, const double *part, double *h_out) {
+// out = w / sqrt(sum of part) (the norm of the last MGS step; block 0 stores
+// it to *h_out); out is w itself, or the next column of an FP32 basis
+template <typename VT = double>
+__global__ __launch_bounds__(kVecThreads) void k_scale_norm(long n, const double *w, const double *part, double *h_out,
+                                                            VT *out) {
   const double d = sqrt(block_total(part, (int)gridDim.x, nullptr));
   if (blockIdx.x == 0 && threadIdx.x == 0) *h_out = d;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    w[i] = w[i] / d;
+    out[i] = (VT)(w[i] / d);
 }
 
 // out = a / *s   (normalisation by a device scalar, sqrt taken if take_sqrt)
-__global__ void k_scale_div(long n, const double *a, double *out, const double *s, int take_sqrt, double *s_out) {
+template <typename VT = double>
+__global__ void k_scale_div(long n, const double *a, VT *out, const double *s, int take_sqrt, double *s_out) {
   double d = take_sqrt ? sqrt(*s) : *s;
   if (blockIdx.x == 0 && threadIdx.x == 0 && s_out) *s_out = d;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = a[i] / d;
+    out[i] = (VT)(a[i] / d);
 }
 
 // r = b - y
@@ -477,10 +482,11 @@ __global__ void k_add(long n, double *x, const double *y) {
 }
 
 // r = sum_{i < k} z[i] * V[:, i]  (sequential in i for each entry)
-__global__ void k_combine(long n, int k, const double *V, long ldv, const double *z, double *r) {
+template <typename VT = double>
+__global__ void k_combine(long n, int k, const VT *V, long ldv, const double *z, double *r) {
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int i = 0; i < k; i++) acc = acc + z[i] * V[(long)i * ldv + e];
+    for (int i = 0; i < k; i++) acc = acc + z[i] * (double)V[(long)i * ldv + e];
     r[e] = acc;
   }
 }
