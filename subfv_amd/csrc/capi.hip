@@ -1756,7 +1756,8 @@ struct subfv_bcsr {
   int mr = 0;
   double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
   double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr, *part2 = nullptr;
-  double *gp1 = nullptr, *gp2 = nullptr, *gp3 = nullptr;  // CGS2 partials (kGsMax x kGsBlocks)
+  double *dcp = nullptr;  // DCGS2 partials ((2 kGsMax + 2) x kGsBlocks)
+  double *dch = nullptr;  // DCGS2 scalars (kDchSize)
   unsigned int *ticket = nullptr;
   // host copies (0-based) for add_block and the sweep schedules
   std::vector<int> h_row, h_col, h_diag;
@@ -1807,23 +1808,21 @@ int spmv(subfv_bcsr *A, bool fp32, const double *x, double *y) {
   return 0;
 }
 
-// TI: the type of bvec (float for a column of an FP32 Krylov basis)
-template <typename TI = double>
-int bjacobi(subfv_bcsr *A, bool fp32, const TI *bvec, double *x) {
+int bjacobi(subfv_bcsr *A, bool fp32, const double *bvec, double *x) {
   hipStream_t s = A->ctx->stream;
   dim3 g(nblk(A->m, 256)), bl(256);
   switch (A->b) {
     case 3:
-      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 3, TI>), g, bl, 0, s, A->m, A->F32, bvec, x);
-      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 3, TI>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 3>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 3>), g, bl, 0, s, A->m, A->F64, bvec, x);
       break;
     case 4:
-      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 4, TI>), g, bl, 0, s, A->m, A->F32, bvec, x);
-      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 4, TI>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 4>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 4>), g, bl, 0, s, A->m, A->F64, bvec, x);
       break;
     default:
-      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 5, TI>), g, bl, 0, s, A->m, A->F32, bvec, x);
-      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 5, TI>), g, bl, 0, s, A->m, A->F64, bvec, x);
+      if (fp32) hipLaunchKernelGGL((k_bcsr_bjacobi<float, 5>), g, bl, 0, s, A->m, A->F32, bvec, x);
+      else hipLaunchKernelGGL((k_bcsr_bjacobi<double, 5>), g, bl, 0, s, A->m, A->F64, bvec, x);
   }
   HIPCHK(A->ctx, hipGetLastError());
   return 0;
@@ -1996,9 +1995,8 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   if (!r) r = balloc(A, &A->z, 256);
   if (!r) r = balloc(A, &A->part, kRedBlocks);
   if (!r) r = balloc(A, &A->part2, kRedBlocks);
-  if (!r) r = balloc(A, &A->gp1, (size_t)kGsMax * kGsBlocks);
-  if (!r) r = balloc(A, &A->gp2, (size_t)kGsMax * kGsBlocks);
-  if (!r) r = balloc(A, &A->gp3, kGsBlocks);
+  if (!r) r = balloc(A, &A->dcp, (size_t)(2 * kGsMax + 2) * kGsBlocks);
+  if (!r) r = balloc(A, &A->dch, (size_t)kDchSize);
   if (!r) r = balloc(A, &A->ticket, 1);
   if (!r) r = balloc(A, &A->perr, 1);
   if (r) {
@@ -2063,31 +2061,35 @@ int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs) {
 
 namespace {
 
-// CGS2 passes of one Arnoldi step against the k columns of V (k <= kGsMax):
-// hcol(1:k) = h1 + h2 and w orthogonalised; the partials of <w, w> are left
-// in A->gp3 for k_scale_norm.
+// One DCGS2 Arnoldi step (k_dc_dot, k_dc_coef, k_dc_update) against the st
+// columns of V, or the cycle's closing pass (fin: k_dc_dot with z = u and
+// k_dc_coef, which corrects the last column of H).
 template <int KM, typename VT>
-void gs_project_k(subfv_bcsr *A, int k, const VT *V, long ldv, double *w, double *hcol) {
+void dc_step_k(subfv_bcsr *A, int st, VT *V, long ldv, double *u, const double *z, double *H, int ldh, bool fin) {
   hipStream_t s = A->ctx->stream;
   dim3 g(kGsBlocks), b(kVecThreads);
-  hipLaunchKernelGGL((k_gs_dot<KM, VT>), g, b, 0, s, A->n, k, V, ldv, (const double *)w, A->gp1);
-  hipLaunchKernelGGL((k_gs_update<KM, VT>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp1, hcol, 0, 1,
-                     A->gp2);
-  hipLaunchKernelGGL((k_gs_update<KM, VT>), g, b, 0, s, A->n, k, V, ldv, w, (const double *)A->gp2, hcol, 1, 0,
-                     A->gp3);
+  double *hin = A->dch + (st & 1) * (kGsMax + 2), *hout = A->dch + ((st + 1) & 1) * (kGsMax + 2);
+  double *coef = A->dch + kDchCoef;
+  hipLaunchKernelGGL((k_dc_dot<KM, VT>), g, b, 0, s, A->n, st, (const VT *)V, ldv, (const double *)u,
+                     fin ? (const double *)u : z, A->dcp);
+  hipLaunchKernelGGL((k_dc_coef<KM>), dim3(1), dim3(kDcCoefThreads), 0, s, st, (const double *)A->dcp, (int)kGsBlocks,
+                     H, ldh, (const double *)hin, hout, coef, st == 0 ? A->dch + kDchBeta0 : nullptr);
+  if (!fin)
+    hipLaunchKernelGGL((k_dc_update<KM, VT>), g, b, 0, s, A->n, st, V, ldv, u, z, (const double *)coef);
 }
 
 template <typename VT>
-void gs_project(subfv_bcsr *A, int k, const VT *V, long ldv, double *w, double *hcol) {
-  switch ((k + 3) / 4) {
-    case 1: gs_project_k<4>(A, k, V, ldv, w, hcol); break;
-    case 2: gs_project_k<8>(A, k, V, ldv, w, hcol); break;
-    case 3: gs_project_k<12>(A, k, V, ldv, w, hcol); break;
-    case 4: gs_project_k<16>(A, k, V, ldv, w, hcol); break;
-    case 5: gs_project_k<20>(A, k, V, ldv, w, hcol); break;
-    case 6: gs_project_k<24>(A, k, V, ldv, w, hcol); break;
-    case 7: gs_project_k<28>(A, k, V, ldv, w, hcol); break;
-    default: gs_project_k<kGsMax>(A, k, V, ldv, w, hcol);
+void dc_step(subfv_bcsr *A, int st, VT *V, long ldv, double *u, const double *z, double *H, int ldh, bool fin) {
+  switch ((st + 3) / 4) {
+    case 0:
+    case 1: dc_step_k<4>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    case 2: dc_step_k<8>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    case 3: dc_step_k<12>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    case 4: dc_step_k<16>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    case 5: dc_step_k<20>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    case 6: dc_step_k<24>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    case 7: dc_step_k<28>(A, st, V, ldv, u, z, H, ldh, fin); break;
+    default: dc_step_k<kGsMax>(A, st, V, ldv, u, z, H, ldh, fin);
   }
 }
 
@@ -2118,12 +2120,16 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
     Hb = A->H;
   }
   auto Vc = [&](int j) { return Vb + (long)(j - 1) * n; };  // 1-based column
-  // FP32-inner block-Jacobi GMRES keeps its Krylov basis in FP32 (in the
-  // same workspace, half of it): the orthogonalisation reads half the bytes;
-  // the products and sums stay FP64 and each restart's residual is FP64
-  const bool vf32 = fp32 && kind == 1 && mr <= kGsMax;
+  // Up to kGsMax columns the Arnoldi process is DCGS2 (k_dc_*): the columns
+  // orthogonalised twice, the second pass of each delayed into the next
+  // step's.  FP32-inner block-Jacobi GMRES then keeps its Krylov basis in
+  // FP32 (in the same workspace, half of it): the orthogonalisation reads
+  // half the bytes; the products and sums stay FP64, so do the provisional
+  // vector and each restart's residual.
+  const bool dc = mr <= kGsMax;
+  const bool vf32 = dc && fp32 && kind == 1;
   float *Vf = reinterpret_cast<float *>(Vb);
-  auto Vfc = [&](int j) { return Vf + (long)(j - 1) * n; };
+  double *du = A->zp, *dz = A->r;  // the provisional vector and A M^-1 of it (r is free during the cycle)
   auto Hd = [&](int i, int j) { return Hb + (i - 1) + (long)ldh * (j - 1); };
   dim3 vg(kRedBlocks), vb(kVecThreads);
   long nspmv = 0;
@@ -2143,21 +2149,31 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       std::fill(y.begin(), y.end(), 0.0);
       y[1] = rho;
       HIPCHK(c, hipMemcpyAsync(A->scal, &rho, sizeof(double), hipMemcpyHostToDevice, s));
-      if (vf32)
-        hipLaunchKernelGGL(k_scale_div<float>, vg, vb, 0, s, n, A->r, Vfc(1), A->scal, 0, (double *)nullptr);
-      else
-        hipLaunchKernelGGL(k_scale_div<double>, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
-      for (int j = 2; j <= mr + 1; j++) {
-        if (vf32) {
-          // w = A M^-1 v(j-1); CGS2 against the FP32 columns; v(j) = w / ||w||
-          if ((r = bjacobi<float>(A, true, Vfc(j - 1), A->tmp))) return r;
-          if ((r = spmv(A, true, A->tmp, A->w))) return r;
+      if (dc) {
+        // u = r / rho; step st: z = A M^-1 u, then V(:, st) and the next u
+        hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, du, A->scal, 0, (double *)nullptr);
+        for (int st = 0; st < mr; st++) {
+          if (kind == 0) {
+            if ((r = spmv(A, fp32, du, dz))) return r;
+          } else {
+            if ((r = precond(A, kind, fp32, du, A->tmp))) return r;
+            if ((r = spmv(A, fp32, A->tmp, dz))) return r;
+          }
           nspmv++;
-          gs_project<float>(A, j - 1, Vf, n, A->w, Hd(1, j - 1));
-          hipLaunchKernelGGL(k_scale_norm<float>, dim3(kGsBlocks), vb, 0, s, n, (const double *)A->w,
-                             (const double *)A->gp3, Hd(j, j - 1), Vfc(j));
-          continue;
+          if (vf32)
+            dc_step<float>(A, st, Vf, n, du, dz, Hb, ldh, false);
+          else
+            dc_step<double>(A, st, Vb, n, du, dz, Hb, ldh, false);
         }
+        // the last column's second pass: h(1:mr, mr) and h(mr + 1, mr)
+        if (vf32)
+          dc_step<float>(A, mr, Vf, n, du, dz, Hb, ldh, true);
+        else
+          dc_step<double>(A, mr, Vb, n, du, dz, Hb, ldh, true);
+      } else {
+        hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
+      }
+      for (int j = 2; !dc && j <= mr + 1; j++) {
         if (kind == 0) {
           if ((r = spmv(A, fp32, Vc(j - 1), Vc(j)))) return r;
         } else {
@@ -2165,30 +2181,28 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
           if ((r = spmv(A, fp32, A->tmp, Vc(j)))) return r;
         }
         nspmv++;
+        // modified Gram-Schmidt against V(1..j-1): each step sums the
+        // previous step's dot (h(i-1, j-1)) itself, so partial buffers alternate
         double *w = Vc(j);
-        if (mr <= kGsMax) {
-          // classical Gram-Schmidt twice against V(1..j-1): h(1:j-1, j-1) =
-          // h1 + h2, then h(j, j-1) = ||w|| and V(:, j) = w / h(j, j-1)
-          gs_project<double>(A, j - 1, Vb, n, w, Hd(1, j - 1));
-          hipLaunchKernelGGL(k_scale_norm<double>, dim3(kGsBlocks), vb, 0, s, n, (const double *)w,
-                             (const double *)A->gp3, Hd(j, j - 1), w);
-        } else {
-          // modified Gram-Schmidt against V(1..j-1): each step sums the
-          // previous step's dot (h(i-1, j-1)) itself, so partial buffers alternate
-          double *pa = A->part, *pb = A->part2;
-          hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)nullptr, (const double *)nullptr,
-                             (double *)nullptr, (const double *)Vc(1), pa);
-          for (int i = 2; i <= j; i++) {
-            hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)Vc(i - 1), (const double *)pa,
-                               Hd(i - 1, j - 1), (const double *)(i < j ? Vc(i) : nullptr), pb);
-            std::swap(pa, pb);
-          }
-          hipLaunchKernelGGL(k_scale_norm<double>, vg, vb, 0, s, n, (const double *)w, (const double *)pa,
-                             Hd(j, j - 1), w);
+        double *pa = A->part, *pb = A->part2;
+        hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)nullptr, (const double *)nullptr,
+                           (double *)nullptr, (const double *)Vc(1), pa);
+        for (int i = 2; i <= j; i++) {
+          hipLaunchKernelGGL(k_mgs_chain, vg, vb, 0, s, n, w, (const double *)Vc(i - 1), (const double *)pa,
+                             Hd(i - 1, j - 1), (const double *)(i < j ? Vc(i) : nullptr), pb);
+          std::swap(pa, pb);
         }
+        hipLaunchKernelGGL(k_scale_norm, vg, vb, 0, s, n, w, (const double *)pa, Hd(j, j - 1));
       }
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipMemcpyAsync(H.data(), Hb, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      if (dc) {
+        // the first column is (r / rho) / beta0: r = rho beta0 V(:, 1)
+        double beta0 = 1.0;
+        HIPCHK(c, hipMemcpyAsync(&beta0, A->dch + kDchBeta0, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        y[1] = rho * beta0;
+      }
       HIPCHK(c, hipStreamSynchronize(s));
       auto h = [&](int i, int j) -> double & { return H[(i - 1) + (size_t)ldh * (j - 1)]; };
       int nr = mr, i, exited = 0;

@@ -81,9 +81,8 @@ SFV_DEV void noswp_apply(const T *f, T *w) {
 }
 
 // x = D^-1 b with the cached factors, in lu_solve_noswp's operation order.
-// TI: the type of b (float when it is a column of an FP32 Krylov basis)
-template <typename T, int B, typename TI = double>
-__global__ void k_bcsr_bjacobi(int m, const T *F, const TI *b, double *x) {
+template <typename T, int B>
+__global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const T *f = F + i * B * B;
@@ -338,135 +337,195 @@ __global__ __launch_bounds__(kVecThreads) void k_mgs_chain(long n, double *w, co
   block_sum_store(acc, part_out);
 }
 
-// ------------------------------------------------ classical Gram-Schmidt, twice
-// The Arnoldi orthogonalisation of GMRES as CGS2: h1 = V^T w, w -= V h1,
-// h2 = V^T w, w -= V h2, h = h1 + h2, then ||w||.  Three passes over V per
-// step (the second fuses the first update with the second projection)
-// instead of one pass per basis vector.  Partial sums are per block (fixed
-// grid kGsBlocks, fixed reduction trees), so results are deterministic.
+// ------------------------------------------------ Gram-Schmidt in block passes
+// Partial sums are per block (fixed grid kGsBlocks, fixed reduction trees),
+// so results are deterministic.
 constexpr int kGsBlocks = 512;
-constexpr int kGsMax = 32;  // basis vectors the CGS2 passes handle (restart <= 32)
+constexpr int kGsMax = 32;  // basis vectors the block passes handle (restart <= 32)
+// the DCGS2 scalars of a GMRES handle: the two provisional Hessenberg
+// columns (kGsMax + 2 each), beta of the cycle's first column, then the
+// step's coefficients (a, b, beta, e: 2 kGsMax + 2)
+constexpr int kDchBeta0 = 2 * (kGsMax + 2), kDchCoef = kDchBeta0 + 1, kDchSize = kDchCoef + 2 * kGsMax + 2;
 
-// per-block sums of acc[c], c < k, into part[c * gridDim.x + blockIdx.x]
+// ---- Arnoldi with delayed reorthogonalisation (DCGS2: Swirydowicz, Langou,
+// Ananthan, Yang and Thomas, "Low synchronization Gram-Schmidt and GMRES
+// algorithms", Numer. Linear Algebra Appl. 28 (2021); Bielich et al.,
+// Parallel Computing 112 (2022)).  Step s holds the orthonormal columns
+// V(:, 0..s-1) and a provisional next vector u, already orthogonalised once,
+// with A M^-1 V(:, s-1) = V(:, 0..s-1) hprov + u.  With z = A M^-1 u, one
+// pass forms a = V^T u, b = V^T z, c = u.u and d = u.z; then
+//   beta = sqrt(c - a.a),  V(:, s) = (u - V a) / beta,
+//   H(0..s-1, s-1) = hprov + a,  H(s, s-1) = beta          (u's second pass)
+//   e = (d - a.b) / beta,  Ha = H(0..s, 0..s-1) a,
+//   hprov' = ((b - Ha(0..s-1)) / beta, (e - Ha(s)) / beta)    (the next column, first pass)
+//   u' = (z - V b - V(:, s) e) / beta
+// because A M^-1 V(:, s) = (z - A M^-1 V a) / beta = (z - V H a) / beta.  The
+// orthogonalisation reads the basis twice per step instead of three times.
+
+// The step's coefficients from the partials of k_dc_dot (nbp blocks), one
+// block of kDcCoefThreads: co[0..s) = a, co[s..2s) = b, co[2s] = c,
+// co[2s+1] = d; then beta, e, Ha; H's corrected column s-1, the new
+// provisional column hprov_out (hprov_in is only read, so the two
+// alternate), and coef = (a, b, beta, e) for the update pass.  beta_out
+// (step 0): beta, the norm of the cycle's first vector.
+constexpr int kDcCoefThreads = 1024, kDcRuns = 16;
 template <int KM>
-SFV_DEV void gs_block_store(const double *acc, int k, double *part) {
-  __shared__ double s[kVecThreads / 64][KM];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int c = 0; c < KM; c++) {
-    if (c < k) {
-      double v = acc[c];
-      for (int o = 32; o > 0; o >>= 1) v = v + __shfl_down(v, o);
-      if (lane == 0) s[wv][c] = v;
+__global__ __launch_bounds__(kDcCoefThreads) void k_dc_coef(int s, const double *part, int nbp, double *H, int ldh,
+                                                           const double *hprov_in, double *hprov_out, double *coef,
+                                                           double *beta_out) {
+  __shared__ double s2[2 * KM + 2][kDcRuns], co[2 * KM + 2], ha[KM + 1], sc[2];
+  __shared__ double hs[KM + 1][KM];  // H(0..s, 0..s-1), upper Hessenberg, column s-1 corrected below
+  const int ncol = 2 * s + 2;
+  // H's final columns 0..s-2 into LDS while the partials are summed
+  for (int q = threadIdx.x; q < (s + 1) * s; q += kDcCoefThreads) {
+    const int i = q % (s + 1), k = q / (s + 1);
+    if (k < s - 1) hs[i][k] = i <= k + 1 ? H[i + (long)ldh * k] : 0.0;
+  }
+  // kDcRuns threads per partial column, each a fixed run of blocks, then the
+  // runs in order
+  for (int c0 = 0; c0 < ncol; c0 += kDcCoefThreads / kDcRuns) {
+    const int c = c0 + (int)(threadIdx.x / kDcRuns), r = threadIdx.x % kDcRuns;
+    if (c < ncol) {
+      const int per = (nbp + kDcRuns - 1) / kDcRuns, b0 = r * per, b1 = min(nbp, b0 + per);
+      double t = 0.0;
+      for (int b = b0; b < b1; b++) t = t + part[(long)c * nbp + b];
+      s2[c][r] = t;
     }
   }
   __syncthreads();
-  if ((int)threadIdx.x < k) {
-    double t = s[0][threadIdx.x];
-    for (int q = 1; q < kVecThreads / 64; q++) t = t + s[q][threadIdx.x];
+  if ((int)threadIdx.x < ncol) {
+    double t = s2[threadIdx.x][0];
+    for (int r = 1; r < kDcRuns; r++) t = t + s2[threadIdx.x][r];
+    co[threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double aa = 0.0, ab = 0.0;
+    for (int k = 0; k < s; k++) {
+      aa = aa + co[k] * co[k];
+      ab = ab + co[k] * co[s + k];
+    }
+    const double beta = sqrt(fmax(co[2 * s] - aa, 0.0));
+    sc[0] = beta;
+    sc[1] = (co[2 * s + 1] - ab) / beta;
+  }
+  __syncthreads();
+  const double beta = sc[0];
+  // Ha(i) = sum over k < s of H(i, k) a(k), i <= s, with column s-1 the
+  // corrected one (hprov_in + a, beta below it); H is upper Hessenberg
+  if (s >= 1 && (int)threadIdx.x <= s) {
+    const int i = threadIdx.x;
+    hs[i][s - 1] = i < s ? hprov_in[i] + co[i] : beta;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x <= s) {
+    const int i = threadIdx.x;
+    double t = 0.0;
+    for (int k = max(i - 1, 0); k < s; k++) t = t + hs[i][k] * co[k];
+    ha[i] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x <= s) {
+    const int i = threadIdx.x;
+    if (s >= 1) H[i + (long)ldh * (s - 1)] = hs[i][s - 1];
+    hprov_out[i] = i < s ? (co[s + i] - ha[i]) / beta : (sc[1] - ha[s]) / beta;
+  }
+  if ((int)threadIdx.x < 2 * s) coef[threadIdx.x] = co[threadIdx.x];
+  if (threadIdx.x == 0) {
+    coef[2 * s] = beta;
+    coef[2 * s + 1] = sc[1];
+    if (beta_out) *beta_out = beta;
+  }
+}
+
+// The step's dot pass: per-block partials of a (columns 0..s-1), b
+// (s..2s-1), c (2s) and d (2s+1) into part[q * gridDim.x + blockIdx.x]
+template <int KM, typename VT>
+__global__ __launch_bounds__(kVecThreads) void k_dc_dot(long n, int s, const VT *V, long ldv, const double *u,
+                                                       const double *z, double *part) {
+  double aa[KM], bb[KM], cc = 0.0, dd = 0.0;
+#pragma unroll
+  for (int c = 0; c < KM; c++) aa[c] = bb[c] = 0.0;
+  // columns c >= s (fewer than 4 of them, or all when s = 0) re-read column
+  // max(s - 1, 0), so every load is issued without a branch; their sums are
+  // not stored
+  const int last = s > 0 ? s - 1 : 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double vv[KM];
+#pragma unroll
+    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < s ? c : last) * ldv + i];
+    const double ui = u[i], zi = z[i];
+#pragma unroll
+    for (int c = 0; c < KM; c++) {
+      aa[c] = aa[c] + vv[c] * ui;
+      bb[c] = bb[c] + vv[c] * zi;
+    }
+    cc = cc + ui * ui;
+    dd = dd + ui * zi;
+  }
+  __shared__ double sh[kVecThreads / 64][2 * KM + 2];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  auto wsum = [&](double v, int slot) {
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_down(v, o);
+    if (lane == 0) sh[wv][slot] = v;
+  };
+#pragma unroll
+  for (int c = 0; c < KM; c++)
+    if (c < s) {
+      wsum(aa[c], c);
+      wsum(bb[c], s + c);
+    }
+  wsum(cc, 2 * s);
+  wsum(dd, 2 * s + 1);
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * s + 2) {
+    double t = sh[0][threadIdx.x];
+    for (int q = 1; q < kVecThreads / 64; q++) t = t + sh[q][threadIdx.x];
     part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
   }
 }
 
-// h[c] = sum over the blocks of part[c * nb + b], c < k (eight threads per
-// column, each a fixed run of blocks, then the eight runs in order)
-template <int KM>
-SFV_DEV void gs_coeffs(const double *part, int k, int nb, double *h) {
-  __shared__ double s2[KM][8];
-  const int c = threadIdx.x >> 3, r = threadIdx.x & 7;
-  if (c < k && c < KM) {
-    const int per = (nb + 7) / 8, b0 = r * per, b1 = min(nb, b0 + per);
-    double t = 0.0;
-    for (int b = b0; b < b1; b++) t = t + part[(long)c * nb + b];
-    s2[c][r] = t;
-  }
+// the step's update pass: V(:, s) = (u - V a) / beta; u = (z - V b - V(:, s) e) / beta
+template <int KM, typename VT>
+__global__ __launch_bounds__(kVecThreads) void k_dc_update(long n, int s, VT *V, long ldv, double *u, const double *z,
+                                                          const double *coef) {
+  __shared__ double co[2 * KM + 2];
+  if ((int)threadIdx.x < 2 * s + 2) co[threadIdx.x] = coef[threadIdx.x];
   __syncthreads();
-  if ((int)threadIdx.x < k) {
-    double t = s2[threadIdx.x][0];
-    for (int q = 1; q < 8; q++) t = t + s2[threadIdx.x][q];
-    h[threadIdx.x] = t;
-  }
-  __syncthreads();
-}
-
-// partials of <V[:, c], w>, c < k, into part[c * gridDim.x + blockIdx.x];
-// VT: the basis' storage type (float for an FP32 basis, summed in double)
-template <int KM, typename VT = double>
-__global__ __launch_bounds__(kVecThreads) void k_gs_dot(long n, int k, const VT *V, long ldv, const double *w,
-                                                        double *part) {
-  double acc[KM];
-#pragma unroll
-  for (int c = 0; c < KM; c++) acc[c] = 0.0;
-  // columns c >= k (KM - k < 4 of them) re-read column k - 1, so every load
-  // of an element is issued without a branch; their sums are not stored
+  const double beta = co[2 * s], e = co[2 * s + 1];
+  const int last = s > 0 ? s - 1 : 0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double vv[KM];
 #pragma unroll
-    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < k ? c : k - 1) * ldv + i];
-    const double wi = w[i];
+    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < s ? c : last) * ldv + i];
+    double q = u[i], x = z[i];
 #pragma unroll
-    for (int c = 0; c < KM; c++) acc[c] = acc[c] + vv[c] * wi;
+    for (int c = 0; c < KM; c++)
+      if (c < s) {
+        q = q - co[c] * vv[c];
+        x = x - co[s + c] * vv[c];
+      }
+    q = q / beta;
+    V[(long)s * ldv + i] = (VT)q;
+    u[i] = (x - q * e) / beta;
   }
-  gs_block_store<KM>(acc, k, part);
 }
 
-// One update pass of CGS2.  h = the sums of part_in's k columns; w = w -
-// sum_c h[c] V[:, c]; block 0 stores h to hcol (or adds it, accumulate).
-// project: then partials of <V[:, c], w> into part_out (layout of k_gs_dot);
-// otherwise partials of <w, w>, one per block.
-template <int KM, typename VT = double>
-__global__ __launch_bounds__(kVecThreads) void k_gs_update(long n, int k, const VT *V, long ldv, double *w,
-                                                           const double *part_in, double *hcol, int accumulate,
-                                                           int project, double *part_out) {
-  __shared__ double h[KM];
-  gs_coeffs<KM>(part_in, k, (int)gridDim.x, h);
-  if (blockIdx.x == 0 && (int)threadIdx.x < k)
-    hcol[threadIdx.x] = accumulate ? hcol[threadIdx.x] + h[threadIdx.x] : h[threadIdx.x];
-  double acc[KM];
-#pragma unroll
-  for (int c = 0; c < KM; c++) acc[c] = 0.0;
-  double nrm = 0.0;
-  // as in k_gs_dot, columns c >= k re-read column k - 1 (branch-free loads)
-  // and are skipped in the update
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double vv[KM];
-#pragma unroll
-    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < k ? c : k - 1) * ldv + i];
-    double x = w[i];
-#pragma unroll
-    for (int c = 0; c < KM; c++) x = (c < k) ? x - h[c] * vv[c] : x;
-    w[i] = x;
-    if (project) {
-#pragma unroll
-      for (int c = 0; c < KM; c++) acc[c] = acc[c] + vv[c] * x;
-    } else {
-      nrm = nrm + x * x;
-    }
-  }
-  if (project)
-    gs_block_store<KM>(acc, k, part_out);
-  else
-    block_sum_store(nrm, part_out);
-}
-
-// out = w / sqrt(sum of part) (the norm of the last MGS step; block 0 stores
-// it to *h_out); out is w itself, or the next column of an FP32 basis
-template <typename VT = double>
-__global__ __launch_bounds__(kVecThreads) void k_scale_norm(long n, const double *w, const double *part, double *h_out,
-                                                            VT *out) {
+// w = w / sqrt(sum of part) (the norm of the last MGS step; block 0 stores
+// it to *h_out)
+__global__ __launch_bounds__(kVecThreads) void k_scale_norm(long n, double *w, const double *part, double *h_out) {
   const double d = sqrt(block_total(part, (int)gridDim.x, nullptr));
   if (blockIdx.x == 0 && threadIdx.x == 0) *h_out = d;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = (VT)(w[i] / d);
+    w[i] = w[i] / d;
 }
 
 // out = a / *s   (normalisation by a device scalar, sqrt taken if take_sqrt)
-template <typename VT = double>
-__global__ void k_scale_div(long n, const double *a, VT *out, const double *s, int take_sqrt, double *s_out) {
+__global__ void k_scale_div(long n, const double *a, double *out, const double *s, int take_sqrt, double *s_out) {
   double d = take_sqrt ? sqrt(*s) : *s;
   if (blockIdx.x == 0 && threadIdx.x == 0 && s_out) *s_out = d;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = (VT)(a[i] / d);
+    out[i] = a[i] / d;
 }
 
 // r = b - y

@@ -57,12 +57,12 @@ SFV_DEV __host__ long diff_block_size(int nf, int ns) {
 // S (F x E), S~^T (E x F) and the products W (E x F), fm * em each, all stored
 // with the vertex's own leading dimensions.  Prism meshes need 26 KB instead
 // of 74 KB for the 16-sub-face limit, so several vertices share a CU.
-SFV_DEV __host__ inline size_t diff_setup_lds_bytes(int fm, int em) {
+SFV_DEV __host__ size_t diff_setup_lds_bytes(int fm, int em) {
   return sizeof(double) * ((size_t)fm * fm + 3ull * fm * em);
 }
 // the large class keeps the same four matrices in a global workspace, one
 // stride of doubles per setup block, kDiffWsBlocks blocks per launch
-SFV_DEV __host__ inline long diff_setup_ws_stride(int fm, int em) { return (long)fm * fm + 3L * fm * em; }
+SFV_DEV __host__ long diff_setup_ws_stride(int fm, int em) { return (long)fm * fm + 3L * fm * em; }
 constexpr int kDiffWsBlocks = 2048;
 
 template <int MS>
