@@ -1757,6 +1757,9 @@ struct subfv_bcsr {
   double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
   double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr, *part2 = nullptr;
   double *dcp = nullptr;  // DCGS2 partials ((2 kGsMax + 2) x kGsBlocks)
+  double *vr = nullptr;   // A D^-1 (block-Jacobi right preconditioning folded in), FP64
+  float *vr32 = nullptr;  // and FP32; formed by the first block-Jacobi GMRES after the values change
+  bool vr_ok = false;
   double *dch = nullptr;  // DCGS2 scalars (kDchSize)
   unsigned int *ticket = nullptr;
   // host copies (0-based) for add_block and the sweep schedules
@@ -1788,23 +1791,42 @@ int balloc(subfv_bcsr *A, T **p, size_t n) {
 
 constexpr int kRedBlocks = 1024;
 
-int spmv(subfv_bcsr *A, bool fp32, const double *x, double *y) {
+// y = A x with the values v64 (or v32 in FP32), A's pattern
+int spmv_v(subfv_bcsr *A, bool fp32, const double *v64, const float *v32, const double *x, double *y) {
   hipStream_t s = A->ctx->stream;
   long thr = (long)A->m * A->b;
   dim3 g(nblk(thr, 256)), bl(256);
   if (A->b == 4) {
-    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 4>), g, bl, 0, s, A->m, A->row, A->col, A->v32, x, y);
-    else hipLaunchKernelGGL((k_bcsr_spmv<double, 4>), g, bl, 0, s, A->m, A->row, A->col, A->v, x, y);
+    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 4>), g, bl, 0, s, A->m, A->row, A->col, v32, x, y);
+    else hipLaunchKernelGGL((k_bcsr_spmv<double, 4>), g, bl, 0, s, A->m, A->row, A->col, v64, x, y);
   } else if (A->b == 5) {
-    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 5>), g, bl, 0, s, A->m, A->row, A->col, A->v32, x, y);
-    else hipLaunchKernelGGL((k_bcsr_spmv<double, 5>), g, bl, 0, s, A->m, A->row, A->col, A->v, x, y);
+    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 5>), g, bl, 0, s, A->m, A->row, A->col, v32, x, y);
+    else hipLaunchKernelGGL((k_bcsr_spmv<double, 5>), g, bl, 0, s, A->m, A->row, A->col, v64, x, y);
   } else if (A->b == 3) {
-    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 3>), g, bl, 0, s, A->m, A->row, A->col, A->v32, x, y);
-    else hipLaunchKernelGGL((k_bcsr_spmv<double, 3>), g, bl, 0, s, A->m, A->row, A->col, A->v, x, y);
+    if (fp32) hipLaunchKernelGGL((k_bcsr_spmv<float, 3>), g, bl, 0, s, A->m, A->row, A->col, v32, x, y);
+    else hipLaunchKernelGGL((k_bcsr_spmv<double, 3>), g, bl, 0, s, A->m, A->row, A->col, v64, x, y);
   } else {
     return fail(A->ctx, SUBFV_E_INVALID, "block size must be 3, 4 or 5");
   }
   HIPCHK(A->ctx, hipGetLastError());
+  return 0;
+}
+
+int spmv(subfv_bcsr *A, bool fp32, const double *x, double *y) { return spmv_v(A, fp32, A->v, A->v32, x, y); }
+
+// A D^-1 for block-Jacobi GMRES, when the values or factors changed since
+int ensure_rprec(subfv_bcsr *A) {
+  if (A->vr_ok) return 0;
+  dim3 g(nblk(A->nnzb, 256)), bl(256);
+  hipStream_t s = A->ctx->stream;
+  if (A->b == 3)
+    hipLaunchKernelGGL(k_bcsr_rprec<3>, g, bl, 0, s, A->nnzb, A->col, A->v, A->F64, A->vr, A->vr32);
+  else if (A->b == 4)
+    hipLaunchKernelGGL(k_bcsr_rprec<4>, g, bl, 0, s, A->nnzb, A->col, A->v, A->F64, A->vr, A->vr32);
+  else
+    hipLaunchKernelGGL(k_bcsr_rprec<5>, g, bl, 0, s, A->nnzb, A->col, A->v, A->F64, A->vr, A->vr32);
+  HIPCHK(A->ctx, hipGetLastError());
+  A->vr_ok = true;
   return 0;
 }
 
@@ -1987,6 +2009,8 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   up(&A->lvl_rows, lrows.data(), lrows.size());
   up(&A->v, v, nv);
   up(&A->v32, v32.data(), nv);
+  if (!r) r = balloc(A, &A->vr, nv);
+  if (!r) r = balloc(A, &A->vr32, nv);
   if (!r) r = balloc(A, &A->F64, (size_t)m * b * b);
   if (!r) r = balloc(A, &A->F32, (size_t)m * b * b);
   for (double **p : {&A->w, &A->r, &A->x, &A->rhs, &A->tmp, &A->zp})
@@ -2152,9 +2176,13 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       if (dc) {
         // u = r / rho; step st: z = A M^-1 u, then V(:, st) and the next u
         hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, du, A->scal, 0, (double *)nullptr);
+        if (kind == 1 && (r = ensure_rprec(A))) return r;
         for (int st = 0; st < mr; st++) {
           if (kind == 0) {
             if ((r = spmv(A, fp32, du, dz))) return r;
+          } else if (kind == 1) {
+            // A M^-1 u as one product with A D^-1 (A->vr)
+            if ((r = spmv_v(A, fp32, A->vr, A->vr32, du, dz))) return r;
           } else {
             if ((r = precond(A, kind, fp32, du, A->tmp))) return r;
             if ((r = spmv(A, fp32, A->tmp, dz))) return r;
@@ -2458,6 +2486,7 @@ int subfv_bcsr_set_values(subfv_bcsr *A, const double *v) {
   const long nv = (long)A->nnzb * A->b * A->b;
   HIPCHK(c, hipMemcpyAsync(A->v, v, nv * sizeof(double), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_to_f32, dim3(nblk(nv, 256)), dim3(256), 0, s, nv, (const double *)A->v, A->v32);
+  A->vr_ok = false;
   dim3 g(nblk(A->m, 256)), bl(256);
   switch (A->b) {
     case 3:
@@ -2494,6 +2523,7 @@ int subfv_bcsr_add_block(subfv_bcsr *A, int32_t i, int32_t j, const double *M) {
   const int bb = A->b * A->b;
   HIPCHK(c, hipMemcpyAsync(A->z, M, bb * sizeof(double), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_add_block, dim3(1), dim3(64), 0, s, bb, k, (const double *)A->z, A->v, A->v32);
+  A->vr_ok = false;
   if (k == A->h_diag[i - 1]) {
     dim3 g(nblk(A->m, 256)), bl(256);
     switch (A->b) {

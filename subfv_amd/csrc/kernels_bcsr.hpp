@@ -80,6 +80,39 @@ SFV_DEV void noswp_apply(const T *f, T *w) {
   }
 }
 
+// The right-preconditioned matrix of block-Jacobi GMRES, block by block:
+// B(q) = A(q) D_j^-1 for block q in column j (0-based col), with D_j^-1
+// formed column by column from the cached factors; FP64 and its FP32
+// rounding.  A (M^-1 u) becomes one product B u.
+template <int B>
+__global__ void k_bcsr_rprec(int nnzb, const int *col, const double *v, const double *F64, double *out,
+                             float *out32) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnzb) return;
+  const double *f = F64 + (long)col[q] * B * B;
+  double di[B * B];
+#pragma unroll
+  for (int k = 0; k < B; k++) {
+    double w[B];
+#pragma unroll
+    for (int r = 0; r < B; r++) w[r] = r == k ? 1.0 : 0.0;
+    noswp_apply<double, B>(f, w);
+#pragma unroll
+    for (int r = 0; r < B; r++) di[r + B * k] = w[r];
+  }
+  const double *a = v + q * B * B;
+#pragma unroll
+  for (int r = 0; r < B; r++)
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; k++) t = t + a[r + B * k] * di[k + B * c];
+      out[q * B * B + r + B * c] = t;
+      out32[q * B * B + r + B * c] = (float)t;
+    }
+}
+
 // x = D^-1 b with the cached factors, in lu_solve_noswp's operation order.
 template <typename T, int B>
 __global__ void k_bcsr_bjacobi(int m, const T *F, const double *b, double *x) {
