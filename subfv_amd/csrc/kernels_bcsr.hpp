@@ -471,14 +471,40 @@ __global__ __launch_bounds__(kDcCoefThreads) void k_dc_coef(int s, const double 
   }
 }
 
+// Wave sums of the NV values of every lane as a butterfly that halves the
+// array at each exchange (reduce-scatter): 63 exchanges for 64 values
+// instead of 6 per value.  NV is a power of two >= 64; on return lane L holds
+// the wave sums of values L * (NV / 64) .. L * (NV / 64) + NV / 64 - 1 in
+// v[0 .. NV / 64).  One exchange step per instantiation, so every index is
+// a compile-time constant and v stays in registers.
+template <int H, int BIT>
+SFV_DEV void wave_reduce_scatter_step(double *v, int lane) {
+  const bool up = (lane & BIT) != 0;
+#pragma unroll
+  for (int c = 0; c < H; c++) {
+    const double send = up ? v[c] : v[c + H];
+    const double keep = up ? v[c + H] : v[c];
+    v[c] = keep + __shfl_xor(send, BIT);
+  }
+  if constexpr (BIT > 1) wave_reduce_scatter_step<H / 2, BIT / 2>(v, lane);
+}
+template <int NV>
+SFV_DEV void wave_reduce_scatter(double *v, int lane) {
+  wave_reduce_scatter_step<NV / 2, 32>(v, lane);
+}
+
+constexpr int dc_pow2(int x) { return x <= 64 ? 64 : 128; }
+
 // The step's dot pass: per-block partials of a (columns 0..s-1), b
 // (s..2s-1), c (2s) and d (2s+1) into part[q * gridDim.x + blockIdx.x]
 template <int KM, typename VT>
 __global__ __launch_bounds__(kVecThreads) void k_dc_dot(long n, int s, const VT *V, long ldv, const double *u,
                                                        const double *z, double *part) {
-  double aa[KM], bb[KM], cc = 0.0, dd = 0.0;
+  // acc: a at [0, KM), b at [KM, 2 KM), c at 2 KM, d at 2 KM + 1, then zeros
+  constexpr int NV = dc_pow2(2 * KM + 2);
+  double acc[NV];
 #pragma unroll
-  for (int c = 0; c < KM; c++) aa[c] = bb[c] = 0.0;
+  for (int c = 0; c < NV; c++) acc[c] = 0.0;
   // columns c >= s (fewer than 4 of them, or all when s = 0) re-read column
   // max(s - 1, 0), so every load is issued without a branch; their sums are
   // not stored
@@ -490,31 +516,24 @@ __global__ __launch_bounds__(kVecThreads) void k_dc_dot(long n, int s, const VT 
     const double ui = u[i], zi = z[i];
 #pragma unroll
     for (int c = 0; c < KM; c++) {
-      aa[c] = aa[c] + vv[c] * ui;
-      bb[c] = bb[c] + vv[c] * zi;
+      acc[c] = acc[c] + vv[c] * ui;
+      acc[KM + c] = acc[KM + c] + vv[c] * zi;
     }
-    cc = cc + ui * ui;
-    dd = dd + ui * zi;
+    acc[2 * KM] = acc[2 * KM] + ui * ui;
+    acc[2 * KM + 1] = acc[2 * KM + 1] + ui * zi;
   }
-  __shared__ double sh[kVecThreads / 64][2 * KM + 2];
+  __shared__ double sh[kVecThreads / 64][NV];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  auto wsum = [&](double v, int slot) {
-    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_down(v, o);
-    if (lane == 0) sh[wv][slot] = v;
-  };
+  wave_reduce_scatter<NV>(acc, lane);
 #pragma unroll
-  for (int c = 0; c < KM; c++)
-    if (c < s) {
-      wsum(aa[c], c);
-      wsum(bb[c], s + c);
-    }
-  wsum(cc, 2 * s);
-  wsum(dd, 2 * s + 1);
+  for (int c = 0; c < NV / 64; c++) sh[wv][lane * (NV / 64) + c] = acc[c];
   __syncthreads();
   if ((int)threadIdx.x < 2 * s + 2) {
-    double t = sh[0][threadIdx.x];
-    for (int q = 1; q < kVecThreads / 64; q++) t = t + sh[q][threadIdx.x];
-    part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    const int t = threadIdx.x;
+    const int src = t < s ? t : (t < 2 * s ? KM + (t - s) : 2 * KM + (t - 2 * s));
+    double v = sh[0][src];
+    for (int q = 1; q < kVecThreads / 64; q++) v = v + sh[q][src];
+    part[(long)t * gridDim.x + blockIdx.x] = v;
   }
 }
 
