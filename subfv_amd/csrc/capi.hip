@@ -622,7 +622,10 @@ int launch_grad(subfv_ctx *c, const double *U, int method, const double *W = nul
   }
   g.cblocks = (int)nblk(c->dm.C, kCellThreads);
   g.cper_xcd = (g.cblocks + 7) / 8;
-  hipLaunchKernelGGL(k_cell_grad, dim3(8 * g.cper_xcd), dim3(kCellThreads), 0, c->stream, g);
+  if (g.method == 4)
+    hipLaunchKernelGGL(k_cell_grad<4>, dim3(8 * g.cper_xcd), dim3(kCellThreads), 0, c->stream, g);
+  else
+    hipLaunchKernelGGL(k_cell_grad<-1>, dim3(8 * g.cper_xcd), dim3(kCellThreads), 0, c->stream, g);
   HIPCHK(c, hipGetLastError());
   return 0;
 }

@@ -298,9 +298,11 @@ struct GradArgs {
   int cper_xcd, cblocks;   // cell blocks (k_cell_grad)
 };
 
+// ROWS staging rows of per-sub-face terms, SUMW per-vertex sums
+template <int ROWS, int SUMW>
 struct GradLds {
-  double red[16][kRedStride];
-  double sum[kGrpVert][28];
+  double red[ROWS][kRedStride];
+  double sum[kGrpVert][SUMW];
   int vtab[kGrpVert + 1];
   uint8_t vbound[kGrpVert];
 };
@@ -315,12 +317,16 @@ struct GradLds {
 // MODE: kNGFull forms and pseudo-inverts the geometric matrix of every
 // vertex; kNGStore does the same and keeps the pseudo-inverse in a.NGP;
 // kNGCached reads it from a.NGP instead (the matrix depends on the mesh only,
-// so the three give the same bits).
+// so the three give the same bits).  Cached, the gradient terms are staged
+// in two rounds (rho, u, v; then w, p) in 9 rows, so the workgroup's LDS
+// is 26 KB instead of 47 KB and five workgroups share a CU instead of three.
 enum { kNGFull = 0, kNGStore = 1, kNGCached = 2 };
 
 template <int MODE>
 __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
-  __shared__ GradLds L;
+  constexpr bool kCached = MODE == kNGCached;
+  constexpr int kRows = kCached ? 9 : 16, kB = kCached ? 0 : 9;  // rows; first bounds row
+  __shared__ GradLds<kRows, kCached ? 15 : 28> L;
   const DevMesh &m = a.m;
   const int lb = xcd_block(a.per_xcd);
   if (lb >= a.nblocks) return;  // uniform over the workgroup
@@ -358,10 +364,10 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
       for (int d = 0; d < 3; d++) L.red[d + 3 * b][tid] = inner ? ar * (n[d] * dx) : 0.0;
     }
   // bounds: the left cell always, the right cell when it is a cell
-  L.red[9][tid] = active ? dmin(wl[0], inner ? wr[0] : 1e100) : 1e100;
-  L.red[10][tid] = active ? dmax(wl[0], inner ? wr[0] : 0.0) : 0.0;
-  L.red[11][tid] = active ? dmin(wl[4], inner ? wr[4] : 1e100) : 1e100;
-  L.red[12][tid] = active ? dmax(wl[4], inner ? wr[4] : 0.0) : 0.0;
+  L.red[kB][tid] = active ? dmin(wl[0], inner ? wr[0] : 1e100) : 1e100;
+  L.red[kB + 1][tid] = active ? dmax(wl[0], inner ? wr[0] : 0.0) : 0.0;
+  L.red[kB + 2][tid] = active ? dmin(wl[4], inner ? wr[4] : 1e100) : 1e100;
+  L.red[kB + 3][tid] = active ? dmax(wl[4], inner ? wr[4] : 0.0) : 0.0;
   __syncthreads();
   if (MODE != kNGCached)
     for (int p = tid; p < nv * 9; p += kGrpThreads) {
@@ -384,7 +390,7 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
       }
     } else {
       const int jb = L.vtab[vv], je = L.vtab[vv + 1];
-      const double *row = L.red[9 + c];
+      const double *row = L.red[kB + c];
       mm = row[jb];
       if (c & 1)
         for (int j = jb + 1; j < je; j++) mm = dmax(mm, row[j]);
@@ -396,15 +402,22 @@ __global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
     a.VMM[4L * (vb + vv) + c] = mm;
   }
   __syncthreads();
-  // pass 2: sum a n (x) (W_R - W_L)
-  for (int k = 0; k < 5; k++) {
-    double dw = wr[k] - wl[k];
-    for (int d = 0; d < 3; d++) L.red[3 * k + d][tid] = inner ? ar * (n[d] * dw) : 0.0;
-  }
-  __syncthreads();
-  for (int p = tid; p < nv * 15; p += kGrpThreads) {
-    const int vv = p / 15, c = p - vv * 15;
-    L.sum[vv][c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
+  // pass 2: sum a n (x) (W_R - W_L), variables k0 .. k1 - 1 per round
+#pragma unroll
+  for (int k0 = 0; k0 < 5; k0 += kCached ? 3 : 5) {
+    const int k1 = kCached ? min(k0 + 3, 5) : 5;
+    if (k0 > 0) __syncthreads();  // the previous round's sums have read the rows
+#pragma unroll
+    for (int k = k0; k < k1; k++) {
+      double dw = wr[k] - wl[k];
+      for (int d = 0; d < 3; d++) L.red[3 * (k - k0) + d][tid] = inner ? ar * (n[d] * dw) : 0.0;
+    }
+    __syncthreads();
+    const int nc = 3 * (k1 - k0);
+    for (int p = tid; p < nv * nc; p += kGrpThreads) {
+      const int vv = p / nc, c = p - vv * nc;
+      L.sum[vv][3 * k0 + c] = seg_sum(L.red[c], L.vtab[vv], L.vtab[vv + 1]);
+    }
   }
   __syncthreads();
   if (tid < nv) {  // first wavefront only (nv <= 64)
@@ -432,7 +445,10 @@ SFV_DEV double oi3(const double *g) { return sqrt(g[0] * g[0] + g[1] * g[1] + g[
 
 // compute_cell_grad_method_{2..6} (:166-320), physical_slope_limiter
 // (:129-164) and relaxed_maximum_principle (:61-127)
+// METHOD >= 0: compiled for that method alone (fewer registers); -1: method
+template <int METHOD>
 __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
+  const int method = METHOD >= 0 ? METHOD : a.method;
   const DevMesh &m = a.m;
   const int lb = xcd_block(a.cper_xcd);  // neighbouring cells share their vertices' gradients
   if (lb >= a.cblocks) return;
@@ -443,12 +459,12 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
   for (int q = 0; q < 15; q++) g[q] = 0.0;
   const int vb = m.e_voff[e], nv = m.e_voff[e + 1] - vb;
   double *go = a.G + 15L * e;
-  if (a.method == 0 || a.method == 1) {
+  if (method == 0 || method == 1) {
     for (int q = 0; q < 15; q++) go[q] = 0.0;
     return;
   }
   const double eps = 1e-3;
-  if (a.method == 2) {
+  if (method == 2) {
     double dvs = 0.0;
     for (int j = 0; j < nv; j++) {
       int v = m.e_vert[vb + j];
@@ -458,7 +474,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
       dvs = dvs + vv;
     }
     for (int q = 0; q < 15; q++) g[q] = g[q] / dvs;
-  } else if (a.method == 3) {
+  } else if (method == 3) {
     int v = m.e_vert[vb];
     for (int q = 0; q < 15; q++) g[q] = a.NG[15L * v + q];
     for (int j = 1; j < nv; j++) {
@@ -478,17 +494,17 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_grad(GradArgs a) {
       for (int q = 0; q < 15; q++) ng[q] = a.NG[15L * v + q];
       for (int k = 0; k < 5; k++) {
         double den = eps + oi3(&ng[3 * k]);
-        if (a.method != 4) den = den * den;
+        if (method != 4) den = den * den;
         double wgt = vv / den;
         for (int d = 0; d < 3; d++) g[3 * k + d] = g[3 * k + d] + wgt * ng[3 * k + d];
         ws[k] = ws[k] + wgt;
       }
-      if (a.method == 6) {
+      if (method == 6) {
         for (int q = 0; q < 15; q++) cg[q] = cg[q] + vv * ng[q];
         cws = cws + vv;
       }
     }
-    if (a.method == 6) {
+    if (method == 6) {
       for (int q = 0; q < 15; q++) cg[q] = cg[q] / cws;
       // the reference passes the whole 3 x 5 central gradient to OI, whose
       // dummy is dimension(3): sequence association hands it the first
