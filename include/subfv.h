@@ -275,9 +275,9 @@ int subfv_bcsr_matmul(subfv_bcsr *A, const double *x, double *y);
 int subfv_bcsr_diag(subfv_bcsr *A, double *x, const double *rhs);
 /* bcsr_gmres_diag(A, x, b, tol, m, conv, ...): restarted right-preconditioned
  * GMRES(restart) from x = 0 with absolute tolerance tol on ||b - Ax||_2.
- * precision 0: FP64 throughout; 1: FP32 SpMV and block-Jacobi inside each
- * cycle with FP64 orthogonalisation, residuals and restarts (iterative
- * refinement).  stats[0] = restarts, stats[1] = final ||b - Ax||_2,
+ * precision 0: FP64 throughout; 1: FP32 SpMV, block-Jacobi and Krylov basis
+ * inside each cycle with FP64 orthogonalisation sums, residuals and restarts
+ * (iterative refinement).  stats[0] = restarts, stats[1] = final ||b - Ax||_2,
  * stats[2] = SpMV count. Returns SUBFV_E_NOT_CONVERGED (soft) if not conv. */
 int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                           int32_t precision, int32_t *conv, double *stats);
