@@ -2099,8 +2099,8 @@ void dc_step_k(subfv_bcsr *A, int st, VT *V, long ldv, double *u, const double *
   double *coef = A->dch + kDchCoef;
   hipLaunchKernelGGL((k_dc_dot<KM, VT>), g, b, 0, s, A->n, st, (const VT *)V, ldv, (const double *)u,
                      fin ? (const double *)u : z, A->dcp);
-  hipLaunchKernelGGL((k_dc_coef<KM>), dim3(1), dim3(kDcCoefThreads), 0, s, st, (const double *)A->dcp, (int)kGsBlocks,
-                     H, ldh, (const double *)hin, hout, coef, st == 0 ? A->dch + kDchBeta0 : nullptr);
+  hipLaunchKernelGGL((k_dc_coef<KM>), dim3(1), dim3(kDcCoefThreads), 0, s, st, (const double *)A->dcp, H, ldh,
+                     (const double *)hin, hout, coef, st == 0 ? A->dch + kDchBeta0 : nullptr);
   if (!fin)
     hipLaunchKernelGGL((k_dc_update<KM, VT>), g, b, 0, s, A->n, st, V, ldv, u, z, (const double *)coef);
 }

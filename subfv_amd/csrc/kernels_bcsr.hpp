@@ -401,27 +401,42 @@ constexpr int kDchBeta0 = 2 * (kGsMax + 2), kDchCoef = kDchBeta0 + 1, kDchSize =
 // provisional column hprov_out (hprov_in is only read, so the two
 // alternate), and coef = (a, b, beta, e) for the update pass.  beta_out
 // (step 0): beta, the norm of the cycle's first vector.
-constexpr int kDcCoefThreads = 1024, kDcRuns = 16;
+constexpr int kDcCoefThreads = 1024, kDcRuns = 16, kDcPer = kGsBlocks / kDcRuns;
+static_assert(kGsBlocks % kDcRuns == 0, "partials split evenly into runs");
+
+// sum over the 2^L lanes of each aligned group (xor butterfly, fixed order)
+template <int L>
+SFV_DEV double group_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < (1 << L); o <<= 1) v = v + __shfl_xor(v, o);
+  return v;
+}
+
 template <int KM>
-__global__ __launch_bounds__(kDcCoefThreads) void k_dc_coef(int s, const double *part, int nbp, double *H, int ldh,
+__global__ __launch_bounds__(kDcCoefThreads) void k_dc_coef(int s, const double *part, double *H, int ldh,
                                                            const double *hprov_in, double *hprov_out, double *coef,
                                                            double *beta_out) {
   __shared__ double s2[2 * KM + 2][kDcRuns], co[2 * KM + 2], ha[KM + 1], sc[2];
   __shared__ double hs[KM + 1][KM];  // H(0..s, 0..s-1), upper Hessenberg, column s-1 corrected below
   const int ncol = 2 * s + 2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // H's final columns 0..s-2 into LDS while the partials are summed
   for (int q = threadIdx.x; q < (s + 1) * s; q += kDcCoefThreads) {
     const int i = q % (s + 1), k = q / (s + 1);
     if (k < s - 1) hs[i][k] = i <= k + 1 ? H[i + (long)ldh * k] : 0.0;
   }
-  // kDcRuns threads per partial column, each a fixed run of blocks, then the
-  // runs in order
+  // kDcRuns threads per partial column, each a fixed run of kDcPer blocks
+  // (loaded together, then added in order), then the runs in order
   for (int c0 = 0; c0 < ncol; c0 += kDcCoefThreads / kDcRuns) {
     const int c = c0 + (int)(threadIdx.x / kDcRuns), r = threadIdx.x % kDcRuns;
     if (c < ncol) {
-      const int per = (nbp + kDcRuns - 1) / kDcRuns, b0 = r * per, b1 = min(nbp, b0 + per);
+      const double *pp = part + (long)c * kGsBlocks + r * kDcPer;
+      double v[kDcPer];
+#pragma unroll
+      for (int b = 0; b < kDcPer; b++) v[b] = pp[b];
       double t = 0.0;
-      for (int b = b0; b < b1; b++) t = t + part[(long)c * nbp + b];
+#pragma unroll
+      for (int b = 0; b < kDcPer; b++) t = t + v[b];
       s2[c][r] = t;
     }
   }
@@ -432,30 +447,29 @@ __global__ __launch_bounds__(kDcCoefThreads) void k_dc_coef(int s, const double 
     co[threadIdx.x] = t;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double aa = 0.0, ab = 0.0;
-    for (int k = 0; k < s; k++) {
-      aa = aa + co[k] * co[k];
-      ab = ab + co[k] * co[s + k];
+  if (wv == 0) {  // a.a and a.b over the first wave, then beta and e
+    const double ak = lane < s ? co[lane] : 0.0;
+    const double aa = group_sum<6>(ak * ak), ab = group_sum<6>(lane < s ? ak * co[s + lane] : 0.0);
+    if (lane == 0) {
+      const double beta = sqrt(fmax(co[2 * s] - aa, 0.0));
+      sc[0] = beta;
+      sc[1] = (co[2 * s + 1] - ab) / beta;
     }
-    const double beta = sqrt(fmax(co[2 * s] - aa, 0.0));
-    sc[0] = beta;
-    sc[1] = (co[2 * s + 1] - ab) / beta;
   }
   __syncthreads();
   const double beta = sc[0];
   // Ha(i) = sum over k < s of H(i, k) a(k), i <= s, with column s-1 the
-  // corrected one (hprov_in + a, beta below it); H is upper Hessenberg
+  // corrected one (hprov_in + a, beta below it); H is upper Hessenberg.
+  // One 32-lane half-wave per row i, lane k the product of column k.
   if (s >= 1 && (int)threadIdx.x <= s) {
     const int i = threadIdx.x;
     hs[i][s - 1] = i < s ? hprov_in[i] + co[i] : beta;
   }
   __syncthreads();
-  if ((int)threadIdx.x <= s) {
-    const int i = threadIdx.x;
-    double t = 0.0;
-    for (int k = max(i - 1, 0); k < s; k++) t = t + hs[i][k] * co[k];
-    ha[i] = t;
+  for (int i = threadIdx.x >> 5; i <= s; i += kDcCoefThreads / 32) {  // uniform over each half-wave
+    const int k = threadIdx.x & 31;
+    const double t = group_sum<5>(k < s && k >= i - 1 ? hs[i][k] * co[k] : 0.0);
+    if (k == 0) ha[i] = t;
   }
   __syncthreads();
   if ((int)threadIdx.x <= s) {
