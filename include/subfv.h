@@ -286,8 +286,9 @@ int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double to
  * forbid hipMalloc): subfv_bcsr_gmres_diag with its Krylov basis V and
  * Hessenberg matrix H in a file-scope device array of 4 Mi doubles (32 MiB per
  * GPU, held until the process ends) instead of memory the library allocates.
- * Requires m*b*(restart + 1) + (restart + 1)*restart <= 4194304 (4 Mi), else
- * SUBFV_E_INVALID with a message; results equal subfv_bcsr_gmres_diag's.
+ * Requires n'*(restart + 1) + (restart + 1)*restart <= 4194304 (4 Mi), with n'
+ * = m*b rounded up to even, else SUBFV_E_INVALID with a message; results
+ * equal subfv_bcsr_gmres_diag's.
  * Calls that overlap on one GPU share the array and race. */
 int subfv_bcsr_gmres_diag_noalloc(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                                   int32_t precision, int32_t *conv, double *stats);

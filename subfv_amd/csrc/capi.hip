@@ -1880,7 +1880,7 @@ int ensure_workspace(subfv_bcsr *A, int mr) {
     }
   }
   int r;
-  if ((r = balloc(A, &A->V, (size_t)A->n * (mr + 1)))) return r;
+  if ((r = balloc(A, &A->V, (size_t)(A->n + (A->n & 1)) * (mr + 1)))) return r;  // columns of even length
   if ((r = balloc(A, &A->H, (size_t)(mr + 1) * mr + 8))) return r;
   A->mr = mr;
   return 0;
@@ -2140,13 +2140,16 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
   // the Krylov basis V (n x (mr + 1)) and H ((mr + 1) x mr): the handle's
   // own workspace, or the caller-given ws that holds both
   int r = 0;
-  double *Vb = ws, *Hb = ws ? ws + (size_t)n * (mr + 1) : nullptr;
+  // V's columns are ldv = n rounded up to even apart (the DCGS2 passes read
+  // two consecutive entries at once)
+  const long ldv = n + (n & 1);
+  double *Vb = ws, *Hb = ws ? ws + (size_t)ldv * (mr + 1) : nullptr;
   if (!ws) {
     if ((r = ensure_workspace(A, mr))) return r;
     Vb = A->V;
     Hb = A->H;
   }
-  auto Vc = [&](int j) { return Vb + (long)(j - 1) * n; };  // 1-based column
+  auto Vc = [&](int j) { return Vb + (long)(j - 1) * ldv; };  // 1-based column
   // Up to kGsMax columns the Arnoldi process is DCGS2 (k_dc_*): the columns
   // orthogonalised twice, the second pass of each delayed into the next
   // step's.  FP32-inner block-Jacobi GMRES then keeps its Krylov basis in
@@ -2192,15 +2195,15 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
           }
           nspmv++;
           if (vf32)
-            dc_step<float>(A, st, Vf, n, du, dz, Hb, ldh, false);
+            dc_step<float>(A, st, Vf, ldv, du, dz, Hb, ldh, false);
           else
-            dc_step<double>(A, st, Vb, n, du, dz, Hb, ldh, false);
+            dc_step<double>(A, st, Vb, ldv, du, dz, Hb, ldh, false);
         }
         // the last column's second pass: h(1:mr, mr) and h(mr + 1, mr)
         if (vf32)
-          dc_step<float>(A, mr, Vf, n, du, dz, Hb, ldh, true);
+          dc_step<float>(A, mr, Vf, ldv, du, dz, Hb, ldh, true);
         else
-          dc_step<double>(A, mr, Vb, n, du, dz, Hb, ldh, true);
+          dc_step<double>(A, mr, Vb, ldv, du, dz, Hb, ldh, true);
       } else {
         hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
       }
@@ -2270,12 +2273,12 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       }
       HIPCHK(c, hipMemcpyAsync(A->z, z.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
       if (kind == 0) {
-        hipLaunchKernelGGL(k_add_combination, vg, vb, 0, s, n, nr, (const double *)Vb, n, (const double *)A->z, A->x);
+        hipLaunchKernelGGL(k_add_combination, vg, vb, 0, s, n, nr, (const double *)Vb, ldv, (const double *)A->z, A->x);
       } else {
         if (vf32)
-          hipLaunchKernelGGL(k_combine<float>, vg, vb, 0, s, n, nr, (const float *)Vf, n, (const double *)A->z, A->r);
+          hipLaunchKernelGGL(k_combine<float>, vg, vb, 0, s, n, nr, (const float *)Vf, ldv, (const double *)A->z, A->r);
         else
-          hipLaunchKernelGGL(k_combine<double>, vg, vb, 0, s, n, nr, (const double *)Vb, n, (const double *)A->z,
+          hipLaunchKernelGGL(k_combine<double>, vg, vb, 0, s, n, nr, (const double *)Vb, ldv, (const double *)A->z,
                              A->r);
         if ((r = precond(A, kind, fp32, A->r, A->zp))) return r;
         hipLaunchKernelGGL(k_add, vg, vb, 0, s, n, A->x, A->zp);
@@ -2315,10 +2318,10 @@ int subfv_bcsr_gmres_diag(subfv_bcsr *A, double *x, const double *rhs, double to
 int subfv_bcsr_gmres_diag_noalloc(subfv_bcsr *A, double *x, const double *rhs, double tol, int32_t restart,
                                   int32_t precision, int32_t *conv, double *stats) {
   if (!A || !x || !rhs || restart < 1 || restart > 200) return SUBFV_E_INVALID;
-  const size_t need = (size_t)A->n * (restart + 1) + (size_t)(restart + 1) * restart;
+  const size_t need = (size_t)(A->n + (A->n & 1)) * (restart + 1) + (size_t)(restart + 1) * restart;
   if (need > kNoAllocDoubles)
     return fail(A->ctx, SUBFV_E_INVALID,
-                "subfv_bcsr_gmres_diag_noalloc: n * (restart + 1) + (restart + 1) * restart = " + std::to_string(need) +
+                "subfv_bcsr_gmres_diag_noalloc: n' * (restart + 1) + (restart + 1) * restart = " + std::to_string(need) +
                     " doubles exceeds the " + std::to_string(kNoAllocDoubles) + "-double scratch array");
   double *ws = noalloc_get(A->ctx->device);
   if (!ws) return fail(A->ctx, SUBFV_E_DEVICE, "scratch array address not set up for this device");

@@ -509,11 +509,19 @@ SFV_DEV void wave_reduce_scatter(double *v, int lane) {
 
 constexpr int dc_pow2(int x) { return x <= 64 ? 64 : 128; }
 
+// two consecutive basis entries (ldv and the element index even)
+template <typename VT> struct Pair;
+template <> struct Pair<double> { using T = double2; };
+template <> struct Pair<float> { using T = float2; };
+
 // The step's dot pass: per-block partials of a (columns 0..s-1), b
-// (s..2s-1), c (2s) and d (2s+1) into part[q * gridDim.x + blockIdx.x]
+// (s..2s-1), c (2s) and d (2s+1) into part[q * gridDim.x + blockIdx.x].  Each
+// thread takes two consecutive elements per iteration (ldv even); an odd n's
+// last element goes to thread 0.
 template <int KM, typename VT>
 __global__ __launch_bounds__(kVecThreads) void k_dc_dot(long n, int s, const VT *V, long ldv, const double *u,
                                                        const double *z, double *part) {
+  using V2 = typename Pair<VT>::T;
   // acc: a at [0, KM), b at [KM, 2 KM), c at 2 KM, d at 2 KM + 1, then zeros
   constexpr int NV = dc_pow2(2 * KM + 2);
   double acc[NV];
@@ -523,16 +531,30 @@ __global__ __launch_bounds__(kVecThreads) void k_dc_dot(long n, int s, const VT 
   // max(s - 1, 0), so every load is issued without a branch; their sums are
   // not stored
   const int last = s > 0 ? s - 1 : 0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double vv[KM];
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+  auto add = [&](double vc, int c, double ui, double zi) {
+    acc[c] = acc[c] + vc * ui;
+    acc[KM + c] = acc[KM + c] + vc * zi;
+  };
+  for (long i2 = tid; i2 < (n >> 1); i2 += stride) {
+    const long i = 2 * i2;
+    V2 vv[KM];
 #pragma unroll
-    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < s ? c : last) * ldv + i];
-    const double ui = u[i], zi = z[i];
+    for (int c = 0; c < KM; c++) vv[c] = *reinterpret_cast<const V2 *>(V + (long)(c < s ? c : last) * ldv + i);
+    const double2 uu = *reinterpret_cast<const double2 *>(u + i), zz = *reinterpret_cast<const double2 *>(z + i);
 #pragma unroll
     for (int c = 0; c < KM; c++) {
-      acc[c] = acc[c] + vv[c] * ui;
-      acc[KM + c] = acc[KM + c] + vv[c] * zi;
+      add((double)vv[c].x, c, uu.x, zz.x);
+      add((double)vv[c].y, c, uu.y, zz.y);
     }
+    acc[2 * KM] = (acc[2 * KM] + uu.x * uu.x) + uu.y * uu.y;
+    acc[2 * KM + 1] = (acc[2 * KM + 1] + uu.x * zz.x) + uu.y * zz.y;
+  }
+  if ((n & 1) && tid == 0) {
+    const long i = n - 1;
+    const double ui = u[i], zi = z[i];
+#pragma unroll
+    for (int c = 0; c < KM; c++) add((double)V[(long)(c < s ? c : last) * ldv + i], c, ui, zi);
     acc[2 * KM] = acc[2 * KM] + ui * ui;
     acc[2 * KM + 1] = acc[2 * KM + 1] + ui * zi;
   }
@@ -551,25 +573,53 @@ __global__ __launch_bounds__(kVecThreads) void k_dc_dot(long n, int s, const VT 
   }
 }
 
-// the step's update pass: V(:, s) = (u - V a) / beta; u = (z - V b - V(:, s) e) / beta
+// the step's update pass: V(:, s) = (u - V a) / beta; u = (z - V b - V(:, s) e) / beta,
+// two consecutive elements per thread and iteration as in k_dc_dot
 template <int KM, typename VT>
 __global__ __launch_bounds__(kVecThreads) void k_dc_update(long n, int s, VT *V, long ldv, double *u, const double *z,
                                                           const double *coef) {
+  using V2 = typename Pair<VT>::T;
   __shared__ double co[2 * KM + 2];
   if ((int)threadIdx.x < 2 * s + 2) co[threadIdx.x] = coef[threadIdx.x];
   __syncthreads();
   const double beta = co[2 * s], e = co[2 * s + 1];
   const int last = s > 0 ? s - 1 : 0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double vv[KM];
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+  for (long i2 = tid; i2 < (n >> 1); i2 += stride) {
+    const long i = 2 * i2;
+    V2 vv[KM];
 #pragma unroll
-    for (int c = 0; c < KM; c++) vv[c] = (double)V[(long)(c < s ? c : last) * ldv + i];
+    for (int c = 0; c < KM; c++) vv[c] = *reinterpret_cast<const V2 *>(V + (long)(c < s ? c : last) * ldv + i);
+    const double2 uu = *reinterpret_cast<const double2 *>(u + i), zz = *reinterpret_cast<const double2 *>(z + i);
+    double q0 = uu.x, q1 = uu.y, x0 = zz.x, x1 = zz.y;
+#pragma unroll
+    for (int c = 0; c < KM; c++)
+      if (c < s) {
+        q0 = q0 - co[c] * (double)vv[c].x;
+        q1 = q1 - co[c] * (double)vv[c].y;
+        x0 = x0 - co[s + c] * (double)vv[c].x;
+        x1 = x1 - co[s + c] * (double)vv[c].y;
+      }
+    q0 = q0 / beta;
+    q1 = q1 / beta;
+    V2 qv;
+    qv.x = (VT)q0;
+    qv.y = (VT)q1;
+    *reinterpret_cast<V2 *>(V + (long)s * ldv + i) = qv;
+    double2 un;
+    un.x = (x0 - q0 * e) / beta;
+    un.y = (x1 - q1 * e) / beta;
+    *reinterpret_cast<double2 *>(u + i) = un;
+  }
+  if ((n & 1) && tid == 0) {
+    const long i = n - 1;
     double q = u[i], x = z[i];
 #pragma unroll
     for (int c = 0; c < KM; c++)
       if (c < s) {
-        q = q - co[c] * vv[c];
-        x = x - co[s + c] * vv[c];
+        const double vc = (double)V[(long)c * ldv + i];
+        q = q - co[c] * vc;
+        x = x - co[s + c] * vc;
       }
     q = q / beta;
     V[(long)s * ldv + i] = (VT)q;
