@@ -319,11 +319,11 @@ struct GradLds {
 // kNGCached reads it from a.NGP instead (the matrix depends on the mesh only,
 // so the three give the same bits).  Cached, the gradient terms are staged
 // in two rounds (rho, u, v; then w, p) in 9 rows, so the workgroup's LDS
-// is 26 KB instead of 47 KB and five workgroups share a CU instead of three.
+// is 26 KB instead of 47 KB and six workgroups share a CU instead of three.
 enum { kNGFull = 0, kNGStore = 1, kNGCached = 2 };
 
 template <int MODE>
-__global__ __launch_bounds__(kGrpThreads) void k_nodal_grad(GradArgs a) {
+__global__ __launch_bounds__(kGrpThreads, 6) void k_nodal_grad(GradArgs a) {  // six waves per SIMD: 76 VGPRs
   constexpr bool kCached = MODE == kNGCached;
   constexpr int kRows = kCached ? 9 : 16, kB = kCached ? 0 : 9;  // rows; first bounds row
   __shared__ GradLds<kRows, kCached ? 15 : 28> L;

@@ -126,6 +126,30 @@ def test_gmres_on_random_system_matches_direct_solve(solver):
     A.close()
 
 
+@pytest.mark.parametrize("m,b,precision", [(201, 3, 0), (201, 3, 1), (77, 5, 0), (77, 5, 1), (150, 4, 1)])
+def test_gmres_odd_and_even_lengths_match_direct_solve(solver, m, b, precision):
+    """Odd n = m b: the Gram-Schmidt passes take two entries per thread, the
+    last entry of an odd n goes to one thread, and the basis columns are n
+    rounded up to even apart.  FP64 and FP32-inner (FP32 basis) against a
+    direct solve, both to the FP64 residual bound."""
+    print(f"[gpu] bcsr gmres m={m} b={b} precision={precision}", flush=True)
+    row, col, v = random_bcsr(m, b, 91 + m + b, diag_shift=3.0)
+    rhs = np.random.default_rng(m + b).standard_normal(m * b)
+    tol = 1e-11 * np.linalg.norm(rhs)
+    A = Bcsr(solver, m, b, row, col, v)
+    x, conv, st = A.gmres_diag(rhs, tol, 20, precision)
+    assert conv, st
+    D = np.zeros((m * b, m * b))
+    rr = row - 1
+    for i in range(m):
+        for j in range(rr[i], rr[i + 1]):
+            c = col[j] - 1
+            D[i * b:(i + 1) * b, c * b:(c + 1) * b] = v[j * b * b:(j + 1) * b * b].reshape(b, b).T
+    assert np.linalg.norm(rhs - D @ x) < tol
+    assert np.allclose(x, np.linalg.solve(D, rhs), rtol=1e-8, atol=1e-10)
+    A.close()
+
+
 def test_gmres_reports_non_convergence(solver):
     print("[gpu] bcsr gmres not converged", flush=True)
     s = lm.build_system(32)
