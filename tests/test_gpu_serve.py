@@ -28,8 +28,9 @@ def test_batched_results_equal_solves_alone(requests, mode):
     print(f"[gpu] serve equality mode={mode}", flush=True)
     alone, _ = S.serve_plain(requests, keep=True, realtime=False)
     # capacity 3 and 6-step chunks: instances are admitted while others run
-    p = S.Params(chunk=6, max_batch=3, use_graph=mode == "graph", persistent=mode.startswith("persistent"),
-                 wgs_per_cu=1 if mode == "persistent1" else 0)
+    pers = mode.startswith("persistent")
+    p = S.Params(chunk=6, max_batch=3, use_graph=mode == "graph", persistent=pers,
+                 wgs_per_cu=1 if mode == "persistent1" else 0, connectivity="host" if pers else "device")
     mixed, _ = S.serve(requests, p, keep=True, realtime=False)
     a = {r.rid: r.sol for r in alone}
     m = {r.rid: r.sol for r in mixed}
@@ -123,10 +124,12 @@ def test_pooled_context_reused_for_a_different_mesh(requests, mode):
     batch used before, so the batch must rebuild its tables and graph."""
     print(f"[gpu] serve pooled context reuse mode={mode}", flush=True)
     alone, _ = S.serve_plain(requests, keep=True, realtime=False)
-    p = S.Params(chunk=6, max_batch=1, prefetch=0, first=1, use_graph=mode == 1, persistent=mode == 2)
+    p = S.Params(chunk=6, max_batch=1, prefetch=0, first=1, use_graph=mode == 1, persistent=mode == 2,
+                 connectivity="host" if mode == 2 else "device")
     one, _ = S.serve(requests, p, keep=True, realtime=False)
     a = {r.rid: r.sol for r in alone}
     m = {r.rid: r.sol for r in one}
     assert set(a) == set(m)
     for rid in a:
         assert np.array_equal(a[rid], m[rid]), rid
+

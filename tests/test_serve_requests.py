@@ -4,6 +4,7 @@ deterministic, quad/tri about 50/50, cell counts log-uniform in
 requested rate; client-side preparation generates each mesh (and, when
 asked, builds its connectivity)."""
 import numpy as np
+import pytest
 
 from subfv_amd import serve as S
 
@@ -65,3 +66,10 @@ def test_a_request_beyond_the_budget_is_refused():
     reqs[0].cells = 64
     with pytest.raises(S.SubfvError, match="HBM budget"):
         S.serve(reqs, S.Params(budget=1024), realtime=False, pool=[])
+
+
+def test_persistent_kernel_refuses_device_connectivity():
+    """The persistent kernel is served with host-built connectivity only:
+    refused before any device work."""
+    with pytest.raises(ValueError, match="host only"):
+        S.serve([], S.Params(persistent=True, connectivity="device"), realtime=False)
