@@ -1764,7 +1764,6 @@ struct subfv_bcsr {
   float *vr32 = nullptr;  // and FP32; formed by the first block-Jacobi GMRES after the values change
   bool vr_ok = false;
   double *dch = nullptr;  // DCGS2 scalars (kDchSize)
-  unsigned int *ticket = nullptr;
   // host copies (0-based) for add_block and the sweep schedules
   std::vector<int> h_row, h_col, h_diag;
   // bcsr_lusgs: rows grouped by dependency level of the lower triangle
@@ -2024,13 +2023,11 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
   if (!r) r = balloc(A, &A->part2, kRedBlocks);
   if (!r) r = balloc(A, &A->dcp, (size_t)(2 * kGsMax + 2) * kGsBlocks);
   if (!r) r = balloc(A, &A->dch, (size_t)kDchSize);
-  if (!r) r = balloc(A, &A->ticket, 1);
   if (!r) r = balloc(A, &A->perr, 1);
   if (r) {
     subfv_bcsr_destroy(A);
     return r;
   }
-  (void)hipMemset(A->ticket, 0, sizeof(unsigned int));
   dim3 g(nblk(m, 256)), bl(256);
   hipStream_t s = c->stream;
   switch (b) {
