@@ -395,8 +395,8 @@ constexpr int kDchBeta0 = 2 * (kGsMax + 2), kDchCoef = kDchBeta0 + 1, kDchSize =
 // because A M^-1 V(:, s) = (z - A M^-1 V a) / beta = (z - V H a) / beta.  The
 // orthogonalisation reads the basis twice per step instead of three times.
 
-// The step's coefficients from the partials of k_dc_dot (nbp blocks), one
-// block of kDcCoefThreads: co[0..s) = a, co[s..2s) = b, co[2s] = c,
+// The step's coefficients from the partials of k_dc_dot (kGsBlocks blocks),
+// one block of kDcCoefThreads: co[0..s) = a, co[s..2s) = b, co[2s] = c,
 // co[2s+1] = d; then beta, e, Ha; H's corrected column s-1, the new
 // provisional column hprov_out (hprov_in is only read, so the two
 // alternate), and coef = (a, b, beta, e) for the update pass.  beta_out
