@@ -7,14 +7,15 @@
 ! own driver calls them, and writes what they return (stream files) for
 ! comparison with the oracle.
 !
-! Usage: reference_api_check mesh.msh bcsr.bin
+! Usage: reference_api_check mesh.msh bcsr.bin [periodic]
 !   mesh.msh  msh 2.2 mesh, all walls; init_1drp state (x < 0.5: rho 1, p 1;
 !             else rho 0.125, p 0.1), multi_point, bc_style 2, boundary_2d,
 !             second order with method 4, cfl 0.4
 !   bcsr.bin  int32 m, b, nnz, row(m + 1), col(nnz), then doubles v(b, b, nnz),
 !             x(b m), tol
 ! Writes mesh.bin (the counts, then the vertices, elements and boundary
-! faces of the built mesh), residual.bin (rhs after flux_limiting, rhs_o2,
+! faces of the built mesh; with "periodic" the mesh is built with
+! periodic_mesh = .true. and mesh.bin is the only output), residual.bin (rhs after flux_limiting, rhs_o2,
 ! grad, nodal grad, sol after add_timedisc_and_solve, dt), viscous.bin (a
 ! smooth state, cell viscosities, compute_mu's result, the two builders'
 ! sums into rhs and diag, a warm state and compute_mu's Sutherland
@@ -28,7 +29,7 @@ program reference_api_check
   implicit none
 
   integer, parameter :: DP = c_double
-  character(len=512) :: mfile, bfile
+  character(len=512) :: mfile, bfile, mode
   type(c_ptr) :: ctx
   type(mesh_type) :: mesh
   type(subfv_config) :: cfg
@@ -44,10 +45,11 @@ program reference_api_check
 
   call get_command_argument(1, mfile)
   call get_command_argument(2, bfile)
+  call get_command_argument(3, mode)
   names = ""
   call read_mesh_msh_2(mesh, "", mfile, 0, names)
   msr = 0
-  call build_mesh(mesh, 1, msr, .true., .true.)
+  call build_mesh(mesh, 1, msr, .true., .true., periodic_mesh=(trim(mode) == "periodic"))
   call compute_geometry_mesh(mesh, .true., .true.)
   ctx = subfv_ref_context()
   if (.not. c_associated(ctx)) error stop "no device context after build_mesh"
@@ -63,6 +65,10 @@ program reference_api_check
   write (u) ((int(mesh%boundary_face(i)%vert(j), c_int32_t), j = 1, mesh%boundary_face(i)%n_vert), &
     i = 1, mesh%n_boundary_faces)
   close (u)
+  if (trim(mode) == "periodic") then
+    call subfv_ref_release()
+    stop
+  end if
   cfg%scheme_id = 1
   cfg%bc_style = 2
   cfg%boundary_2d = 1

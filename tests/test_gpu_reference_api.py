@@ -102,6 +102,19 @@ def test_reference_signature_procedures(tmp_path):
     assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
+def test_reference_signature_mesh_setup_periodic(tmp_path):
+    """read_mesh_msh_2 + build_mesh(..., periodic_mesh=.true.) +
+    compute_geometry_mesh: the mesh_type the reference-signature procedures
+    fill on a periodic quad mesh against the oracle's periodic build."""
+    print("[gpu] reference-signature mesh set-up, periodic", flush=True)
+    mesh = mg.quad_mesh(10, x0=-0.5, y0=-0.5)
+    mg.write_msh(mesh, str(tmp_path / "mesh.msh"))
+    out = subprocess.run([EXE, "mesh.msh", "none", "periodic"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    _check_mesh(tmp_path / "mesh.bin", mesh, O.OracleMesh(mesh, periodic=True))
+
+
 def _take(buf, pos, n, dt):
     size = np.dtype(dt).itemsize * n
     return buf[pos:pos + size].view(dt).copy(), pos + size
