@@ -1760,6 +1760,12 @@ struct subfv_bcsr {
   double *V = nullptr, *w = nullptr, *r = nullptr, *x = nullptr, *rhs = nullptr, *tmp = nullptr, *zp = nullptr;
   double *H = nullptr, *scal = nullptr, *z = nullptr, *part = nullptr, *part2 = nullptr;
   double *dcp = nullptr;  // DCGS2 partials ((2 kGsMax + 2) x kGsBlocks)
+  // one DCGS2 Arnoldi cycle as a graph, replayed every restart; the restart
+  // length, preconditioner, precision and basis it was captured for
+  hipGraph_t cyc_graph = nullptr;
+  hipGraphExec_t cyc_exec = nullptr;
+  int cyc_mr = -1, cyc_kind = -1, cyc_fp32 = -1;
+  const void *cyc_v = nullptr;
   double *vr = nullptr;   // A D^-1 (block-Jacobi right preconditioning folded in), FP64
   float *vr32 = nullptr;  // and FP32; formed by the first block-Jacobi GMRES after the values change
   bool vr_ok = false;
@@ -2055,6 +2061,8 @@ int subfv_bcsr_create(subfv_ctx *c, subfv_bcsr **out, int32_t m, int32_t b, int3
 void subfv_bcsr_destroy(subfv_bcsr *A) {
   if (!A) return;
   if (A->ctx && A->ctx->stream) (void)hipStreamSynchronize(A->ctx->stream);
+  if (A->cyc_exec) (void)hipGraphExecDestroy(A->cyc_exec);
+  if (A->cyc_graph) (void)hipGraphDestroy(A->cyc_graph);
   for (void *p : A->allocs) (void)hipFree(p);
   delete A;
 }
@@ -2177,30 +2185,59 @@ int gmres_impl(subfv_bcsr *A, int kind, double *x_out, const double *rhs, double
       y[1] = rho;
       HIPCHK(c, hipMemcpyAsync(A->scal, &rho, sizeof(double), hipMemcpyHostToDevice, s));
       if (dc) {
-        // u = r / rho; step st: z = A M^-1 u, then V(:, st) and the next u
-        hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, du, A->scal, 0, (double *)nullptr);
         if (kind == 1 && (r = ensure_rprec(A))) return r;
-        for (int st = 0; st < mr; st++) {
-          if (kind == 0) {
-            if ((r = spmv(A, fp32, du, dz))) return r;
-          } else if (kind == 1) {
-            // A M^-1 u as one product with A D^-1 (A->vr)
-            if ((r = spmv_v(A, fp32, A->vr, A->vr32, du, dz))) return r;
-          } else {
-            if ((r = precond(A, kind, fp32, du, A->tmp))) return r;
-            if ((r = spmv(A, fp32, A->tmp, dz))) return r;
+        // the cycle's launches: u = r / rho (rho in A->scal); step st: z =
+        // A M^-1 u, then V(:, st) and the next u; the last column's second pass
+        auto cycle = [&]() -> int {
+          int rr;
+          hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, du, A->scal, 0, (double *)nullptr);
+          for (int st = 0; st < mr; st++) {
+            if (kind == 0) {
+              if ((rr = spmv(A, fp32, du, dz))) return rr;
+            } else if (kind == 1) {
+              // A M^-1 u as one product with A D^-1 (A->vr)
+              if ((rr = spmv_v(A, fp32, A->vr, A->vr32, du, dz))) return rr;
+            } else {
+              if ((rr = precond(A, kind, fp32, du, A->tmp))) return rr;
+              if ((rr = spmv(A, fp32, A->tmp, dz))) return rr;
+            }
+            if (vf32)
+              dc_step<float>(A, st, Vf, ldv, du, dz, Hb, ldh, false);
+            else
+              dc_step<double>(A, st, Vb, ldv, du, dz, Hb, ldh, false);
           }
-          nspmv++;
           if (vf32)
-            dc_step<float>(A, st, Vf, ldv, du, dz, Hb, ldh, false);
+            dc_step<float>(A, mr, Vf, ldv, du, dz, Hb, ldh, true);
           else
-            dc_step<double>(A, st, Vb, ldv, du, dz, Hb, ldh, false);
+            dc_step<double>(A, mr, Vb, ldv, du, dz, Hb, ldh, true);
+          return 0;
+        };
+        // captured once per restart length, preconditioner, precision and
+        // basis, then replayed every restart (no memory operations inside)
+        if (!(A->cyc_exec && A->cyc_mr == mr && A->cyc_kind == kind && A->cyc_fp32 == (int)fp32 &&
+              A->cyc_v == (const void *)Vb)) {
+          if (A->cyc_exec) (void)hipGraphExecDestroy(A->cyc_exec);
+          if (A->cyc_graph) (void)hipGraphDestroy(A->cyc_graph);
+          A->cyc_exec = nullptr;
+          A->cyc_graph = nullptr;
+          A->cyc_mr = -1;
+          HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          const int rc = cycle();
+          hipGraph_t g = nullptr;
+          const hipError_t ec = hipStreamEndCapture(s, &g);
+          if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(c, SUBFV_E_DEVICE, std::string("GMRES cycle capture: ") + hipGetErrorString(ec));
+          }
+          A->cyc_graph = g;
+          HIPCHK(c, hipGraphInstantiate(&A->cyc_exec, g, nullptr, nullptr, 0));
+          A->cyc_mr = mr;
+          A->cyc_kind = kind;
+          A->cyc_fp32 = (int)fp32;
+          A->cyc_v = (const void *)Vb;
         }
-        // the last column's second pass: h(1:mr, mr) and h(mr + 1, mr)
-        if (vf32)
-          dc_step<float>(A, mr, Vf, ldv, du, dz, Hb, ldh, true);
-        else
-          dc_step<double>(A, mr, Vb, ldv, du, dz, Hb, ldh, true);
+        HIPCHK(c, hipGraphLaunch(A->cyc_exec, s));
+        nspmv += mr;
       } else {
         hipLaunchKernelGGL(k_scale_div, vg, vb, 0, s, n, A->r, Vc(1), A->scal, 0, (double *)nullptr);
       }
@@ -2396,6 +2433,7 @@ int subfv_bcsr_set_colors(subfv_bcsr *A, const int32_t *color, int32_t *n_sets) 
   HIPCHK(c, hipMemcpy(A->color, col1.data(), m * sizeof(int), hipMemcpyHostToDevice));
   A->set_off = off;
   A->nsets = ns;
+  A->cyc_mr = -1;  // a captured GMRES cycle holds the old colour sets
   if (n_sets) *n_sets = ns;
   return 0;
 }
