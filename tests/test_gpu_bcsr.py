@@ -150,6 +150,29 @@ def test_gmres_odd_and_even_lengths_match_direct_solve(solver, m, b, precision):
     A.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gmres_cycle_graph_replays_are_deterministic(solver, precision):
+    """Each Arnoldi cycle is a captured graph replayed at every restart: the
+    same solve twice (the second replays the first's graph), a solve with
+    another restart length (recaptured), then the first again (recaptured
+    back) give the same bits, with eager host work between the replays."""
+    print(f"[gpu] bcsr gmres graph replays precision={precision}", flush=True)
+    m, b = 300, 4
+    row, col, v = random_bcsr(m, b, 55, diag_shift=0.6)
+    rhs = np.random.default_rng(56).standard_normal(m * b)
+    tol = 1e-10 * np.linalg.norm(rhs)
+    A = Bcsr(solver, m, b, row, col, v)
+    x1, c1, st1 = A.gmres_diag(rhs, tol, 20, precision)
+    assert c1 and st1["spmv"] > 20  # several cycles: the graph is replayed
+    x2, c2, _ = A.gmres_diag(rhs, tol, 20, precision)
+    x3, c3, _ = A.gmres_diag(rhs, tol, 12, precision)
+    x4, c4, _ = A.gmres_diag(rhs, tol, 20, precision)
+    assert c2 and c3 and c4
+    assert np.array_equal(x1, x2) and np.array_equal(x1, x4)
+    assert np.linalg.norm(x3 - x1) <= 1e-7 * np.linalg.norm(x1)
+    A.close()
+
+
 def test_gmres_reports_non_convergence(solver):
     print("[gpu] bcsr gmres not converged", flush=True)
     s = lm.build_system(32)
