@@ -219,8 +219,8 @@ int subfv_set_diffusion(subfv_ctx *ctx, int32_t activate, double mu_p, double cv
 int subfv_set_sutherland(subfv_ctx *ctx, int32_t activate, double mu0, double T0);
 /* compute_mu (ns_vectorial_diffusion_module.F90:322-347): mu_arr(n_elems) and
  * mu0corr for the state sol(5, n_elems): Sutherland's law per cell on the
- * device when subfv_set_sutherland activated it, else mu_p of
- * subfv_set_diffusion (mu0corr = mu_p).  The state becomes the context's. */
+ * device when subfv_set_sutherland activated it (sol then becomes the
+ * context's state), else mu_p of subfv_set_diffusion (mu0corr = mu_p). */
 int subfv_compute_mu(subfv_ctx *ctx, const double *sol, double *mu_arr, double *mu0corr);
 /* build_momentum_diff_mat_into_global_mat_diag (ns_vectorial_diffusion_module.F90:200-320,
  * parts = 1) and build_diff_mat_into_global_mat_diag (ns_diffusion_module.F90:112-184,
@@ -228,7 +228,8 @@ int subfv_compute_mu(subfv_ctx *ctx, const double *sol, double *mu_arr, double *
  * mu_arr(n_elems) and mu0corr: their contributions are added to the caller's
  * rhs(5, n_elems) (rows 2-5) and diag(5, 5, n_elems), as the reference's
  * builders add them.  The nodal systems are formed on the device for the given
- * viscosities; the other viscous parameters come from subfv_set_diffusion. */
+ * viscosities; the other viscous parameters come from subfv_set_diffusion;
+ * sol becomes the context's state. */
 int subfv_add_diffusion(subfv_ctx *ctx, const double *sol, const double *mu_arr, double mu0corr, int32_t parts,
                         double *rhs, double *diag);
 

@@ -68,10 +68,16 @@ module subfv_reference_api
 contains
 
   ! the device context the procedures run on (a context build_mesh made
-  ! before is destroyed)
+  ! before is destroyed, unless it is c itself: the caller then owns it)
   subroutine subfv_ref_use_context(c)
     type(c_ptr), intent(in) :: c
-    if (own_ctx) call subfv_ref_release()
+    if (own_ctx) then
+      if (c_associated(ctx, c)) then
+        own_ctx = .false.
+      else
+        call subfv_ref_release()
+      end if
+    end if
     ctx = c
   end subroutine
 
