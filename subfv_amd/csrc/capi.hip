@@ -796,9 +796,8 @@ __global__ __launch_bounds__(kCellThreads) void k_mu_min(const double *blk, int 
   if (threadIdx.x == 0) *out = v;
 }
 
-// rdiff and diag of the current state (build_momentum_diff_mat_into_global_mat_diag
-// and build_diff_mat_into_global_mat_diag)
-// rdiff and diag of the state U.  mu_cell / mu_min: given cell viscosities
+// rdiff and diag of the state U (build_momentum_diff_mat_into_global_mat_diag
+// and build_diff_mat_into_global_mat_diag).  mu_cell / mu_min: given cell viscosities
 // and their mu0corr (device), for which the nodal systems are formed anew
 // (the constant-viscosity blocks are then invalidated); parts: 1 momentum,
 // 2 heat, 3 both.
@@ -843,10 +842,14 @@ int launch_diffusion(subfv_ctx *c, const double *U, const double *mu_cell = null
   {
     Prof pa(c, "diff_apply");
     if (c->diff_large)
-      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSubL, 1>), dim3(8 * a.vper_xcd), dim3(64), 0, c->stream, a);
+      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSubL, kDiffMaxSubL, kDiffStage, 1, 7>), dim3(8 * a.vper_xcd),
+                         dim3(64), 0, c->stream, a);
+    else if (c->diff_em <= 3 * kDiffMaxSubC)
+      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSub, kDiffMaxSubC, kDiffStageC, kDiffWaves, 8>),
+                         dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0, c->stream, a);
     else
-      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSub, kDiffWaves>), dim3(8 * a.vper_xcd), dim3(64 * kDiffWaves), 0,
-                         c->stream, a);
+      hipLaunchKernelGGL((k_diff_apply<kDiffMaxSub, kDiffMaxSub, kDiffStage, kDiffWaves, 7>), dim3(8 * a.vper_xcd),
+                         dim3(64 * kDiffWaves), 0, c->stream, a);
   }
   hipLaunchKernelGGL(k_diff_gather, dim3(8 * a.cper_xcd), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());

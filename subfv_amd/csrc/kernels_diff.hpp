@@ -473,16 +473,23 @@ constexpr int kDiffWaves = 4;
 // b1 (ns^2 + 2 ns), the diagonal 3x3 blocks of S~^T N^-1 S (9 ns); vertices
 // with more sub-elements read them from the block in HBM instead
 constexpr int kDiffStage = 240;
+// compact class of the apply: up to kDiffMaxSub sub-faces and kDiffMaxSubC
+// sub-elements (prism and hex meshes), 4 KB of LDS per vertex, so ten
+// workgroups of four vertices share a CU instead of six
+constexpr int kDiffMaxSubC = 8;
+constexpr int kDiffStageC = kDiffMaxSubC * kDiffMaxSubC + 23 * kDiffMaxSubC;  // 248
 
-template <int MS>
+// MF: sub-faces, ME: sub-elements per vertex; STAGE: doubles of staging
+template <int MF, int ME, int STAGE>
 struct DiffApplyLds {
-  static constexpr int kPc = kDiffStage > 13 * MS ? kDiffStage : 13 * MS;
-  double Ve[3 * MS], Vf[3 * MS], SVe[3 * MS];
-  double th[MS], rho[MS], P[MS][5], vol[MS], pu[MS];
-  double nrm[MS][4];           // unit normal and area of each sub-face of the vertex
-  int le[MS];                  // left cell of each sub-face
-  int cell[MS];                // cell of each slot
-  int ssf[MS][kSlotSrc + 1];   // local sub-faces of each slot's sub-element, count first
+  static constexpr int kStage = STAGE;
+  static constexpr int kPc = STAGE > 13 * ME ? STAGE : 13 * ME;
+  double Ve[3 * ME], Vf[3 * MF], SVe[3 * ME];
+  double th[ME], rho[ME], P[ME][5], vol[ME], pu[ME];
+  double nrm[MF][4];           // unit normal and area of each sub-face of the vertex
+  int le[MF];                  // left cell of each sub-face
+  int cell[ME];                // cell of each slot
+  int ssf[ME][kSlotSrc + 1];   // local sub-faces of each slot's sub-element, count first
   double pc[kPc];              // the slot terms' block data, staged while the row pass streams;
                                // then the stresses (9 ns) and work terms (4 ns)
 };
@@ -598,10 +605,12 @@ SFV_DEV void diff_slot_terms(const DiffArgs &a, LdsT &L, int lane, int ns, int s
 //     SVe = S~^T N^-1 S Ve, coalesced;
 // (c) the slot terms, spread over the lanes by (slot, entry), read only LDS
 //     and the vertex's block the wavefront has just streamed.
-// MS: the size class; WAVES: vertices (wavefronts) per workgroup.
-template <int MS, int WAVES>
-__global__ __launch_bounds__(64 * WAVES, 7) void k_diff_apply(DiffArgs a) {
-  __shared__ DiffApplyLds<MS> lds_all[WAVES];
+// MF, ME, STAGE: the size class (DiffApplyLds); WAVES: vertices
+// (wavefronts) per workgroup; MINW: wavefronts per SIMD the registers allow.
+template <int MF, int ME, int STAGE, int WAVES, int MINW>
+__global__ __launch_bounds__(64 * WAVES, MINW) void k_diff_apply(DiffArgs a) {
+  using Lds = DiffApplyLds<MF, ME, STAGE>;
+  __shared__ Lds lds_all[WAVES];
   const DevMesh &m = a.m;
   // the wavefront's index is uniform: the vertex's offsets come through the
   // scalar cache
@@ -610,7 +619,7 @@ __global__ __launch_bounds__(64 * WAVES, 7) void k_diff_apply(DiffArgs a) {
   if (lb >= a.vblocks) return;
   const long v = (long)lb * WAVES + wib;
   if (v >= m.V) return;  // whole wavefront
-  DiffApplyLds<MS> &L = lds_all[wib];
+  Lds &L = lds_all[wib];
   const int sf0 = m.sf_off[v], nf = m.sf_off[v + 1] - sf0;
   const int se0 = m.se_off[v], ns = m.se_off[v + 1] - se0;
   const int F = 3 * nf, E = 3 * ns;
@@ -650,7 +659,7 @@ __global__ __launch_bounds__(64 * WAVES, 7) void k_diff_apply(DiffArgs a) {
     L.le[l] = m.sf_le[s];
   }
   // stage the slot terms' block data: the loads overlap the row pass below
-  const bool staged = ns * ns + 23 * ns <= kDiffStage;
+  const bool staged = ns * ns + 23 * ns <= Lds::kStage;
   const int o_tail = 12 * ns, o_diag = 12 * ns + ns * ns + 2 * ns;
   if (staged) {
     for (int t = lane; t < 12 * ns; t += 64) L.pc[t] = sb[t];  // sb then blcs, contiguous
