@@ -474,8 +474,9 @@ constexpr int kDiffWaves = 4;
 // with more sub-elements read them from the block in HBM instead
 constexpr int kDiffStage = 240;
 // compact class of the apply: up to kDiffMaxSub sub-faces and kDiffMaxSubC
-// sub-elements (prism and hex meshes), 4 KB of LDS per vertex, so ten
-// workgroups of four vertices share a CU instead of six
+// sub-elements (prism and hex meshes), 4 KB of LDS per vertex, so eight
+// workgroups of four vertices (eight wavefronts per SIMD, the most a SIMD
+// holds) share a CU instead of seven
 constexpr int kDiffMaxSubC = 8;
 constexpr int kDiffStageC = kDiffMaxSubC * kDiffMaxSubC + 23 * kDiffMaxSubC;  // 248
 
