@@ -30,8 +30,9 @@ namespace sfv {
 
 // Two size classes of vertex: up to kDiffMaxSub sub-faces and sub-elements
 // (prism, hex and quad meshes; the setup's matrices in LDS, four vertices per
-// apply workgroup) and up to kDiffMaxSubL (tetrahedral meshes; the setup's
-// matrices in a global workspace, one vertex per apply workgroup).
+// apply workgroup, the apply in a compact LDS layout when no vertex has more
+// than kDiffMaxSubC sub-elements) and up to kDiffMaxSubL (tetrahedral meshes;
+// the setup's matrices in a global workspace, one vertex per apply workgroup).
 constexpr int kDiffMaxSub = 16;             // sub-faces (and sub-elements) per vertex, small class
 constexpr int kDiffMaxF = 3 * kDiffMaxSub;  // 48
 constexpr int kDiffMaxSubL = 64;            // large class
