@@ -699,6 +699,7 @@ __global__ __launch_bounds__(64 * WAVES, MINW) void k_diff_apply(DiffArgs a) {
 // LDS into rdiff (4 per cell) and diag (25 per cell), written by consecutive
 // lanes too.
 constexpr int kDgCells = 14;  // 14 x 18 = 252 of 256 lanes
+constexpr int kDgBatch = 8;   // rows loaded together per lane
 __global__ __launch_bounds__(256) void k_diff_gather(DiffArgs a) {
   __shared__ double s[kDgCells][kDiffSlot];
   const DevMesh &m = a.m;
@@ -709,8 +710,21 @@ __global__ __launch_bounds__(256) void k_diff_gather(DiffArgs a) {
   if (cl < kDgCells && c0 + cl < m.C) {
     const long c = c0 + cl;
     const int b = m.e_voff[c], nv = m.e_voff[c + 1] - b;
+    // kDgBatch slot ids, then their rows' entries (row 0 stands in past the
+    // cell's last vertex), so the loads of a batch are in flight together;
+    // the sum takes the cell's rows in vertex order
     double acc = 0.0;
-    for (int j = 0; j < nv; j++) acc = acc + a.PD[(long)kDiffSlot * m.c_slot[b + j] + k];
+    for (int j0 = 0; j0 < nv; j0 += kDgBatch) {
+      int sl[kDgBatch];
+#pragma unroll
+      for (int u = 0; u < kDgBatch; u++) sl[u] = j0 + u < nv ? m.c_slot[b + j0 + u] : 0;
+      double x[kDgBatch];
+#pragma unroll
+      for (int u = 0; u < kDgBatch; u++) x[u] = a.PD[(long)kDiffSlot * sl[u] + k];
+#pragma unroll
+      for (int u = 0; u < kDgBatch; u++)
+        if (j0 + u < nv) acc = acc + x[u];
+    }
     s[cl][k] = acc;
   }
   __syncthreads();
